@@ -1,0 +1,287 @@
+"""Benchmark: textured-2DGS forward+backward at scene scale on one MI355X.
+
+Workload (the headline configuration): N = 1,000,000 Gaussians, 50x50 RGBA
+float32 textures per Gaussian (40 GB), SH degree 3, one 1297x840 camera (fov
+60 deg) at radius 5 inside the cloud, 16x16 tiles, white background, L1 loss
+to a seeded random target.  A step is one full render-path iteration through
+the public API: projection -> tile intersection + sort -> SH -> textured
+rasterizer forward -> L1 loss -> backward through rasterizer, SH and
+projection (gradients for means, quats, scales, opacities, SH coefficients and
+textures; ``.grad`` is reset to None before each step, as an optimizer's
+zero_grad does).  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0).  ``roofline`` is for the dominant kernel (the
+rasterizer backward), ``cpu_baseline`` times the CPU oracle on a bounded tile
+sample and extrapolates.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fwd+bwd ms/iter at 1M textured 2DGS, 1297x840; % HBM roofline vs plain"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000, help="Gaussians (default: the headline 1M)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-tiles", type=int, default=60, help="tiles in the CPU oracle sample")
+    return ap.parse_args()
+
+
+def make_inputs(n, dev):
+    from textured_gaussians_amd import scenes
+    sc = scenes.config2(n=n, texture_device=dev).to(dev)
+    params = dict(means=sc.means, quats=sc.quats, scales=sc.scales,
+                  opacities_logit=torch.logit(sc.opacities.clamp(1e-6, 1 - 1e-6)), sh=sc.colors, textures=sc.textures)
+    for p in params.values():
+        p.requires_grad_(True)
+    return sc, params
+
+
+def step(tg, sc, params, textured=True):
+    for p in params.values():
+        p.grad = None
+    op = torch.sigmoid(params["opacities_logit"])
+    if textured:
+        out = tg.rasterization_textured_gaussians(
+            params["means"], params["quats"], params["scales"], op, params["sh"], params["textures"], sc.viewmats,
+            sc.Ks, sc.width, sc.height, sh_degree=3, backgrounds=sc.backgrounds)
+    else:
+        out = tg.rasterization_2dgs(params["means"], params["quats"], params["scales"], op, params["sh"], sc.viewmats,
+                                    sc.Ks, sc.width, sc.height, sh_degree=3, backgrounds=sc.backgrounds)
+    loss = (out[0] - sc.target).abs().mean()
+    loss.backward()
+    return out
+
+
+def timed_steps(tg, sc, params, steps, warmup, textured):
+    from textured_gaussians_amd.cuda import _ops
+    for _ in range(warmup):
+        step(tg, sc, params, textured)
+    torch.cuda.synchronize()
+    timer = {}
+    _ops.set_kernel_timer(timer)
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = step(tg, sc, params, textured)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    _ops.set_kernel_timer(None)
+    kt = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in timer.items()}
+    return dt * 1e3, kt, out
+
+
+def algorithmic_bytes(out, params, sc, textured, K=3):
+    """Per-launch algorithmic HBM bytes of the rasterizer forward and backward.
+
+    I = intersections, R = 76 B per intersection record (4 id + 8 mean2d + 36
+    ray transform + 4 opacity + 12 colour + 12 normal), P = pixels,
+    U = distinct (Gaussian, texel) pairs touched (16 B each; counted as the
+    texels with a nonzero gradient, a lower bound of the texels read),
+    V = visible (camera, Gaussian) pairs.
+      forward : I*R + 16U + 4*C*tiles + P*(4K+32)
+      backward: I*R + 16U + P*(4K+36) + 32U + 2V*(68+4K)
+    """
+    meta = out[-1]
+    I = meta["flatten_ids"].numel()
+    C = sc.viewmats.shape[0]
+    P = C * sc.width * sc.height
+    tiles = meta["tile_width"] * meta["tile_height"]
+    V = int((meta["radii"] > 0).sum())
+    U = 0
+    if textured:
+        g = params["textures"].grad
+        U = int((g != 0).any(-1).sum())  # chunked would save memory; the grad is already resident
+    R = 76
+    bf = I * R + 16 * U + 4 * C * tiles + P * (4 * K + 32)
+    bb = I * R + 16 * U + P * (4 * K + 36) + 32 * U + 2 * V * (68 + 4 * K)
+    return dict(I=I, P=P, V=V, U=U, tiles=tiles, bytes_fwd=bf, bytes_bwd=bb)
+
+
+def copy_bandwidth(dev):
+    """Measured device-to-device copy rate (read + write bytes / time)."""
+    n = 1 << 30
+    a = torch.empty(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    del a, b
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(sc, meta_gpu, n_tiles_sample):
+    """The CPU oracle (float32 torch on the host) on a bounded sample: full
+    projection + tile intersection + SH of all Gaussians, and the rasterizer
+    forward+backward on every k-th tile, extrapolated to all tiles by the
+    per-tile work sum(I_tile * 256)."""
+    import numpy as np
+    from oracle import isect as O_isect
+    from oracle.projection import project_2dgs
+    from oracle.raster import rasterize_to_pixels
+    from oracle.sh import spherical_harmonics
+
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    f = torch.float32
+    cpu = lambda t: t.detach().cpu().to(f) if t.is_floating_point() else t.detach().cpu()  # noqa: E731
+    means, quats, scales = cpu(sc.means), cpu(sc.quats), cpu(sc.scales)
+    viewmats, Ks = cpu(sc.viewmats), cpu(sc.Ks)
+    sh = cpu(sc.colors)
+    W, H = sc.width, sc.height
+    # (1) projection + SH, forward and backward, all Gaussians
+    t0 = time.perf_counter()
+    m = means.clone().requires_grad_(True)
+    q = quats.clone().requires_grad_(True)
+    s = scales.clone().requires_grad_(True)
+    shc = sh.clone().requires_grad_(True)
+    radii, m2, dep, rt, nrm = project_2dgs(m, q, s, viewmats, Ks, W, H)
+    dirs = m[None] - torch.inverse(viewmats)[:, None, :3, 3]
+    col = torch.clamp_min(spherical_harmonics(3, dirs, shc[None], radii > 0) + 0.5, 0.0)
+    ((m2.sum() + rt.sum() + nrm.sum() + col.sum())).backward()
+    t_proj = time.perf_counter() - t0
+    # (2) tile intersection + sort + offsets (numpy), all Gaussians
+    t0 = time.perf_counter()
+    tw, th = math.ceil(W / 16), math.ceil(H / 16)
+    _, ids, flat = O_isect.isect_tiles(m2.detach().numpy(), radii.numpy(), dep.detach().numpy(), 16, tw, th)
+    offs = O_isect.isect_offset_encode(ids, 1, tw, th)
+    t_isect = time.perf_counter() - t0
+    # (3) rasterizer fwd+bwd on a tile sample, on the GPU's projected inputs
+    offs_flat = offs.reshape(-1).astype(np.int64)
+    I = ids.size
+    ends = np.append(offs_flat[1:], I)
+    per_tile = (ends - offs_flat).clip(min=0)
+    n_t = offs_flat.size
+    stride = max(1, n_t // n_tiles_sample)
+    sample = list(range(stride // 2, n_t, stride))[:n_tiles_sample]
+    # compact sub-problem: only the Gaussians the sampled tiles reference
+    sel = np.concatenate([flat[offs_flat[t]:ends[t]] for t in sample]) if sample else np.zeros(0, np.int32)
+    uniq, inv = np.unique(sel, return_inverse=True)
+    Nloc = max(1, uniq.size)
+    ut = torch.from_numpy(uniq.astype(np.int64))
+    tex = sc.textures[ut.to(sc.textures.device)].detach().cpu().to(f).requires_grad_(True)
+    mm2 = cpu(meta_gpu["means2d"])[0, ut][None].requires_grad_(True)
+    rrt = cpu(meta_gpu["ray_transforms"])[0, ut][None].requires_grad_(True)
+    nn_ = cpu(meta_gpu["normals"])[0, ut][None].requires_grad_(True)
+    cc = col.detach()[0, ut][None].clone().requires_grad_(True)
+    oo = torch.sigmoid(torch.randn(1, Nloc)).requires_grad_(True)
+    # each sampled tile gets its [start, end) in the compact list
+    new_flat = torch.from_numpy(inv.astype(np.int32))
+    tiles_list, o = [], 0
+    for t in sample:
+        tiles_list.append((0, t // tw, t % tw, o, o + int(per_tile[t])))
+        o += int(per_tile[t])
+    loc_offs = torch.zeros(1, th, tw, dtype=torch.int32)
+    t0 = time.perf_counter()
+    outr = rasterize_to_pixels(mm2, rrt, cc, oo, nn_, W, H, 16, loc_offs, new_flat, textures=tex,
+                               backgrounds=cpu(sc.backgrounds), uv_grad="none", distort_grad="reference",
+                               tiles=tiles_list)
+    tgt = cpu(sc.target)
+    loss = (outr["render_colors"] - tgt).abs().sum() / (H * W * 3)
+    loss.backward()
+    t_sample = time.perf_counter() - t0
+    work_all = float((per_tile * 256).sum())
+    work_smp = float((per_tile[sample] * 256).sum())
+    t_raster = t_sample * (work_all / max(work_smp, 1.0))
+    total = t_proj + t_isect + t_raster
+    try:
+        model = [l for l in open("/proc/cpuinfo") if l.startswith("model name")][0].split(":")[1].strip()
+    except Exception:
+        model = platform.processor()
+    return dict(value=round(total * 1e3, 1), unit="ms/iter", cores=torch.get_num_threads(), kind="port",
+                sample=(f"extrapolated: full projection+SH fwd/bwd ({t_proj:.1f}s) and tile intersection "
+                        f"({t_isect:.1f}s) of all {means.shape[0]} Gaussians, rasterizer fwd+bwd on {len(sample)} "
+                        f"of {n_t} tiles "
+                        f"({t_sample:.1f}s) scaled by sum(I_tile*256); fp32 torch oracle; cpu: {model}"))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import textured_gaussians_amd as tg
+    from textured_gaussians_amd.cuda._backend import lib
+    lib()
+    dev = torch.device("cuda:0")
+    torch.manual_seed(0)
+    sc, params = make_inputs(args.n, dev)
+    torch.cuda.synchronize()
+    print("[bench] inputs resident; timing textured steps", file=sys.stderr, flush=True)
+
+    ms, kt, out = timed_steps(tg, sc, params, args.steps, args.warmup, True)
+    ab = algorithmic_bytes(out, params, sc, True)
+    t_bwd = kt.get("raster_bwd", float("nan"))
+    t_fwd = kt.get("raster_fwd", float("nan"))
+    achieved_bwd = ab["bytes_bwd"] / (t_bwd * 1e-3) / 1e9
+    achieved_fr = (ab["bytes_fwd"] + ab["bytes_bwd"]) / ((t_fwd + t_bwd) * 1e-3) / 1e9
+    del out
+    plain = None
+    print(f"[bench] textured {ms:.2f} ms/iter; timing plain 2DGS", file=sys.stderr, flush=True)
+    if not args.no_plain:
+        pms, pkt, pout = timed_steps(tg, sc, params, max(3, args.steps // 2), 1, False)
+        pab = algorithmic_bytes(pout, params, sc, False)
+        pt = pkt.get("raster_fwd", float("nan")) + pkt.get("raster_bwd", float("nan"))
+        p_ach = (pab["bytes_fwd"] + pab["bytes_bwd"]) / (pt * 1e-3) / 1e9
+        plain = dict(ms_per_iter=round(pms, 3), raster_fwd_ms=round(pkt.get("raster_fwd", 0), 3),
+                     raster_bwd_ms=round(pkt.get("raster_bwd", 0), 3), raster_hbm_frac=round(p_ach / HBM_PEAK_GBS, 4))
+        del pout
+    copy_gbs = copy_bandwidth(dev)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "raster_bwd_traffic_config2.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    line = {
+        "metric": METRIC, "value": round(ms, 3), "unit": "ms/iter", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene of the headline workload)",
+        "config": {"workload": "textured-2DGS fwd+bwd, N=1M, 50x50 RGBA f32 textures, SH3, 1297x840, 1 camera",
+                   "n_gaussians": args.n, "width": sc.width, "height": sc.height, "tile": 16,
+                   "n_isects": ab["I"], "visible": ab["V"], "texels_touched": ab["U"]},
+        "mpix_per_s": round(ab["P"] / (ms * 1e-3) / 1e6, 2),
+        "raster_ms": {"fwd": round(t_fwd, 3), "bwd": round(t_bwd, 3)},
+        "raster_hbm_frac_fwd_bwd": round(achieved_fr / HBM_PEAK_GBS, 4),
+        "plain_2dgs": plain,
+        "copy_bandwidth_GBs": round(copy_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved_bwd, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_bwd / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "raster_bwd_kernel<float,3,true>", "algorithmic_bytes": ab["bytes_bwd"]},
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] CPU oracle baseline (bounded sample)", file=sys.stderr, flush=True)
+        with torch.no_grad():
+            radii, m2, dep, rt, nrm = tg.fully_fused_projection_2dgs(sc.means, sc.quats, sc.scales, sc.viewmats,
+                                                                     sc.Ks, sc.width, sc.height)
+        meta = dict(means2d=m2, ray_transforms=rt, normals=nrm)
+        try:
+            line["cpu_baseline"] = cpu_baseline(sc, meta, args.cpu_tiles)
+        except MemoryError as e:  # pragma: no cover
+            line["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

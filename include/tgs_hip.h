@@ -1,0 +1,196 @@
+/*
+ * tgs_hip.h - C ABI of libtgs_hip.so, the MI355X (gfx950) kernels of the
+ * textured 2D-Gaussian-splatting render path.
+ *
+ * Conventions
+ *  - Every pointer is a device pointer unless its name ends in _host.
+ *  - Sizes are int64_t; floating-point scalars are passed as double and
+ *    converted to the working precision selected by `dtype`.
+ *  - `dtype` selects the arithmetic of the floating-point tensors:
+ *    TGS_F32 (float) or TGS_F64 (double).  Integer tensors are fixed-width.
+ *  - The library never allocates or frees device memory.  Outputs and
+ *    scratch ("temp") buffers are owned by the caller; outputs documented as
+ *    "accumulated" must be zero-filled by the caller before the call.
+ *  - All work is enqueued on `stream` (a hipStream_t; NULL = legacy default
+ *    stream).  No entry point synchronises the device.
+ *  - Return value: TGS_OK (0) or a nonzero error code; tgs_last_error()
+ *    then returns a message (thread-local, valid until the next call).
+ *
+ * Each entry point names the reference operator it replaces: the pybind op
+ * in textured_gaussians/cuda/csrc/ext.cpp and its declaration in
+ * textured_gaussians/cuda/csrc/bindings.h.
+ */
+#ifndef TGS_HIP_H
+#define TGS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *tgs_stream_t;
+
+enum { TGS_F32 = 0, TGS_F64 = 1 };
+enum { TGS_OK = 0, TGS_ERR_ARG = 1, TGS_ERR_LAUNCH = 2, TGS_ERR_UNSUPPORTED = 3 };
+
+/* Gradient rule for the bilinear texture lookup w.r.t. the splat coordinate.
+ * TGS_UV_NONE   - no d(texture sample)/ds term: the reference "bilinear"
+ *                 backward (rasterize_to_pixels_textured_gaussians_2dgs_bwd.cu).
+ * TGS_UV_DOUBLE - du/ds = (res-1)/range: the reference "bilinear_bwd2"
+ *                 backward (rasterize_to_pixels_textured_gaussians_2dgs_bwd2.cu:611-676).
+ * TGS_UV_EXACT  - du/ds = (res-1)/(2 range): the true derivative.          */
+enum { TGS_UV_NONE = 0, TGS_UV_EXACT = 1, TGS_UV_DOUBLE = 2 };
+
+/* Projection VJP rule.
+ * TGS_VJP_REFERENCE      - fully_fused_projection_bwd_2dgs (utils.cuh:659-724)
+ * TGS_VJP_REFERENCE_NORM - fully_fused_projection_bwd2_2dgs (utils.cuh:726-791)
+ * TGS_VJP_EXACT          - the true derivative of the forward.             */
+enum { TGS_VJP_REFERENCE = 0, TGS_VJP_REFERENCE_NORM = 1, TGS_VJP_EXACT = 2 };
+
+const char *tgs_last_error(void);
+int tgs_abi_version(void);
+
+/* ---------------------------------------------------------------- projection
+ * Replaces `fully_fused_projection_fwd_2dgs` (ext.cpp:53-55, bindings.h:282-293).
+ * means[N,3] quats[N,4] scales[N,3] viewmats[C,4,4] Ks[C,3,3] ->
+ * radii int32[C,N], means2d[C,N,2], depths[C,N], ray_transforms[C,N,3,3],
+ * normals[C,N,3].  Culled entries get radii 0 and zeros elsewhere.        */
+int tgs_projection_2dgs_fwd(int dtype, int64_t C, int64_t N, const void *means,
+                            const void *quats, const void *scales, const void *viewmats,
+                            const void *Ks, int32_t width, int32_t height,
+                            double near_plane, double far_plane, double radius_clip,
+                            int32_t *radii, void *means2d, void *depths,
+                            void *ray_transforms, void *normals, tgs_stream_t stream);
+
+/* Replaces `fully_fused_projection_bwd_2dgs` and `fully_fused_projection_bwd2_2dgs`
+ * (ext.cpp:56-61, bindings.h:295-330).  v_means[N,3], v_quats[N,4],
+ * v_scales[N,3] are accumulated.  v_viewmats[C,4,4] (may be NULL) is
+ * accumulated in TGS_VJP_EXACT mode and left untouched otherwise (the
+ * reference never writes it).                                              */
+int tgs_projection_2dgs_bwd(int dtype, int64_t C, int64_t N, const void *means,
+                            const void *quats, const void *scales, const void *viewmats,
+                            const void *Ks, const int32_t *radii,
+                            const void *ray_transforms, const void *v_means2d,
+                            const void *v_depths, const void *v_normals,
+                            const void *v_ray_transforms, int vjp_mode, void *v_means,
+                            void *v_quats, void *v_scales, void *v_viewmats,
+                            tgs_stream_t stream);
+
+/* ------------------------------------------------------- tile intersection
+ * Replaces `isect_tiles` (ext.cpp:32, bindings.h:110-121) in three phases:
+ * count -> (caller reads cum_tiles_per_gauss[C*N-1] = n_isects) -> fill -> sort.
+ * means2d f32[C,N,2], radii i32[C,N], depths f32[C,N] (unpacked layout).    */
+size_t tgs_isect_count_temp_bytes(int64_t n_elems);
+int tgs_isect_tiles_count(int64_t C, int64_t N, const float *means2d, const int32_t *radii,
+                          int32_t tile_size, int32_t tile_width, int32_t tile_height,
+                          int32_t *tiles_per_gauss, int64_t *cum_tiles_per_gauss,
+                          void *temp, size_t temp_bytes, tgs_stream_t stream);
+/* keys: cam << (32+tile_bits) | tile << 32 | int32 bits of depth (sign-extended) */
+int tgs_isect_tiles_fill(int64_t C, int64_t N, const float *means2d, const int32_t *radii,
+                         const float *depths, const int64_t *cum_tiles_per_gauss,
+                         int32_t tile_size, int32_t tile_width, int32_t tile_height,
+                         int64_t *isect_ids, int32_t *flatten_ids, tgs_stream_t stream);
+/* Stable LSD radix sort of (key, value) pairs on key bits [0, end_bit)
+ * (CUB DeviceRadixSort::SortPairs with DoubleBuffer, isect_tiles.cu:252-287).
+ * The result is in (keys, values) if *selector_host == 0 after the call,
+ * else in (keys_alt, values_alt).  The selector is known on the host.     */
+size_t tgs_sort_pairs_temp_bytes(int64_t n);
+int tgs_sort_pairs(int64_t n, int32_t end_bit, int64_t *keys, int32_t *values,
+                   int64_t *keys_alt, int32_t *values_alt, void *temp, size_t temp_bytes,
+                   int32_t *selector_host, tgs_stream_t stream);
+/* Replaces `isect_offset_encode` (ext.cpp:33, bindings.h:123-127).
+ * offsets i32[C, tile_height, tile_width]; zero-filled when n_isects == 0. */
+int tgs_isect_offset_encode(int64_t n_isects, const int64_t *isect_ids, int64_t C,
+                            int32_t tile_width, int32_t tile_height, int32_t *offsets,
+                            tgs_stream_t stream);
+
+/* ------------------------------------------------------ spherical harmonics
+ * Replaces `compute_sh_fwd` / `compute_sh_bwd` (ext.cpp:11-12, bindings.h:194-206).
+ * n = number of directions; coeffs[n,K,3]; masks uint8[n] or NULL.
+ * Masked-out colours are written as 0.  v_coeffs[n,K,3] is written (not
+ * accumulated; zero for masked rows and bands above `degree`); v_dirs (may
+ * be NULL) is accumulated.                                                 */
+int tgs_sh_fwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs,
+               const void *coeffs, const uint8_t *masks, void *colors, tgs_stream_t stream);
+int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs,
+               const void *coeffs, const uint8_t *masks, const void *v_colors,
+               void *v_coeffs, void *v_dirs, tgs_stream_t stream);
+
+/* --------------------------------------------------------------- rasterizer
+ * Replaces `rasterize_to_pixels_fwd_textured_gaussians` (ext.cpp:86-88,
+ * bindings.h:509-538) when textures != NULL, and `rasterize_to_pixels_fwd_2dgs`
+ * (ext.cpp:76-77) when textures == NULL.  K = colour channels, 1..4.
+ * means2d[C,N,2] ray_transforms[C,N,3,3] colors[C,N,K] opacities[C,N]
+ * normals[C,N,3] textures[N,tex_h,tex_w,4] backgrounds[C,K] (may be NULL)
+ * masks uint8[C,tile_h,tile_w] (may be NULL).
+ * Outputs (written): render_colors[C,H,W,K] render_alphas[C,H,W]
+ * render_normals[C,H,W,3] render_distort[C,H,W] render_median[C,H,W]
+ * last_ids i32[C,H,W] median_ids i32[C,H,W].
+ * Outputs (accumulated): contrib_sum[N], contrib_count[N].
+ * render_accum_depth[C,H,W] (may be NULL; no reference counterpart): the
+ * forward's distortion accumulator sum_i w_i colors[i, K-1], for the exact
+ * distortion gradient.                                                    */
+int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
+                      const void *ray_transforms, const void *colors, const void *opacities,
+                      const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
+                      double texture_range_x, double texture_range_y,
+                      const void *backgrounds, const uint8_t *masks, int32_t width,
+                      int32_t height, int32_t tile_size, int32_t tile_width,
+                      int32_t tile_height, const int32_t *tile_offsets,
+                      const int32_t *flatten_ids, int64_t n_isects,
+                      double gs_contrib_threshold, double g_weight, void *render_colors,
+                      void *render_alphas, void *render_normals, void *render_distort,
+                      void *render_median, int32_t *last_ids, int32_t *median_ids,
+                      void *contrib_sum, void *contrib_count, void *render_accum_depth,
+                      tgs_stream_t stream);
+
+/* Replaces `rasterize_to_pixels_bwd_textured_gaussians` /
+ * `rasterize_to_pixels_bwd2_textured_gaussians` (ext.cpp:89-94,
+ * bindings.h:540-629; uv_grad_mode TGS_UV_NONE / TGS_UV_DOUBLE) and
+ * `rasterize_to_pixels_bwd_2dgs` (ext.cpp:78-80) when textures == NULL.
+ * v_render_distort may be NULL (no distortion term).  render_accum_depth
+ * NULL selects the reference's distortion rule, which seeds the depth
+ * accumulator with render_colors[..., K-1] (that includes the texture colour
+ * and T*background of the channel); non-NULL gives the true gradient.
+ * Accumulated outputs (caller zero-fills): v_means2d_abs[C,N,2] (may be
+ * NULL), v_means2d[C,N,2], v_ray_transforms[C,N,3,3], v_colors[C,N,K],
+ * v_opacities[C,N], v_textures[N,tex_h,tex_w,4] (NULL if textures NULL),
+ * v_normals[C,N,3].  v_densify[C,N,2] (may be NULL) is written after the
+ * accumulation: v_ray_transforms[.,0,2] * T[2,2], v_ray_transforms[.,1,2] * T[2,2]
+ * (the reference reads it while other waves still accumulate; this is the
+ * settled value).                                                         */
+int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
+                      const void *ray_transforms, const void *colors, const void *opacities,
+                      const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
+                      double texture_range_x, double texture_range_y,
+                      const void *backgrounds, const uint8_t *masks, int32_t width,
+                      int32_t height, int32_t tile_size, int32_t tile_width,
+                      int32_t tile_height, const int32_t *tile_offsets,
+                      const int32_t *flatten_ids, int64_t n_isects, double g_weight,
+                      const void *render_colors, const void *render_alphas,
+                      const int32_t *last_ids, const int32_t *median_ids,
+                      const void *render_accum_depth, const void *v_render_colors,
+                      const void *v_render_alphas,
+                      const void *v_render_normals, const void *v_render_distort,
+                      const void *v_render_median, int uv_grad_mode, void *v_means2d_abs,
+                      void *v_means2d, void *v_ray_transforms, void *v_colors,
+                      void *v_opacities, void *v_textures, void *v_normals, void *v_densify,
+                      tgs_stream_t stream);
+
+/* ------------------------------------------------------------ fused Adam
+ * Replaces `selective_adam_update` (ext.cpp:73, bindings.h:486-497, adam.cu):
+ * rows g < N with visibility[g] != 0 are updated element-wise,
+ * m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; p -= lr m / (sqrt(v) + eps)
+ * (no bias correction).  param/grad/exp_avg/exp_avg_sq are f32[N*M].     */
+int tgs_selective_adam_update(int64_t N, int64_t M, float *param, const float *grad,
+                              float *exp_avg, float *exp_avg_sq, const uint8_t *visibility,
+                              double lr, double b1, double b2, double eps,
+                              tgs_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGS_HIP_H */

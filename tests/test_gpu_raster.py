@@ -1,0 +1,191 @@
+"""Rasterizer forward/backward on the GPU vs the oracle, fed identical inputs
+(the GPU's projection outputs and tile lists).
+
+Tolerances: float32 kernels - rendered colour/alpha/normal within 1e-4 max-abs,
+gradients within 1e-3 relative (max |d| / max |ref| per tensor); float64
+kernels - 1e-9, i.e. the same arithmetic as the oracle."""
+import pytest
+import torch
+
+from oracle.raster import rasterize_to_pixels
+from textured_gaussians_amd import scenes
+from textured_gaussians_amd.cuda import _ops
+
+from gpu_util import DEV, cpu64, gpu_stages, rel_err
+
+pytestmark = pytest.mark.gpu
+
+UV = {"none": _ops.rasterize_to_pixels_bwd_textured_gaussians,
+      "exact": _ops.rasterize_to_pixels_bwd_exact_textured_gaussians,
+      "double": _ops.rasterize_to_pixels_bwd2_textured_gaussians}
+
+
+def _inputs(dtype, K=3, n=2500, w=80, h=64, tex=12, seed=0):
+    sc = scenes.config1(n=n, width=w, height=h, tex_res=tex, seed=seed, sh_degree=None)
+    st = gpu_stages(sc, dtype)
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(seed + 11)
+    colors = torch.rand(C, N, K, generator=g, dtype=torch.float64)
+    opac = sc.opacities.double()[None].repeat(C, 1)
+    bg = torch.rand(C, K, generator=g, dtype=torch.float64)
+    return sc, st, colors, opac, bg
+
+
+def _oracle_fwd(st, colors, opac, textures, bg, uv="exact", g_weight=1.0, thr=0.0, distort="exact"):
+    s = st["scene"]
+    return rasterize_to_pixels(cpu64(st["means2d"]), cpu64(st["rt"]), colors, opac, cpu64(st["normals"]), s.width,
+                               s.height, st["tile_size"], st["offsets"].cpu(), st["flatten_ids"].cpu(),
+                               textures=textures, backgrounds=bg, g_weight=g_weight, gs_contrib_threshold=thr,
+                               uv_grad=uv, distort_grad=distort)
+
+
+def _gpu_fwd(st, colors, opac, textures, bg, dtype, g_weight=1.0, thr=0.0):
+    s = st["scene"]
+    dv = lambda t: None if t is None else t.to(DEV, dtype).contiguous()  # noqa: E731
+    if textures is not None:
+        return _ops.rasterize_to_pixels_fwd_textured_gaussians_full(
+            st["means2d"], st["rt"], dv(colors), dv(opac), dv(textures), 3.0, 3.0, st["normals"], dv(bg), None,
+            s.width, s.height, st["tile_size"], st["offsets"], st["flatten_ids"], thr, g_weight)
+    return _ops.rasterize_to_pixels_fwd_2dgs_full(st["means2d"], st["rt"], dv(colors), dv(opac), st["normals"],
+                                                  dv(bg), None, s.width, s.height, st["tile_size"], st["offsets"],
+                                                  st["flatten_ids"], thr)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("textured", [True, False])
+@pytest.mark.parametrize("K", [1, 3, 4])
+def test_raster_fwd(dtype, textured, K):
+    sc, st, colors, opac, bg = _inputs(dtype, K=K)
+    tex = sc.textures.double() if textured else None
+    out = _gpu_fwd(st, colors, opac, tex, bg, dtype)
+    ref = _oracle_fwd(st, colors, opac, tex, bg)
+    names = ["render_colors", "render_alphas", "render_normals", "render_distort", "render_median"]
+    tol = 1e-4 if dtype == torch.float32 else 1e-9
+    # float32: compare away from pixels with an entry within rounding margin of a
+    # discrete decision (those must stay rare); float64: every pixel
+    keep = torch.ones_like(ref["fragile"])
+    if dtype == torch.float32:
+        # exclude pixels that are ill-conditioned in float32 arithmetic itself: an entry
+        # within rounding margin of a discrete decision, or the float32 oracle departing
+        # from the float64 oracle by more than a tenth of the tolerance
+        r32 = rasterize_to_pixels(*[t.float() if torch.is_tensor(t) and t.is_floating_point() else t for t in (
+            cpu64(st["means2d"]), cpu64(st["rt"]), colors, opac, cpu64(st["normals"]))], st["scene"].width,
+            st["scene"].height, st["tile_size"], st["offsets"].cpu(), st["flatten_ids"].cpu(),
+            textures=None if tex is None else tex.float(), backgrounds=bg.float())
+        dev32 = (r32["render_colors"].double() - ref["render_colors"]).abs().amax(-1)
+        dev32 = torch.maximum(dev32, (r32["render_alphas"].double() - ref["render_alphas"]).abs()[..., 0])
+        keep = ~ref["fragile"] & (dev32 < 1e-5)
+    assert keep.float().mean().item() > 0.99
+    for nm, o in zip(names, out[:5]):
+        d = (cpu64(o) - ref[nm]).abs().amax(-1) if o.dim() == 4 else (cpu64(o) - ref[nm]).abs()
+        bad = (d > tol) & keep
+        assert not bad.any(), (nm, d[keep].max().item(), int(bad.sum()), bad.nonzero()[:4].tolist(),
+                               ref["fragile"].float().mean().item())
+    assert torch.equal(out[5].cpu()[keep], ref["last_ids"][keep])
+    assert torch.equal(out[6].cpu()[keep], ref["median_ids"][keep])
+    assert torch.equal(out[8].cpu().double(), ref["contrib_count"]) or dtype == torch.float32
+    assert rel_err(out[7], ref["contrib_sum"]) < (1e-5 if dtype == torch.float32 else 1e-10)
+
+
+def test_raster_fwd_contrib_threshold_and_gweight():
+    sc, st, colors, opac, bg = _inputs(torch.float64)
+    tex = sc.textures.double()
+    out = _gpu_fwd(st, colors, opac, tex, bg, torch.float64, g_weight=0.7, thr=0.3)
+    ref = _oracle_fwd(st, colors, opac, tex, bg, g_weight=0.7, thr=0.3)
+    assert (cpu64(out[0]) - ref["render_colors"]).abs().max().item() < 1e-9
+    assert torch.equal(out[8].cpu().double(), ref["contrib_count"])
+
+
+def _bwd_case(dtype, uv, textured=True, K=3, use_bg=True, seed=0):
+    sc, st, colors, opac, bg = _inputs(dtype, K=K, seed=seed)
+    if not use_bg:
+        bg = None
+    s = st["scene"]
+    tex = sc.textures.double() if textured else None
+    out = _gpu_fwd(st, colors, opac, tex, bg, dtype)
+    C, H, W = st["C"], s.height, s.width
+    g = torch.Generator().manual_seed(seed + 7)
+    vc = torch.randn(C, H, W, K, generator=g, dtype=torch.float64)
+    va = torch.randn(C, H, W, 1, generator=g, dtype=torch.float64)
+    vn = torch.randn(C, H, W, 3, generator=g, dtype=torch.float64)
+    vd = torch.randn(C, H, W, 1, generator=g, dtype=torch.float64)
+    vm = torch.randn(C, H, W, 1, generator=g, dtype=torch.float64)
+    dv = lambda t: None if t is None else t.to(DEV, dtype).contiguous()  # noqa: E731
+    densify = torch.zeros_like(st["means2d"])
+    args = [st["means2d"], st["rt"], dv(colors), dv(opac)]
+    if textured:
+        args += [dv(tex), 3.0, 3.0]
+    args += [st["normals"], densify, dv(bg), None, W, H, st["tile_size"], st["offsets"], st["flatten_ids"]]
+    if textured:
+        args += [1.0]
+    args += [out[0], out[1], out[5], out[6], dv(vc), dv(va), dv(vn), dv(vd), dv(vm), True]
+    exact = uv == "exact"
+    accd = out[9] if exact else None
+    if textured:
+        res = UV[uv](*args, render_accum_depth=accd)
+    else:
+        res = _ops.rasterize_to_pixels_bwd_2dgs(*args, render_accum_depth=accd)
+    # oracle: autograd of the restated forward
+    leaves = dict(means2d=cpu64(st["means2d"]).requires_grad_(True), rt=cpu64(st["rt"]).requires_grad_(True),
+                  colors=colors.clone().requires_grad_(True), opac=opac.clone().requires_grad_(True),
+                  normals=cpu64(st["normals"]).requires_grad_(True))
+    if textured:
+        leaves["tex"] = tex.clone().requires_grad_(True)
+    pairs = []
+    ref = rasterize_to_pixels(leaves["means2d"], leaves["rt"], leaves["colors"], leaves["opac"], leaves["normals"],
+                              W, H, st["tile_size"], st["offsets"].cpu(), st["flatten_ids"].cpu(),
+                              textures=leaves.get("tex"), backgrounds=bg, uv_grad=uv, pair_means=pairs,
+                              distort_grad="exact" if exact else "reference")
+    L = ((ref["render_colors"] * vc).sum() + (ref["render_alphas"] * va).sum() +
+         (ref["render_normals"] * vn).sum() + (ref["render_distort"] * vd).sum() +
+         (ref["render_median"] * vm).sum())
+    L.backward()
+    absg = torch.zeros_like(leaves["means2d"]).reshape(-1, 2)
+    for c, gid, mp in pairs:
+        if mp.grad is not None:
+            absg.index_add_(0, gid, mp.grad.abs().sum(0))
+    if textured:
+        v_abs, v_m2, v_rt, v_col, v_op, v_tex, v_nrm, v_dens = res
+    else:
+        v_abs, v_m2, v_rt, v_col, v_op, v_nrm, v_dens = res
+        v_tex = None
+    got = dict(means2d=v_m2, rt=v_rt, colors=v_col, opac=v_op, normals=v_nrm)
+    if textured:
+        got["tex"] = v_tex
+    # float32: relative error in the L2 norm (single ill-conditioned pairs, e.g. splats
+    # seen edge-on, dominate the max); float64: max-abs relative
+    tol = 1e-3 if dtype == torch.float32 else 1e-8
+    for k, v in got.items():
+        ref_g = leaves[k].grad
+        if dtype == torch.float32:
+            e = (cpu64(v) - ref_g).norm().item() / max(ref_g.norm().item(), 1e-30)
+        else:
+            e = rel_err(v, ref_g)
+        assert e < tol, (k, e)
+    if dtype == torch.float32:
+        assert (cpu64(v_abs).reshape(-1, 2) - absg).norm() / absg.norm() < tol
+    else:
+        assert rel_err(v_abs.reshape(-1, 2), absg) < tol
+    exp_dens = torch.stack([cpu64(v_rt)[..., 0, 2], cpu64(v_rt)[..., 1, 2]], -1) * cpu64(st["rt"])[..., 2, 2, None]
+    assert rel_err(v_dens, exp_dens) < (1e-6 if dtype == torch.float32 else 1e-12)
+
+
+@pytest.mark.parametrize("uv", ["none", "exact", "double"])
+def test_raster_bwd_textured_f64(uv):
+    _bwd_case(torch.float64, uv)
+
+
+@pytest.mark.parametrize("uv", ["none", "exact"])
+def test_raster_bwd_textured_f32(uv):
+    _bwd_case(torch.float32, uv)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("rule", ["none", "exact"])
+def test_raster_bwd_plain(dtype, rule):
+    _bwd_case(dtype, rule, textured=False)
+
+
+@pytest.mark.parametrize("K", [1, 4])
+def test_raster_bwd_channels_nobg(K):
+    _bwd_case(torch.float64, "exact", K=K, use_bg=False, seed=3)

@@ -1,0 +1,122 @@
+// Shared device helpers and C-ABI error plumbing for libtgs_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "tgs_hip.h"
+
+namespace tgs {
+
+// ------------------------------------------------------------ error plumbing
+void set_error(const char *fmt, ...);
+
+#define TGS_CHECK_ARG(cond, ...)              \
+    do {                                      \
+        if (!(cond)) {                        \
+            ::tgs::set_error(__VA_ARGS__);    \
+            return TGS_ERR_ARG;               \
+        }                                     \
+    } while (0)
+
+#define TGS_CHECK_LAUNCH(what)                                                  \
+    do {                                                                        \
+        hipError_t e_ = hipGetLastError();                                      \
+        if (e_ != hipSuccess) {                                                 \
+            ::tgs::set_error("%s: launch failed: %s", what, hipGetErrorString(e_)); \
+            return TGS_ERR_LAUNCH;                                              \
+        }                                                                       \
+    } while (0)
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ------------------------------------------------------------ small vectors
+template <typename T> struct v3 {
+    T x, y, z;
+};
+template <typename T> __device__ __forceinline__ v3<T> mk3(T x, T y, T z) { return {x, y, z}; }
+template <typename T> __device__ __forceinline__ v3<T> operator+(v3<T> a, v3<T> b) {
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <typename T> __device__ __forceinline__ v3<T> operator-(v3<T> a, v3<T> b) {
+    return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename T> __device__ __forceinline__ v3<T> operator*(T s, v3<T> a) {
+    return {s * a.x, s * a.y, s * a.z};
+}
+template <typename T> __device__ __forceinline__ T dot(v3<T> a, v3<T> b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename T> __device__ __forceinline__ v3<T> cross(v3<T> a, v3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// 3x3 matrix, row-major: m[r][c]
+template <typename T> struct m33 {
+    T m[3][3];
+};
+template <typename T> __device__ __forceinline__ m33<T> matmul(const m33<T> &a, const m33<T> &b) {
+    m33<T> o;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o.m[r][c] = a.m[r][0] * b.m[0][c] + a.m[r][1] * b.m[1][c] + a.m[r][2] * b.m[2][c];
+    return o;
+}
+template <typename T> __device__ __forceinline__ m33<T> transpose(const m33<T> &a) {
+    m33<T> o;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o.m[r][c] = a.m[c][r];
+    return o;
+}
+template <typename T> __device__ __forceinline__ v3<T> matvec(const m33<T> &a, v3<T> v) {
+    return {a.m[0][0] * v.x + a.m[0][1] * v.y + a.m[0][2] * v.z,
+            a.m[1][0] * v.x + a.m[1][1] * v.y + a.m[1][2] * v.z,
+            a.m[2][0] * v.x + a.m[2][1] * v.y + a.m[2][2] * v.z};
+}
+
+template <typename T> __device__ __forceinline__ T tgs_exp(T x);
+template <> __device__ __forceinline__ float tgs_exp<float>(float x) { return __expf(x); }
+template <> __device__ __forceinline__ double tgs_exp<double>(double x) { return exp(x); }
+
+template <typename T> __device__ __forceinline__ T tgs_rsqrt(T x);
+template <> __device__ __forceinline__ float tgs_rsqrt<float>(float x) { return rsqrtf(x); }
+template <> __device__ __forceinline__ double tgs_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
+
+// Atomic add without return value (lowers to global_atomic_add_f32/f64).
+template <typename T> __device__ __forceinline__ void atomic_add(T *p, T v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum over the 64 lanes of the wavefront; every lane must be active.
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, kWave);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Block-index remap so that consecutive work items land on the same XCD.
+// The dispatcher hands workgroup b to XCD (b mod 8); this returns the work
+// item for b such that each XCD owns one contiguous range of items.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+    constexpr uint32_t X = 8;
+    const uint32_t q = nb / X, r = nb % X;
+    const uint32_t x = b % X, k = b / X;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+}  // namespace tgs

@@ -1,0 +1,784 @@
+// Tile rasterizer for (textured) 2D Gaussian splats: front-to-back alpha
+// compositing with a per-Gaussian bilinear RGBA texture, and its backward.
+//
+// Forward semantics: textured_gaussians/cuda/csrc/rasterize_to_pixels_textured_gaussians_2dgs_fwd.cu:105-445
+// (TEX=true) and rasterize_to_pixels_2dgs_fwd.cu (TEX=false); bilinear lookup:
+// filters/bilinear.cuh:15-64.  Backward: ..._textured_gaussians_2dgs_bwd.cu:118-702,
+// ..._bwd2.cu:611-676 (uv rule), rasterize_to_pixels_2dgs_bwd.cu.
+//
+// Work decomposition (CDNA4): one workgroup of tile_size^2 threads (256 for
+// 16x16 tiles = four wave64s, each owning 4 pixel rows) per screen tile.
+// Batches of the tile's depth-sorted Gaussians are staged in LDS as
+// structure-of-arrays (all lanes read the same LDS word: a broadcast, no bank
+// conflicts).  The per-entry loop is wave-uniform: finished lanes stay in the
+// loop predicated off, so wave reductions (per-Gaussian gradient sums,
+// contribution counters) always run with a full exec mask; a wave leaves the
+// batch when a ballot shows all its lanes done, the workgroup leaves the tile
+// when a barrier count shows all pixels done.  Workgroups are remapped so each
+// XCD renders one contiguous band of tiles (neighbouring tiles share
+// Gaussians, hence texels, in that XCD's L2).  Textures are addressed with
+// 64-bit offsets and read one RGBA texel per 16-byte load.
+#include "common.h"
+
+namespace tgs {
+
+template <typename S> struct Texel {
+    S c[4];
+};
+template <typename S> __device__ __forceinline__ Texel<S> load_texel(const S *p);
+template <> __device__ __forceinline__ Texel<float> load_texel<float>(const float *p) {
+    const float4 v = *(const float4 *)p;
+    return {{v.x, v.y, v.z, v.w}};
+}
+template <> __device__ __forceinline__ Texel<double> load_texel<double>(const double *p) {
+    const double2 a = *(const double2 *)p;
+    const double2 b = *(const double2 *)(p + 2);
+    return {{a.x, a.y, b.x, b.y}};
+}
+
+template <typename S> struct RasterArgs {
+    int64_t C, N, n_isects;
+    const S *means2d, *rt, *colors, *opac, *normals, *textures, *bg;
+    const uint8_t *masks;
+    int64_t tex_h, tex_w;
+    S trx, try_, g_weight, vis_bias, thr;
+    int32_t W, H, ts, tw, th;
+    const int32_t *offsets, *flatten_ids;
+    // forward outputs
+    S *out_c, *out_a, *out_n, *out_d, *out_m, *csum, *ccnt;
+    S *out_accd;  // optional: sum_i w_i depth_i per pixel (the forward's distortion accumulator)
+    int32_t *last_ids, *median_ids;
+    // backward inputs
+    const S *r_c, *r_a, *v_c, *v_a, *v_n, *v_d, *v_m;
+    const S *r_accd;  // optional: see out_accd; NULL = the reference's seed, render_colors[..., K-1]
+    const int32_t *r_last, *r_med;
+    int uv_mode;
+    // backward outputs
+    S *g_xy_abs, *g_xy, *g_rt, *g_col, *g_op, *g_tex, *g_nrm;
+};
+
+// LDS batch of `bs` Gaussians, structure of arrays:
+// id | mx | my | op | T[9] | col[K] | nrm[3]
+template <typename S, int K> struct Batch {
+    int32_t *id;
+    S *f;
+    int bs;
+    static constexpr int kFields = 15 + K;
+    __host__ __device__ static size_t bytes(int bs) { return ((size_t)bs * 4 + 15) / 16 * 16 + (size_t)kFields * bs * sizeof(S); }
+    __device__ Batch(char *base, int bs_) : bs(bs_) {
+        id = (int32_t *)base;
+        f = (S *)(base + ((size_t)bs * 4 + 15) / 16 * 16);
+    }
+    __device__ __forceinline__ S &mx(int s) { return f[s]; }
+    __device__ __forceinline__ S &my(int s) { return f[bs + s]; }
+    __device__ __forceinline__ S &op(int s) { return f[2 * bs + s]; }
+    __device__ __forceinline__ S &T(int k, int s) { return f[(3 + k) * bs + s]; }
+    __device__ __forceinline__ S &col(int k, int s) { return f[(12 + k) * bs + s]; }
+    __device__ __forceinline__ S &nrm(int k, int s) { return f[(12 + K + k) * bs + s]; }
+    __device__ __forceinline__ void load(int s, int32_t g, const RasterArgs<S> &a) {
+        id[s] = g;
+        const int64_t gg = g;
+        mx(s) = a.means2d[gg * 2];
+        my(s) = a.means2d[gg * 2 + 1];
+        op(s) = a.opac[gg];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) T(k, s) = a.rt[gg * 9 + k];
+#pragma unroll
+        for (int k = 0; k < K; ++k) col(k, s) = a.colors[gg * K + k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) nrm(k, s) = a.normals[gg * 3 + k];
+    }
+};
+
+// Bilinear lookup state for one (pixel, Gaussian) pair.
+template <typename S> struct Bilerp {
+    bool in_range;
+    S wu, wv;
+    int64_t o00, o10, o01, o11;  // element offsets of the four texels
+};
+
+template <typename S>
+__device__ __forceinline__ Bilerp<S> bilerp_setup(S sx, S sy, int64_t n, const RasterArgs<S> &a) {
+#pragma clang fp contract(off)
+    Bilerp<S> b;
+    // inside [-range, range]^2; NaN coordinates count as outside so no texel
+    // address is ever formed from them
+    b.in_range = (sx >= -a.trx) && (sx <= a.trx) && (sy >= -a.try_) && (sy <= a.try_);
+    const S umax = S(a.tex_w - 1), vmax = S(a.tex_h - 1);
+    S u = (sx + a.trx) / (a.trx * S(2)) * umax;
+    S v = (sy + a.try_) / (a.try_ * S(2)) * vmax;
+    u = u < S(0) ? S(0) : (u > umax ? umax : u);
+    v = v < S(0) ? S(0) : (v > vmax ? vmax : v);
+    if (!b.in_range) { u = 0; v = 0; }
+    const S fu = floor(u), fv = floor(v);
+    const int64_t u0 = (int64_t)fu, v0 = (int64_t)fv;
+    const int64_t u1 = (int64_t)ceil(u), v1 = (int64_t)ceil(v);
+    b.wu = u - fu;
+    b.wv = v - fv;
+    const int64_t base = n * a.tex_h;
+    b.o00 = ((base + v0) * a.tex_w + u0) * 4;
+    b.o10 = ((base + v0) * a.tex_w + u1) * 4;
+    b.o01 = ((base + v1) * a.tex_w + u0) * 4;
+    b.o11 = ((base + v1) * a.tex_w + u1) * 4;
+    return b;
+}
+
+
+// Per-(pixel, Gaussian) geometry, shared by the forward and the backward so that
+// both take identical keep/skip decisions (FP contraction disabled here).
+template <typename S> struct PairGeom {
+    v3<S> hu, hv;
+    S cz, sx, sy, w3, w2, dx, dy, sigma, ker, vis;
+    bool okz, use3;
+};
+
+template <typename S>
+__device__ __forceinline__ PairGeom<S> pair_geom(S px, S py, const S Tm[9], S mx, S my, S vis_bias, S g_weight) {
+#pragma clang fp contract(off)
+    PairGeom<S> p;
+    p.hu = mk3(px * Tm[6] - Tm[0], px * Tm[7] - Tm[1], px * Tm[8] - Tm[2]);
+    p.hv = mk3(py * Tm[6] - Tm[3], py * Tm[7] - Tm[4], py * Tm[8] - Tm[5]);
+    p.cz = p.hu.x * p.hv.y - p.hu.y * p.hv.x;
+    p.okz = p.cz != S(0);
+    const S czs = p.okz ? p.cz : S(1);
+    p.sx = (p.hu.y * p.hv.z - p.hu.z * p.hv.y) / czs;
+    p.sy = (p.hu.z * p.hv.x - p.hu.x * p.hv.z) / czs;
+    p.w3 = p.sx * p.sx + p.sy * p.sy;
+    p.dx = mx - px;
+    p.dy = my - py;
+    p.w2 = S(2) * (p.dx * p.dx + p.dy * p.dy);
+    p.use3 = p.w3 <= p.w2;
+    p.sigma = S(0.5) * (p.use3 ? p.w3 : p.w2);
+    p.ker = tgs_exp<S>(-p.sigma);
+    p.vis = vis_bias + g_weight * p.ker;
+    return p;
+}
+
+template <typename S> __device__ __forceinline__ S pair_alpha(S op, S vis, S a_s) {
+#pragma clang fp contract(off)
+    const S al = op * vis * a_s;
+    return al < S(0.999) ? al : S(0.999);
+}
+
+template <typename S> __device__ __forceinline__ bool pair_keep(const PairGeom<S> &p, S alpha) {
+    return p.okz && !(p.sigma < S(0)) && !(alpha < S(1) / S(255));
+}
+
+template <typename S> struct Texels {
+    Texel<S> t00, t10, t01, t11;
+};
+
+// Texture-gradient update of one pixel for one texel row: the two horizontally
+// adjacent texels (u0, v) and (u1, v), RGBA each, 8 values; du = element
+// distance from the first texel to the second (0 when u0 == u1).
+template <typename S> struct TexRec {
+    int64_t off;
+    int32_t du;
+    int32_t live;
+    S v[8];
+};
+
+// Bilinear blend of the four texels: a_s (alpha channel) and the K colour channels.
+template <typename S, int K>
+__device__ __forceinline__ S bilerp_blend(const Bilerp<S> &b, const Texels<S> &t, S tex[K]) {
+#pragma clang fp contract(off)
+    const S w00 = (S(1) - b.wu) * (S(1) - b.wv), w10 = b.wu * (S(1) - b.wv);
+    const S w01 = (S(1) - b.wu) * b.wv, w11 = b.wu * b.wv;
+#pragma unroll
+    for (int k = 0; k < K; ++k) tex[k] = w00 * t.t00.c[k] + w10 * t.t10.c[k] + w01 * t.t01.c[k] + w11 * t.t11.c[k];
+    return w00 * t.t00.c[3] + w10 * t.t10.c[3] + w01 * t.t01.c[3] + w11 * t.t11.c[3];
+}
+
+template <typename S> __device__ __forceinline__ Texels<S> load_texels(const S *tex, const Bilerp<S> &b) {
+    Texels<S> t;
+    t.t00 = load_texel(tex + b.o00);
+    t.t10 = load_texel(tex + b.o10);
+    t.t01 = load_texel(tex + b.o01);
+    t.t11 = load_texel(tex + b.o11);
+    return t;
+}
+
+// ------------------------------------------------------------------ forward
+template <typename S, int K, bool TEX>
+__global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) {
+    extern __shared__ __align__(16) char smem[];
+    const int bs = blockDim.x;
+    Batch<S, K> B(smem, bs);
+    S *s_csum = (S *)(smem + Batch<S, K>::bytes(bs));  // per-entry contribution sums of the batch
+    S *s_ccnt = s_csum + bs;
+
+    const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
+    const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t c = tile_lin / tiles_per_cam;
+    const int64_t tid = tile_lin % tiles_per_cam;
+    const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
+    const int r = threadIdx.x;
+    const int i = ty * a.ts + r / a.ts, j = tx * a.ts + r % a.ts;
+    const bool inside = i < a.H && j < a.W;
+    const int64_t pix = (c * a.H + i) * a.W + j;
+    const S px = S(j) + S(0.5), py = S(i) + S(0.5);
+
+    if (a.masks && !a.masks[tile_lin]) {
+        if (inside) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) a.out_c[pix * K + k] = a.bg ? a.bg[c * K + k] : S(0);
+            a.out_a[pix] = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) a.out_n[pix * 3 + k] = 0;
+            a.out_d[pix] = 0;
+            a.out_m[pix] = 0;
+            a.last_ids[pix] = 0;
+            a.median_ids[pix] = 0;
+            if (a.out_accd) a.out_accd[pix] = 0;
+        }
+        return;
+    }
+
+    const int32_t start = a.offsets[tile_lin];
+    const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : a.offsets[tile_lin + 1];
+    const int nbatch = end > start ? (end - start + bs - 1) / bs : 0;
+
+    S T = 1, distort = 0, acc_vd = 0, median = 0;
+    int32_t cur = 0, median_id = 0;
+    S pc[K], pn[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < K; ++k) pc[k] = 0;
+    bool done = !inside;
+
+    for (int b = 0; b < nbatch; ++b) {
+        if (__syncthreads_count(done) >= bs) break;  // every pixel of the tile is finished
+        const int32_t bstart = start + b * bs;
+        const int32_t idx = bstart + r;
+        if (idx < end) B.load(r, a.flatten_ids[idx], a);
+        s_csum[r] = 0;
+        s_ccnt[r] = 0;
+        __syncthreads();
+        const int bsize = min(bs, end - bstart);
+        for (int t = 0; t < bsize; ++t) {
+            if (__ballot(!done) == 0) break;  // wave-uniform exit
+            S contrib = 0;
+            bool counted = false;
+            if (!done) {
+                S Tm[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) Tm[k] = B.T(k, t);
+                const PairGeom<S> p = pair_geom<S>(px, py, Tm, B.mx(t), B.my(t), a.vis_bias, a.g_weight);
+                if (p.okz) {
+                    S a_s = 1, tex[K];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) tex[k] = 0;
+                    if constexpr (TEX) {
+                        const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, (int64_t)B.id[t] % a.N, a);
+                        if (bl.in_range) a_s = bilerp_blend<S, K>(bl, load_texels(a.textures, bl), tex);
+                    }
+                    const S alpha = pair_alpha(B.op(t), p.vis, a_s);
+                    if (pair_keep(p, alpha)) {
+                        const S nT = T * (S(1) - alpha);
+                        if (nT <= S(1e-4)) {
+                            done = true;  // this entry is excluded
+                        } else {
+                            const S w = alpha * T;
+#pragma unroll
+                            for (int k = 0; k < K; ++k) pc[k] += (B.col(k, t) + tex[k]) * w;
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) pn[k] += B.nrm(k, t) * w;
+                            const S depth = B.col(K - 1, t);
+                            distort += S(2) * (w * depth * (S(1) - T) - w * acc_vd);
+                            acc_vd += w * depth;
+                            if (T > S(0.5)) {
+                                median = depth;
+                                median_id = bstart + t;
+                            }
+                            cur = bstart + t;
+                            T = nT;
+                            if (alpha > a.thr) {
+                                contrib = alpha;
+                                counted = true;
+                            }
+                        }
+                    }
+                }
+            }
+            // per-Gaussian contribution counters: wave sum -> LDS -> one global atomic per tile
+            const uint64_t cm = __ballot(counted);
+            if (cm) {
+                const S s = wave_sum(contrib);
+                if ((threadIdx.x & (kWave - 1)) == 0) {
+                    atomicAdd(&s_csum[t], s);
+                    atomicAdd(&s_ccnt[t], S(__popcll(cm)));
+                }
+            }
+        }
+        __syncthreads();
+        if (r < bsize && s_ccnt[r] != S(0)) {
+            const int64_t n = (int64_t)B.id[r] % a.N;
+            atomic_add(a.csum + n, s_csum[r]);
+            atomic_add(a.ccnt + n, s_ccnt[r]);
+        }
+    }
+    if (inside) {
+        a.out_a[pix] = S(1) - T;
+#pragma unroll
+        for (int k = 0; k < K; ++k) a.out_c[pix * K + k] = a.bg ? pc[k] + T * a.bg[c * K + k] : pc[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.out_n[pix * 3 + k] = pn[k];
+        a.last_ids[pix] = cur;
+        a.out_d[pix] = distort;
+        a.out_m[pix] = median;
+        a.median_ids[pix] = median_id;
+        if (a.out_accd) a.out_accd[pix] = acc_vd;
+    }
+}
+
+// ----------------------------------------------------------------- backward
+template <typename S, int K, bool TEX>
+__global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
+    extern __shared__ __align__(16) char smem[];
+    const int bs = blockDim.x;
+    Batch<S, K> B(smem, bs);
+    // per-wave staging of texture-gradient updates (2 texel rows x 64 pixels)
+    TexRec<S> *stage = (TexRec<S> *)(smem + Batch<S, K>::bytes(bs)) + (threadIdx.x / kWave) * (2 * kWave);
+
+    const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
+    const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t c = tile_lin / tiles_per_cam;
+    const int64_t tid = tile_lin % tiles_per_cam;
+    if (a.masks && !a.masks[tile_lin]) return;
+    const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
+    const int r = threadIdx.x;
+    const int lane = r & (kWave - 1);
+    const int i = ty * a.ts + r / a.ts, j = tx * a.ts + r % a.ts;
+    const bool inside = i < a.H && j < a.W;
+    const int64_t pix = inside ? (c * a.H + i) * a.W + j : (c * a.H + a.H - 1) * a.W + a.W - 1;
+    const S px = S(j) + S(0.5), py = S(i) + S(0.5);
+
+    const int32_t start = a.offsets[tile_lin];
+    const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : a.offsets[tile_lin + 1];
+    const int nbatch = end > start ? (end - start + bs - 1) / bs : 0;
+
+    // transmittance after the last contributing entry; walked back by division
+    const S T_final = S(1) - a.r_a[pix];
+    S T = T_final;
+    S buf[K], bufn[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < K; ++k) buf[k] = 0;
+    const int32_t bin_final = inside ? a.r_last[pix] : 0;
+    const int32_t median_idx = inside ? a.r_med[pix] : 0;
+    S vc[K], vn[3];
+#pragma unroll
+    for (int k = 0; k < K; ++k) vc[k] = a.v_c[pix * K + k];
+    const S va = a.v_a[pix];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vn[k] = a.v_n[pix * 3 + k];
+    S vdist = 0, acc_d = 0, acc_w = 0, acc_d_buf = 0, acc_w_buf = 0, dist_buf = 0;
+    if (a.v_d) {
+        vdist = a.v_d[pix];
+        // The reference seeds the depth accumulator with render_colors[..., K-1], which also holds
+        // the texture colour and T*background of that channel; with r_accd the forward's own
+        // accumulator is used instead and the distortion gradient is the true one.
+        acc_d_buf = a.r_accd ? a.r_accd[pix] : a.r_c[pix * K + K - 1];
+        acc_d = acc_d_buf;
+        acc_w_buf = a.r_a[pix];
+        acc_w = acc_w_buf;
+    }
+    const bool has_d = vdist != S(0);  // the distortion terms all carry a factor v_distort
+    const S vmed = a.v_m[pix];
+    S bg_dot = 0;
+    if (a.bg) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) bg_dot += a.bg[c * K + k] * vc[k];
+    }
+    const int32_t wave_bin_final = wave_max(inside ? bin_final : int32_t(-1));
+    S du_ds = 0, dv_ds = 0;
+    if (TEX && a.uv_mode == TGS_UV_EXACT) {
+        du_ds = S(a.tex_w - 1) / (S(2) * a.trx);
+        dv_ds = S(a.tex_h - 1) / (S(2) * a.try_);
+    } else if (TEX && a.uv_mode == TGS_UV_DOUBLE) {
+        du_ds = S(a.tex_w - 1) / a.trx;
+        dv_ds = S(a.tex_h - 1) / a.try_;
+    }
+    const bool uv_term = TEX && a.uv_mode != TGS_UV_NONE;
+
+    for (int b = 0; b < nbatch; ++b) {
+        __syncthreads();
+        const int32_t bend = end - 1 - b * bs;  // list position of slot 0 (entries are stored back to front)
+        const int bsize = min(bs, bend + 1 - start);
+        const int32_t idx = bend - r;
+        if (idx >= start) B.load(r, a.flatten_ids[idx], a);
+        __syncthreads();
+        const int t0 = max(0, bend - wave_bin_final);
+        for (int t = t0; t < bsize; ++t) {
+            const int32_t gidx = bend - t;
+            S Tm[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Tm[k] = B.T(k, t);
+            const PairGeom<S> p = pair_geom<S>(px, py, Tm, B.mx(t), B.my(t), a.vis_bias, a.g_weight);
+            bool valid = inside && gidx <= bin_final && p.okz;
+            const S opac = B.op(t);
+            const int32_t g = B.id[t];
+            S a_s = 1, tex[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) tex[k] = 0;
+            Bilerp<S> bl;
+            Texels<S> tt;
+            bl.in_range = false;
+            if constexpr (TEX) {
+                if (valid) {
+                    bl = bilerp_setup<S>(p.sx, p.sy, (int64_t)g % a.N, a);
+                    if (bl.in_range) {
+                        tt = load_texels(a.textures, bl);
+                        a_s = bilerp_blend<S, K>(bl, tt, tex);
+                    }
+                }
+            }
+            const S alpha = pair_alpha(opac, p.vis, a_s);
+            valid = valid && pair_keep(p, alpha);
+            if (__ballot(valid) == 0) continue;  // wave-uniform
+
+            bool tex_live = false;
+            S g_col[K], g_n[3] = {0, 0, 0}, g_T[9], g_xy[2] = {0, 0}, g_abs[2] = {0, 0}, g_op = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) g_col[k] = 0;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) g_T[k] = 0;
+            if (valid) {
+                if (gidx == median_idx) g_col[K - 1] += vmed;
+                const S ra = S(1) / (S(1) - alpha);
+                T *= ra;
+                const S fac = alpha * T;
+                S v_alpha = 0;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    g_col[k] += fac * vc[k];
+                    v_alpha += ((B.col(k, t) + tex[k]) * T - buf[k] * ra) * vc[k];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    g_n[k] = fac * vn[k];
+                    v_alpha += (B.nrm(k, t) * T - bufn[k] * ra) * vn[k];
+                }
+                v_alpha += T_final * ra * va;
+                v_alpha += -T_final * ra * bg_dot;
+                if (has_d) {
+                    const S depth = B.col(K - 1, t);
+                    const S dl_dw = S(2) * (S(2) * (depth * acc_w_buf - acc_d_buf) + (acc_d - depth * acc_w));
+                    v_alpha += (dl_dw * T - dist_buf * ra) * vdist;
+                    acc_d_buf -= fac * depth;
+                    acc_w_buf -= fac;
+                    dist_buf += dl_dw * fac;
+                    g_col[K - 1] += S(2) * fac * (S(2) - S(2) * T - acc_w + fac) * vdist;
+                }
+                const bool unclamped = opac * p.vis * a_s <= S(0.999);
+                S vsx = 0, vsy = 0;
+                if (unclamped) {
+                    const S v_G = opac * v_alpha * a_s;
+                    const S dk = -a.g_weight * p.ker;  // d(vis)/d(sigma)
+                    if (p.use3) {
+                        vsx = v_G * dk * p.sx;
+                        vsy = v_G * dk * p.sy;
+                    } else {
+                        g_xy[0] = v_G * dk * S(2) * p.dx;
+                        g_xy[1] = v_G * dk * S(2) * p.dy;
+                        g_abs[0] = fabs(g_xy[0]);
+                        g_abs[1] = fabs(g_xy[1]);
+                    }
+                    g_op = p.vis * v_alpha * a_s;
+                }
+                if constexpr (TEX) {
+                    if (bl.in_range) {
+                        const S w00 = (S(1) - bl.wu) * (S(1) - bl.wv), w10 = bl.wu * (S(1) - bl.wv);
+                        const S w01 = (S(1) - bl.wu) * bl.wv, w11 = bl.wu * bl.wv;
+                        // d(loss)/d(sample) per channel: colour channels k < K, alpha channel 3
+                        // (for K = 4 channel 3 carries both, as in the forward's sum)
+                        const S v_as = unclamped ? p.vis * opac * v_alpha : S(0);
+                        S q[4] = {0, 0, 0, 0};
+#pragma unroll
+                        for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
+                        q[3] += v_as;
+                        TexRec<S> &r0 = stage[lane];
+                        TexRec<S> &r1 = stage[kWave + lane];
+                        r0.off = bl.o00;
+                        r0.du = (int32_t)(bl.o10 - bl.o00);
+                        r1.off = bl.o01;
+                        r1.du = (int32_t)(bl.o11 - bl.o01);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            r0.v[k] = w00 * q[k];
+                            r0.v[4 + k] = w10 * q[k];
+                            r1.v[k] = w01 * q[k];
+                            r1.v[4 + k] = w11 * q[k];
+                        }
+                        tex_live = true;
+                        if (uv_term) {
+                            // d(sample)/du and d(sample)/dv of the bilinear blend, against the
+                            // sample's cotangent (colour channels, plus alpha when unclamped)
+                            S gu = 0, gv = 0;
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                const S vk = fac * vc[k];
+                                gu += vk * ((S(1) - bl.wv) * (tt.t10.c[k] - tt.t00.c[k]) + bl.wv * (tt.t11.c[k] - tt.t01.c[k]));
+                                gv += vk * ((S(1) - bl.wu) * (tt.t01.c[k] - tt.t00.c[k]) + bl.wu * (tt.t11.c[k] - tt.t10.c[k]));
+                            }
+                            gu += v_as * ((S(1) - bl.wv) * (tt.t10.c[3] - tt.t00.c[3]) + bl.wv * (tt.t11.c[3] - tt.t01.c[3]));
+                            gv += v_as * ((S(1) - bl.wu) * (tt.t01.c[3] - tt.t00.c[3]) + bl.wu * (tt.t11.c[3] - tt.t10.c[3]));
+                            vsx += gu * du_ds;
+                            vsy += gv * dv_ds;
+                        }
+                    }
+                }
+                if (vsx != S(0) || vsy != S(0)) {
+                    // s = (cr.x, cr.y) / cr.z, cr = hu x hv, hu = px T2 - T0, hv = py T2 - T1
+                    const S ax = vsx / p.cz, ay = vsy / p.cz;
+                    const v3<S> vcr = mk3(ax, ay, -(ax * p.sx + ay * p.sy));
+                    const v3<S> vhu = cross(p.hv, vcr);
+                    const v3<S> vhv = cross(vcr, p.hu);
+                    g_T[0] = -vhu.x; g_T[1] = -vhu.y; g_T[2] = -vhu.z;
+                    g_T[3] = -vhv.x; g_T[4] = -vhv.y; g_T[5] = -vhv.z;
+                    g_T[6] = px * vhu.x + py * vhv.x;
+                    g_T[7] = px * vhu.y + py * vhv.y;
+                    g_T[8] = px * vhu.z + py * vhv.z;
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) buf[k] += (tex[k] + B.col(k, t)) * fac;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) bufn[k] += B.nrm(k, t) * fac;
+            }
+            if constexpr (TEX) {
+                // Texture-gradient flush, transposed through LDS: in each round 8 consecutive
+                // lanes add the 8 values of one pixel's texel row (two adjacent RGBA texels,
+                // 32 contiguous bytes), so one atomic instruction touches 8 such runs instead
+                // of 64 scattered dwords.
+                const uint64_t live = __ballot(tex_live);
+                if (live) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const int sub = lane >> 3, e = lane & 7;
+#pragma unroll 1
+                    for (int grp = 0; grp < 8; ++grp) {
+                        if (((live >> (grp * 8)) & 0xffull) == 0) continue;
+                        const int src = grp * 8 + sub;
+                        if ((live >> src) & 1ull) {
+#pragma unroll
+                            for (int row = 0; row < 2; ++row) {
+                                const TexRec<S> &rr = stage[row * kWave + src];
+                                const S val = rr.v[e];
+                                if (val != S(0)) atomic_add(a.g_tex + rr.off + (e >> 2) * rr.du + (e & 3), val);
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            // per-Gaussian sums over the wavefront; lane f then adds field f
+#pragma unroll
+            for (int k = 0; k < K; ++k) g_col[k] = wave_sum(g_col[k]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g_n[k] = wave_sum(g_n[k]);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) g_T[k] = wave_sum(g_T[k]);
+            g_xy[0] = wave_sum(g_xy[0]);
+            g_xy[1] = wave_sum(g_xy[1]);
+            g_op = wave_sum(g_op);
+            if (a.g_xy_abs) {
+                g_abs[0] = wave_sum(g_abs[0]);
+                g_abs[1] = wave_sum(g_abs[1]);
+            }
+            {
+                // lane f adds field f of each per-Gaussian gradient array (one instruction per array)
+                const int64_t gg = g;
+                S v = 0;
+#pragma unroll
+                for (int f = 0; f < 9; ++f) v = lane == f ? g_T[f] : v;
+                if (lane < 9 && v != S(0)) atomic_add(a.g_rt + gg * 9 + lane, v);
+                v = 0;
+#pragma unroll
+                for (int f = 0; f < K; ++f) v = lane == f ? g_col[f] : v;
+                if (lane < K && v != S(0)) atomic_add(a.g_col + gg * K + lane, v);
+                v = lane == 0 ? g_n[0] : (lane == 1 ? g_n[1] : g_n[2]);
+                if (lane < 3 && v != S(0)) atomic_add(a.g_nrm + gg * 3 + lane, v);
+                v = lane == 0 ? g_xy[0] : g_xy[1];
+                if (lane < 2 && v != S(0)) atomic_add(a.g_xy + gg * 2 + lane, v);
+                if (a.g_xy_abs) {
+                    v = lane == 0 ? g_abs[0] : g_abs[1];
+                    if (lane < 2 && v != S(0)) atomic_add(a.g_xy_abs + gg * 2 + lane, v);
+                }
+                if (lane == 0 && g_op != S(0)) atomic_add(a.g_op + gg, g_op);
+            }
+        }
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void densify_kernel(int64_t n, const S *__restrict__ rt, const S *__restrict__ g_rt,
+                                                      S *__restrict__ v_densify) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const S depth = rt[i * 9 + 8];
+    v_densify[i * 2] = g_rt[i * 9 + 2] * depth;
+    v_densify[i * 2 + 1] = g_rt[i * 9 + 5] * depth;
+}
+
+template <typename S, int K, bool TEX> static void launch_fwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
+    raster_fwd_kernel<S, K, TEX><<<grid, bs, Batch<S, K>::bytes(bs) + 2 * bs * sizeof(S), st>>>(a);
+}
+template <typename S, int K, bool TEX> static void launch_bwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
+    const size_t stage = TEX ? (size_t)(bs / kWave) * 2 * kWave * sizeof(TexRec<S>) : 0;
+    raster_bwd_kernel<S, K, TEX><<<grid, bs, Batch<S, K>::bytes(bs) + stage, st>>>(a);
+}
+
+template <typename S, bool BWD>
+static int dispatch(const RasterArgs<S> &a, int K, bool tex, hipStream_t st) {
+    const unsigned grid = (unsigned)(a.C * a.tw * a.th);
+    const int bs = a.ts * a.ts;
+    if (grid == 0) return TGS_OK;
+#define TGS_CASE(KK)                                                   \
+    case KK:                                                           \
+        if (BWD) {                                                     \
+            if (tex) launch_bwd<S, KK, true>(a, grid, bs, st);         \
+            else launch_bwd<S, KK, false>(a, grid, bs, st);            \
+        } else {                                                       \
+            if (tex) launch_fwd<S, KK, true>(a, grid, bs, st);         \
+            else launch_fwd<S, KK, false>(a, grid, bs, st);            \
+        }                                                              \
+        break;
+    switch (K) {
+        TGS_CASE(1)
+        TGS_CASE(2)
+        TGS_CASE(3)
+        TGS_CASE(4)
+        default:
+            set_error("rasterize: Unsupported number of channels: %d", K);
+            return TGS_ERR_UNSUPPORTED;
+    }
+#undef TGS_CASE
+    TGS_CHECK_LAUNCH(BWD ? "rasterize_bwd" : "rasterize_fwd");
+    return TGS_OK;
+}
+
+template <typename S>
+static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *means2d, const void *rt,
+                        const void *colors, const void *opac, const void *normals, const void *textures,
+                        int64_t tex_h, int64_t tex_w, double trx, double try_, const void *bg, const uint8_t *masks,
+                        int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th, const int32_t *offsets,
+                        const int32_t *flatten_ids, int64_t n_isects, double g_weight) {
+    memset(&a, 0, sizeof(a));
+    a.C = C; a.N = N; a.n_isects = n_isects;
+    a.means2d = (const S *)means2d; a.rt = (const S *)rt; a.colors = (const S *)colors;
+    a.opac = (const S *)opac; a.normals = (const S *)normals; a.textures = (const S *)textures;
+    a.bg = (const S *)bg; a.masks = masks;
+    a.tex_h = tex_h; a.tex_w = tex_w; a.trx = (S)trx; a.try_ = (S)try_;
+    if (textures) {
+        a.g_weight = (S)g_weight;
+        a.vis_bias = S(0.998) - (S)g_weight;
+    } else {
+        a.g_weight = S(1);
+        a.vis_bias = S(0);
+    }
+    a.W = W; a.H = H; a.ts = ts; a.tw = tw; a.th = th;
+    a.offsets = offsets; a.flatten_ids = flatten_ids;
+}
+
+static int check_common(int64_t C, int64_t N, int32_t K, const void *textures, int64_t tex_h, int64_t tex_w,
+                        int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th, int64_t n_isects) {
+    TGS_CHECK_ARG(C >= 0 && N > 0 && C * N < (int64_t(1) << 31), "rasterize: bad sizes C=%lld N=%lld",
+                  (long long)C, (long long)N);
+    TGS_CHECK_ARG(K >= 1 && K <= 4, "rasterize: Unsupported number of channels: %d", K);
+    TGS_CHECK_ARG(ts == 8 || ts == 16, "rasterize: tile_size %d unsupported (8 or 16)", ts);
+    TGS_CHECK_ARG((int64_t)tw * ts >= W && (int64_t)th * ts >= H, "rasterize: tile grid does not cover the image");
+    TGS_CHECK_ARG(n_isects >= 0 && n_isects < (int64_t(1) << 31), "rasterize: bad n_isects");
+    TGS_CHECK_ARG(!textures || (tex_h >= 1 && tex_w >= 1), "rasterize: bad texture size");
+    return TGS_OK;
+}
+
+}  // namespace tgs
+
+using namespace tgs;
+
+extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
+                                 const void *ray_transforms, const void *colors, const void *opacities,
+                                 const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
+                                 double texture_range_x, double texture_range_y, const void *backgrounds,
+                                 const uint8_t *masks, int32_t width, int32_t height, int32_t tile_size,
+                                 int32_t tile_width, int32_t tile_height, const int32_t *tile_offsets,
+                                 const int32_t *flatten_ids, int64_t n_isects, double gs_contrib_threshold,
+                                 double g_weight, void *render_colors, void *render_alphas, void *render_normals,
+                                 void *render_distort, void *render_median, int32_t *last_ids, int32_t *median_ids,
+                                 void *contrib_sum, void *contrib_count, void *render_accum_depth,
+                                 tgs_stream_t stream) {
+    int rc = check_common(C, N, K, textures, tex_h, tex_w, width, height, tile_size, tile_width, tile_height,
+                          n_isects);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+#define TGS_FWD(S)                                                                                              \
+    {                                                                                                           \
+        RasterArgs<S> a;                                                                                        \
+        fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
+                       texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
+                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight);                 \
+        a.thr = (S)gs_contrib_threshold;                                                                        \
+        a.out_c = (S *)render_colors; a.out_a = (S *)render_alphas; a.out_n = (S *)render_normals;              \
+        a.out_d = (S *)render_distort; a.out_m = (S *)render_median; a.last_ids = last_ids;                     \
+        a.median_ids = median_ids; a.csum = (S *)contrib_sum; a.ccnt = (S *)contrib_count;                      \
+        a.out_accd = (S *)render_accum_depth;                                                                   \
+        return dispatch<S, false>(a, K, textures != nullptr, st);                                               \
+    }
+    if (dtype == TGS_F32) TGS_FWD(float)
+    if (dtype == TGS_F64) TGS_FWD(double)
+#undef TGS_FWD
+    set_error("rasterize_fwd: unsupported dtype %d", dtype);
+    return TGS_ERR_UNSUPPORTED;
+}
+
+extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
+                                 const void *ray_transforms, const void *colors, const void *opacities,
+                                 const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
+                                 double texture_range_x, double texture_range_y, const void *backgrounds,
+                                 const uint8_t *masks, int32_t width, int32_t height, int32_t tile_size,
+                                 int32_t tile_width, int32_t tile_height, const int32_t *tile_offsets,
+                                 const int32_t *flatten_ids, int64_t n_isects, double g_weight,
+                                 const void *render_colors, const void *render_alphas, const int32_t *last_ids,
+                                 const int32_t *median_ids, const void *render_accum_depth,
+                                 const void *v_render_colors, const void *v_render_alphas,
+                                 const void *v_render_normals, const void *v_render_distort,
+                                 const void *v_render_median, int uv_grad_mode, void *v_means2d_abs, void *v_means2d,
+                                 void *v_ray_transforms, void *v_colors, void *v_opacities, void *v_textures,
+                                 void *v_normals, void *v_densify, tgs_stream_t stream) {
+    int rc = check_common(C, N, K, textures, tex_h, tex_w, width, height, tile_size, tile_width, tile_height,
+                          n_isects);
+    if (rc) return rc;
+    TGS_CHECK_ARG(uv_grad_mode >= 0 && uv_grad_mode <= 2, "rasterize_bwd: bad uv_grad_mode %d", uv_grad_mode);
+    TGS_CHECK_ARG(!textures || v_textures, "rasterize_bwd: v_textures required with textures");
+    hipStream_t st = (hipStream_t)stream;
+#define TGS_BWD(S)                                                                                              \
+    {                                                                                                           \
+        RasterArgs<S> a;                                                                                        \
+        fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
+                       texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
+                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight);                 \
+        a.r_c = (const S *)render_colors; a.r_a = (const S *)render_alphas; a.r_last = last_ids;                \
+        a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
+        a.r_accd = (const S *)render_accum_depth;                                                               \
+        a.v_n = (const S *)v_render_normals; a.v_d = (const S *)v_render_distort;                               \
+        a.v_m = (const S *)v_render_median; a.uv_mode = uv_grad_mode;                                           \
+        a.g_xy_abs = (S *)v_means2d_abs; a.g_xy = (S *)v_means2d; a.g_rt = (S *)v_ray_transforms;              \
+        a.g_col = (S *)v_colors; a.g_op = (S *)v_opacities; a.g_tex = (S *)v_textures; a.g_nrm = (S *)v_normals; \
+        if (n_isects > 0) {                                                                                     \
+            rc = dispatch<S, true>(a, K, textures != nullptr, st);                                              \
+            if (rc) return rc;                                                                                  \
+        }                                                                                                       \
+        if (v_densify) {                                                                                        \
+            const int64_t n = C * N;                                                                            \
+            densify_kernel<S><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, (const S *)ray_transforms,         \
+                                                                          (const S *)v_ray_transforms,          \
+                                                                          (S *)v_densify);                      \
+            TGS_CHECK_LAUNCH("rasterize_bwd densify");                                                          \
+        }                                                                                                       \
+        return TGS_OK;                                                                                          \
+    }
+    if (dtype == TGS_F32) TGS_BWD(float)
+    if (dtype == TGS_F64) TGS_BWD(double)
+#undef TGS_BWD
+    set_error("rasterize_bwd: unsupported dtype %d", dtype);
+    return TGS_ERR_UNSUPPORTED;
+}

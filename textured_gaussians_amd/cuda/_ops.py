@@ -1,0 +1,433 @@
+"""The native op table: same names, positional arguments and returns as the
+reference's pybind registry (``textured_gaussians/cuda/csrc/ext.cpp`` /
+``bindings.h``) for the textured-2DGS render path.
+
+Each op validates its tensors like the reference (``kernel_utils.h:11-17``:
+"x must be a CUDA tensor", "x must be contiguous"), allocates its outputs with
+torch's caching allocator on the input's device, and calls the C ABI of
+libtgs_hip.so on torch's current stream.  Nothing here computes on the CPU.
+"""
+import ctypes
+from typing import Optional, Tuple
+
+import torch
+from torch import Tensor
+
+from ._backend import check, lib
+
+_DTYPES = {torch.float32: 0, torch.float64: 1}
+
+UV_NONE, UV_EXACT, UV_DOUBLE = 0, 1, 2
+VJP_REFERENCE, VJP_REFERENCE_NORM, VJP_EXACT = 0, 1, 2
+
+
+def _p(t: Optional[Tensor]):
+    return None if t is None else t.data_ptr()
+
+
+# Optional per-launch timing of the rasterizer kernels: when a dict is
+# installed, each rasterizer call records a pair of HIP events on the stream it
+# launches on, under "raster_fwd" / "raster_bwd".
+_TIMER = None
+
+
+def set_kernel_timer(store: Optional[dict]) -> None:
+    global _TIMER
+    _TIMER = store
+
+
+def _timed(name, t: Tensor):
+    class _Ctx:
+        def __enter__(self):
+            if _TIMER is not None:
+                st = torch.cuda.current_stream(t.device)
+                self.e0 = torch.cuda.Event(enable_timing=True)
+                self.e1 = torch.cuda.Event(enable_timing=True)
+                self.e0.record(st)
+
+        def __exit__(self, *exc):
+            if _TIMER is not None:
+                self.e1.record(torch.cuda.current_stream(t.device))
+                _TIMER.setdefault(name, []).append((self.e0, self.e1))
+    return _Ctx()
+
+
+def _stream(t: Tensor):
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _check_input(t: Tensor, name: str) -> None:
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+def _dtype_code(t: Tensor, name: str) -> int:
+    if t.dtype not in _DTYPES:
+        raise RuntimeError(f"{name}: unsupported dtype {t.dtype} (float32 or float64)")
+    return _DTYPES[t.dtype]
+
+
+def _same(dtype: torch.dtype, **ts: Tensor) -> None:
+    for n, t in ts.items():
+        if t is not None and t.dtype != dtype:
+            raise RuntimeError(f"{n} must have dtype {dtype}, got {t.dtype}")
+
+
+# ------------------------------------------------------------------ projection
+def fully_fused_projection_fwd_2dgs(means, quats, scales, viewmats, Ks, image_width, image_height,
+                                    eps2d, near_plane, far_plane, radius_clip):
+    for n, t in dict(means=means, quats=quats, scales=scales, viewmats=viewmats, Ks=Ks).items():
+        _check_input(t, n)
+    dt = _dtype_code(means, "means")
+    _same(means.dtype, quats=quats, scales=scales, viewmats=viewmats, Ks=Ks)
+    N, C = means.size(0), viewmats.size(0)
+    with torch.cuda.device(means.device):
+        radii = torch.empty((C, N), dtype=torch.int32, device=means.device)
+        means2d = torch.empty((C, N, 2), dtype=means.dtype, device=means.device)
+        depths = torch.empty((C, N), dtype=means.dtype, device=means.device)
+        rt = torch.empty((C, N, 3, 3), dtype=means.dtype, device=means.device)
+        normals = torch.empty((C, N, 3), dtype=means.dtype, device=means.device)
+        check(lib().tgs_projection_2dgs_fwd(dt, C, N, _p(means), _p(quats), _p(scales), _p(viewmats), _p(Ks),
+                                            int(image_width), int(image_height), float(near_plane),
+                                            float(far_plane), float(radius_clip), _p(radii), _p(means2d),
+                                            _p(depths), _p(rt), _p(normals), _stream(means)),
+              "fully_fused_projection_fwd_2dgs")
+    return radii, means2d, depths, rt, normals
+
+
+def _projection_bwd(mode, means, quats, scales, viewmats, Ks, image_width, image_height, radii, ray_transforms,
+                    v_means2d, v_depths, v_normals, v_ray_transforms, viewmats_requires_grad):
+    ts = dict(means=means, quats=quats, scales=scales, viewmats=viewmats, Ks=Ks, radii=radii,
+              ray_transforms=ray_transforms, v_means2d=v_means2d, v_depths=v_depths, v_normals=v_normals,
+              v_ray_transforms=v_ray_transforms)
+    for n, t in ts.items():
+        _check_input(t, n)
+    dt = _dtype_code(means, "means")
+    N, C = means.size(0), viewmats.size(0)
+    with torch.cuda.device(means.device):
+        v_means = torch.zeros_like(means)
+        v_quats = torch.zeros_like(quats)
+        v_scales = torch.zeros_like(scales)
+        v_viewmats = torch.zeros_like(viewmats) if viewmats_requires_grad else None
+        vv = v_viewmats if (viewmats_requires_grad and mode == VJP_EXACT) else None
+        check(lib().tgs_projection_2dgs_bwd(dt, C, N, _p(means), _p(quats), _p(scales), _p(viewmats), _p(Ks),
+                                            _p(radii), _p(ray_transforms), _p(v_means2d), _p(v_depths),
+                                            _p(v_normals), _p(v_ray_transforms), mode, _p(v_means), _p(v_quats),
+                                            _p(v_scales), _p(vv), _stream(means)),
+              "fully_fused_projection_bwd_2dgs")
+    return v_means, v_quats, v_scales, v_viewmats
+
+
+def fully_fused_projection_bwd_2dgs(*args):
+    """Reference VJP (``fully_fused_projection_2dgs_bwd.cu``)."""
+    return _projection_bwd(VJP_REFERENCE, *args)
+
+
+def fully_fused_projection_bwd2_2dgs(*args):
+    """Reference ``norm_rot_grad`` VJP (``fully_fused_projection_2dgs_bwd2.cu``)."""
+    return _projection_bwd(VJP_REFERENCE_NORM, *args)
+
+
+def fully_fused_projection_bwd_exact_2dgs(*args):
+    """True derivative of the forward (no reference counterpart)."""
+    return _projection_bwd(VJP_EXACT, *args)
+
+
+# ------------------------------------------------------------------- isect
+def _bit_length(x: int) -> int:
+    return int(x).bit_length()
+
+
+def isect_tiles(means2d, radii, depths, camera_ids, gaussian_ids, C, tile_size, tile_width, tile_height,
+                sort, double_buffer):
+    _check_input(means2d, "means2d")
+    _check_input(radii, "radii")
+    _check_input(depths, "depths")
+    if means2d.dim() == 2:
+        if camera_ids is None or gaussian_ids is None:
+            raise RuntimeError("When packed is set, camera_ids and gaussian_ids must be provided.")
+        raise NotImplementedError("isect_tiles: the packed layout is not implemented")
+    if means2d.dtype != torch.float32 or depths.dtype != torch.float32:
+        # the key layout stores float32 depth bits (isect_tiles.cu:92)
+        means2d = means2d.float().contiguous()
+        depths = depths.float().contiguous()
+    C_, N = means2d.size(0), means2d.size(1)
+    n = C_ * N
+    dev = means2d.device
+    L = lib()
+    n_tiles = tile_width * tile_height
+    tile_bits = _bit_length(n_tiles)
+    cam_bits = _bit_length(C)
+    assert tile_bits + cam_bits <= 32
+    with torch.cuda.device(dev):
+        st = _stream(means2d)
+        tpg = torch.empty((C_, N), dtype=torch.int32, device=dev)
+        if n == 0:
+            return tpg, torch.empty(0, dtype=torch.int64, device=dev), torch.empty(0, dtype=torch.int32, device=dev)
+        cum = torch.empty(n, dtype=torch.int64, device=dev)
+        tb = L.tgs_isect_count_temp_bytes(n)
+        temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=dev)
+        check(L.tgs_isect_tiles_count(C_, N, _p(means2d), _p(radii), tile_size, tile_width, tile_height, _p(tpg),
+                                      _p(cum), _p(temp), tb, st), "isect_tiles")
+        n_isects = int(cum[-1].item())  # device -> host (isect_tiles.cu:200-201)
+        isect_ids = torch.empty(n_isects, dtype=torch.int64, device=dev)
+        flatten_ids = torch.empty(n_isects, dtype=torch.int32, device=dev)
+        if n_isects == 0:
+            return tpg, isect_ids, flatten_ids
+        check(L.tgs_isect_tiles_fill(C_, N, _p(means2d), _p(radii), _p(depths), _p(cum), tile_size, tile_width,
+                                     tile_height, _p(isect_ids), _p(flatten_ids), st), "isect_tiles")
+        if not sort:
+            return tpg, isect_ids, flatten_ids
+        keys_alt = torch.empty_like(isect_ids)
+        vals_alt = torch.empty_like(flatten_ids)
+        sb = L.tgs_sort_pairs_temp_bytes(n_isects)
+        stemp = torch.empty(sb, dtype=torch.uint8, device=dev)
+        sel = ctypes.c_int32(0)
+        check(L.tgs_sort_pairs(n_isects, 32 + tile_bits + cam_bits, _p(isect_ids), _p(flatten_ids), _p(keys_alt),
+                               _p(vals_alt), _p(stemp), sb, ctypes.byref(sel), st), "isect_tiles sort")
+        if sel.value == 1:
+            return tpg, keys_alt, vals_alt
+        return tpg, isect_ids, flatten_ids
+
+
+def isect_offset_encode(isect_ids, C, tile_width, tile_height):
+    _check_input(isect_ids, "isect_ids")
+    dev = isect_ids.device
+    with torch.cuda.device(dev):
+        offsets = torch.empty((C, tile_height, tile_width), dtype=torch.int32, device=dev)
+        check(lib().tgs_isect_offset_encode(isect_ids.numel(), _p(isect_ids), C, tile_width, tile_height,
+                                            _p(offsets), _stream(isect_ids)), "isect_offset_encode")
+    return offsets
+
+
+# -------------------------------------------------------------------- SH
+def compute_sh_fwd(degrees_to_use, dirs, coeffs, masks):
+    _check_input(dirs, "dirs")
+    _check_input(coeffs, "coeffs")
+    if masks is not None:
+        _check_input(masks, "masks")
+    if coeffs.size(-1) != 3:
+        raise RuntimeError("coeffs must have last dimension 3")
+    if dirs.size(-1) != 3:
+        raise RuntimeError("dirs must have last dimension 3")
+    dt = _dtype_code(dirs, "dirs")
+    K = coeffs.size(-2)
+    n = dirs.numel() // 3
+    with torch.cuda.device(dirs.device):
+        colors = torch.empty_like(dirs)
+        check(lib().tgs_sh_fwd(dt, n, K, degrees_to_use, _p(dirs), _p(coeffs),
+                               _p(masks.view(torch.uint8)) if masks is not None else None, _p(colors),
+                               _stream(dirs)), "compute_sh_fwd")
+    return colors
+
+
+def compute_sh_bwd(K, degrees_to_use, dirs, coeffs, masks, v_colors, compute_v_dirs):
+    _check_input(dirs, "dirs")
+    _check_input(coeffs, "coeffs")
+    _check_input(v_colors, "v_colors")
+    if masks is not None:
+        _check_input(masks, "masks")
+    dt = _dtype_code(dirs, "dirs")
+    n = dirs.numel() // 3
+    with torch.cuda.device(dirs.device):
+        v_coeffs = torch.empty_like(coeffs)
+        v_dirs = torch.zeros_like(dirs) if compute_v_dirs else None
+        check(lib().tgs_sh_bwd(dt, n, K, degrees_to_use, _p(dirs), _p(coeffs),
+                               _p(masks.view(torch.uint8)) if masks is not None else None, _p(v_colors),
+                               _p(v_coeffs), _p(v_dirs), _stream(dirs)), "compute_sh_bwd")
+    return v_coeffs, v_dirs
+
+
+# --------------------------------------------------------------- rasterizer
+def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
+                masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
+                g_weight):
+    ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
+              normals=normals, tile_offsets=tile_offsets, flatten_ids=flatten_ids)
+    if textured:
+        ts["textures"] = textures
+    for n, t in ts.items():
+        _check_input(t, n)
+    if backgrounds is not None:
+        _check_input(backgrounds, "backgrounds")
+    if masks is not None:
+        _check_input(masks, "masks")
+    if means2d.dim() != 3:
+        raise NotImplementedError("rasterize_to_pixels: the packed layout is not implemented")
+    dt = _dtype_code(means2d, "means2d")
+    _same(means2d.dtype, ray_transforms=ray_transforms, colors=colors, opacities=opacities, normals=normals,
+          textures=textures if textured else None, backgrounds=backgrounds)
+    C = tile_offsets.size(0)
+    N = means2d.size(1)
+    K = colors.size(-1)
+    th_t, tw_t = tile_offsets.size(1), tile_offsets.size(2)
+    H, W = image_height, image_width
+    dev, fdt = means2d.device, means2d.dtype
+    if textured:
+        tex_h, tex_w = textures.size(1), textures.size(2)
+    else:
+        tex_h = tex_w = 0
+    with torch.cuda.device(dev):
+        renders = torch.empty((C, H, W, K), dtype=fdt, device=dev)
+        alphas = torch.empty((C, H, W, 1), dtype=fdt, device=dev)
+        last_ids = torch.empty((C, H, W), dtype=torch.int32, device=dev)
+        median_ids = torch.empty((C, H, W), dtype=torch.int32, device=dev)
+        rnormals = torch.empty((C, H, W, 3), dtype=fdt, device=dev)
+        rdistort = torch.empty((C, H, W, 1), dtype=fdt, device=dev)
+        rmedian = torch.empty((C, H, W, 1), dtype=fdt, device=dev)
+        csum = torch.zeros((N,), dtype=fdt, device=dev)
+        ccnt = torch.zeros((N,), dtype=fdt, device=dev)
+        accd = torch.empty((C, H, W), dtype=fdt, device=dev)
+        with _timed("raster_fwd", means2d):
+            check(lib().tgs_rasterize_fwd(
+              dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
+              _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
+              _p(masks.view(torch.uint8)) if masks is not None else None, W, H, tile_size, tw_t, th_t,
+              _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
+              _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
+              _p(csum), _p(ccnt), _p(accd), _stream(means2d)), "rasterize_to_pixels_fwd")
+    return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd
+
+
+def rasterize_to_pixels_fwd_textured_gaussians(means2d, ray_transforms, colors, opacities, textures,
+                                               texture_range_x, texture_range_y, normals, backgrounds, masks,
+                                               image_width, image_height, tile_size, tile_offsets, flatten_ids,
+                                               gs_contrib_threshold, g_weight):
+    return _raster_fwd(True, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
+                       texture_range_y, normals, backgrounds, masks, image_width, image_height, tile_size,
+                       tile_offsets, flatten_ids, gs_contrib_threshold, g_weight)[:9]
+
+
+def rasterize_to_pixels_fwd_textured_gaussians_full(*args):
+    """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
+    per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W]."""
+    return _raster_fwd(True, *args)
+
+
+def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
+                                 image_width, image_height, tile_size, tile_offsets, flatten_ids,
+                                 gs_contrib_threshold):
+    return rasterize_to_pixels_fwd_2dgs_full(means2d, ray_transforms, colors, opacities, normals, backgrounds,
+                                             masks, image_width, image_height, tile_size, tile_offsets,
+                                             flatten_ids, gs_contrib_threshold)[:9]
+
+
+def rasterize_to_pixels_fwd_2dgs_full(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
+                                      image_width, image_height, tile_size, tile_offsets, flatten_ids,
+                                      gs_contrib_threshold):
+    return _raster_fwd(False, means2d, ray_transforms, colors, opacities, None, 1.0, 1.0, normals, backgrounds,
+                       masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
+                       gs_contrib_threshold, 1.0)
+
+
+def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals,
+                densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
+                g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
+                v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None):
+    ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
+              normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
+              render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
+              median_ids=median_ids, v_render_colors=v_render_colors, v_render_alphas=v_render_alphas,
+              v_render_normals=v_render_normals, v_render_distort=v_render_distort,
+              v_render_median=v_render_median)
+    if textured:
+        ts["textures"] = textures
+    for n, t in ts.items():
+        _check_input(t, n)
+    if backgrounds is not None:
+        _check_input(backgrounds, "backgrounds")
+    if masks is not None:
+        _check_input(masks, "masks")
+    if means2d.dim() != 3:
+        raise NotImplementedError("rasterize_to_pixels: the packed layout is not implemented")
+    dt = _dtype_code(means2d, "means2d")
+    C = tile_offsets.size(0)
+    N = means2d.size(1)
+    K = colors.size(-1)
+    th_t, tw_t = tile_offsets.size(1), tile_offsets.size(2)
+    dev = means2d.device
+    if textured:
+        tex_h, tex_w = textures.size(1), textures.size(2)
+    else:
+        tex_h = tex_w = 0
+    with torch.cuda.device(dev):
+        v_means2d = torch.zeros_like(means2d)
+        v_rt = torch.zeros_like(ray_transforms)
+        v_colors = torch.zeros_like(colors)
+        v_textures = torch.zeros_like(textures) if textured else None
+        v_normals = torch.zeros_like(normals)
+        v_opacities = torch.zeros_like(opacities)
+        v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
+        v_densify = torch.zeros_like(densify)
+        with _timed("raster_bwd", means2d):
+            check(lib().tgs_rasterize_bwd(
+              dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
+              _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
+              _p(masks.view(torch.uint8)) if masks is not None else None, image_width, image_height, tile_size,
+              tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(g_weight),
+              _p(render_colors), _p(render_alphas), _p(last_ids), _p(median_ids), _p(render_accum_depth),
+              _p(v_render_colors),
+              _p(v_render_alphas), _p(v_render_normals), _p(v_render_distort), _p(v_render_median), uv_mode,
+              _p(v_means2d_abs), _p(v_means2d), _p(v_rt), _p(v_colors), _p(v_opacities), _p(v_textures),
+              _p(v_normals), _p(v_densify) if densify.shape == means2d.shape else None,
+              _stream(means2d)), "rasterize_to_pixels_bwd")
+    if textured:
+        return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_textures, v_normals, v_densify
+    return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_normals, v_densify
+
+
+def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None):
+    """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
+    return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth)
+
+
+def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None):
+    """Reference ``bilinear_bwd2`` rule (du/ds = (res-1)/range)."""
+    return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth)
+
+
+def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None):
+    """True derivative: du/ds = (res-1)/(2 range), and the forward's own
+    distortion accumulator when ``render_accum_depth`` is given (no reference
+    counterpart)."""
+    return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth)
+
+
+def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
+                                 image_width, image_height, tile_size, tile_offsets, flatten_ids, render_colors,
+                                 render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
+                                 v_render_normals, v_render_distort, v_render_median, absgrad,
+                                 render_accum_depth=None):
+    return _raster_bwd(False, UV_NONE, means2d, ray_transforms, colors, opacities, None, 1.0, 1.0, normals,
+                       densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets,
+                       flatten_ids, 1.0, render_colors, render_alphas, last_ids, median_ids, v_render_colors,
+                       v_render_alphas, v_render_normals, v_render_distort, v_render_median, absgrad,
+                       render_accum_depth=render_accum_depth)
+
+
+# ------------------------------------------------------------------ Adam
+def selective_adam_update(param, param_grad, exp_avg, exp_avg_sq, tiles_touched, lr, b1, b2, eps, N, M):
+    for n, t in dict(param=param, param_grad=param_grad, exp_avg=exp_avg, exp_avg_sq=exp_avg_sq,
+                     tiles_touched=tiles_touched).items():
+        _check_input(t, n)
+    if param.dtype != torch.float32:
+        raise RuntimeError("selective_adam_update: param must be float32")
+    with torch.cuda.device(param.device):
+        check(lib().tgs_selective_adam_update(N, M, _p(param), _p(param_grad), _p(exp_avg), _p(exp_avg_sq),
+                                              _p(tiles_touched.view(torch.uint8)), float(lr), float(b1),
+                                              float(b2), float(eps), _stream(param)), "selective_adam_update")
+
+
+class _OpTable:
+    """Attribute access by op name, like the reference's ``_C`` module."""
+
+    def __getattr__(self, name):
+        fn = globals().get(name)
+        if fn is None or name.startswith("_"):
+            raise AttributeError(name)
+        return fn
+
+
+_C = _OpTable()

@@ -1,0 +1,353 @@
+"""Python op wrappers and autograd Functions of the textured-2DGS path.
+
+Mirrors the reference's ``textured_gaussians/cuda/_wrapper.py`` for the ops on
+this path: same function names, argument names, defaults, returns and
+autograd behaviour (``spherical_harmonics`` :54-80 / :1233-1263,
+``fully_fused_projection_2dgs`` :1267-1484, ``isect_tiles`` :349-420,
+``isect_offset_encode`` :423-440, ``rasterize_to_pixels_2dgs`` :1647-1784 /
+:2767-2944, ``rasterize_to_pixels_textured_gaussians`` :1806-2002 /
+:2947-3155, ``selective_adam_update`` :26-41).  Native ops are resolved by name
+from the op table, like the reference's ``_make_lazy_cuda_func``.
+
+Differences from the reference (each a fix or an addition):
+* ``filtering="bilinear_exact"`` and ``proj_grad="exact"`` select true
+  derivatives (the reference's rules are kept as the defaults).
+* ``v_backgrounds`` is returned when ``backgrounds`` requires grad (the
+  reference tests ``needs_input_grad[7]``, a float argument, so never does).
+* No ``torch.cuda.synchronize()`` after the backward kernels.
+"""
+from typing import Callable, Optional, Tuple
+
+import torch
+from torch import Tensor
+
+from ._ops import _C
+
+
+def _make_lazy_cuda_func(name: str) -> Callable:
+    def call(*args, **kwargs):
+        return getattr(_C, name)(*args, **kwargs)
+
+    return call
+
+
+def selective_adam_update(param: Tensor, param_grad: Tensor, exp_avg: Tensor, exp_avg_sq: Tensor,
+                          tiles_touched: Tensor, lr: float, b1: float, b2: float, eps: float, N: int,
+                          M: int) -> None:
+    _make_lazy_cuda_func("selective_adam_update")(param, param_grad, exp_avg, exp_avg_sq, tiles_touched, lr, b1,
+                                                  b2, eps, N, M)
+
+
+# ----------------------------------------------------------------------- SH
+def spherical_harmonics(degrees_to_use: int, dirs: Tensor, coeffs: Tensor,
+                        masks: Optional[Tensor] = None) -> Tensor:
+    """Computes spherical harmonics.  dirs [..., 3], coeffs [..., K, 3], masks
+    [...] -> colors [..., 3]."""
+    assert (degrees_to_use + 1) ** 2 <= coeffs.shape[-2], coeffs.shape
+    assert dirs.shape[:-1] == coeffs.shape[:-2], (dirs.shape, coeffs.shape)
+    assert dirs.shape[-1] == 3, dirs.shape
+    assert coeffs.shape[-1] == 3, coeffs.shape
+    if masks is not None:
+        assert masks.shape == dirs.shape[:-1], masks.shape
+        masks = masks.contiguous()
+    return _SphericalHarmonics.apply(degrees_to_use, dirs.contiguous(), coeffs.contiguous(), masks)
+
+
+class _SphericalHarmonics(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sh_degree: int, dirs: Tensor, coeffs: Tensor, masks: Tensor) -> Tensor:
+        colors = _make_lazy_cuda_func("compute_sh_fwd")(sh_degree, dirs, coeffs, masks)
+        ctx.save_for_backward(dirs, coeffs, masks)
+        ctx.sh_degree = sh_degree
+        ctx.num_bases = coeffs.shape[-2]
+        return colors
+
+    @staticmethod
+    def backward(ctx, v_colors: Tensor):
+        dirs, coeffs, masks = ctx.saved_tensors
+        compute_v_dirs = ctx.needs_input_grad[1]
+        v_coeffs, v_dirs = _make_lazy_cuda_func("compute_sh_bwd")(
+            ctx.num_bases, ctx.sh_degree, dirs, coeffs, masks, v_colors.contiguous(), compute_v_dirs)
+        if not compute_v_dirs:
+            v_dirs = None
+        return None, v_dirs, v_coeffs, None
+
+
+# --------------------------------------------------------------- projection
+def fully_fused_projection_2dgs(means: Tensor, quats: Tensor, scales: Tensor, viewmats: Tensor, Ks: Tensor,
+                                width: int, height: int, eps2d: float = 0.3, near_plane: float = 0.01,
+                                far_plane: float = 1e10, radius_clip: float = 0.0, packed: bool = False,
+                                sparse_grad: bool = False, norm_rot_grad: bool = False,
+                                proj_grad: str = "reference") -> Tuple[Tensor, Tensor, Tensor, Tensor, Tensor]:
+    """Prepare 2D Gaussians for rasterization: returns (radii [C,N] int32,
+    means2d [C,N,2], depths [C,N], ray_transforms [C,N,3,3], normals [C,N,3]).
+
+    ``proj_grad="exact"`` makes the backward the true derivative (the
+    reference's VJP drops part of the means2d term, ``utils.cuh:684-703``)."""
+    C = viewmats.size(0)
+    N = means.size(0)
+    assert means.size() == (N, 3), means.size()
+    assert viewmats.size() == (C, 4, 4), viewmats.size()
+    assert Ks.size() == (C, 3, 3), Ks.size()
+    assert quats is not None, "quats is required"
+    assert scales is not None, "scales is required"
+    assert quats.size() == (N, 4), quats.size()
+    assert scales.size() == (N, 3), scales.size()
+    assert proj_grad in ("reference", "exact"), proj_grad
+    if sparse_grad:
+        assert packed, "sparse_grad is only supported when packed is True"
+    if packed:
+        raise NotImplementedError("fully_fused_projection_2dgs: packed=True is not implemented")
+    return _FullyFusedProjection2DGS.apply(means.contiguous(), quats.contiguous(), scales.contiguous(),
+                                           viewmats.contiguous(), Ks.contiguous(), width, height, eps2d,
+                                           near_plane, far_plane, radius_clip, norm_rot_grad, proj_grad)
+
+
+class _FullyFusedProjection2DGS(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane,
+                radius_clip, norm_rot_grad, proj_grad):
+        radii, means2d, depths, ray_transforms, normals = _make_lazy_cuda_func("fully_fused_projection_fwd_2dgs")(
+            means, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane, radius_clip)
+        ctx.save_for_backward(means, quats, scales, viewmats, Ks, radii, ray_transforms, normals)
+        ctx.width, ctx.height = width, height
+        ctx.norm_rot_grad = norm_rot_grad
+        ctx.proj_grad = proj_grad
+        ctx.mark_non_differentiable(radii)
+        return radii, means2d, depths, ray_transforms, normals
+
+    @staticmethod
+    def backward(ctx, v_radii, v_means2d, v_depths, v_ray_transforms, v_normals):
+        means, quats, scales, viewmats, Ks, radii, ray_transforms, normals = ctx.saved_tensors
+        if ctx.proj_grad == "exact":
+            name = "fully_fused_projection_bwd_exact_2dgs"
+        elif ctx.norm_rot_grad:
+            name = "fully_fused_projection_bwd2_2dgs"
+        else:
+            name = "fully_fused_projection_bwd_2dgs"
+        v_means, v_quats, v_scales, v_viewmats = _make_lazy_cuda_func(name)(
+            means, quats, scales, viewmats, Ks, ctx.width, ctx.height, radii, ray_transforms,
+            v_means2d.contiguous(), v_depths.contiguous(), v_normals.contiguous(), v_ray_transforms.contiguous(),
+            ctx.needs_input_grad[3])
+        g = ctx.needs_input_grad
+        return (v_means if g[0] else None, v_quats if g[1] else None, v_scales if g[2] else None,
+                v_viewmats if g[3] else None, None, None, None, None, None, None, None, None, None)
+
+
+# -------------------------------------------------------------------- isect
+@torch.no_grad()
+def isect_tiles(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, tile_width: int,
+                tile_height: int, sort: bool = True, packed: bool = False, n_cameras: Optional[int] = None,
+                camera_ids: Optional[Tensor] = None, gaussian_ids: Optional[Tensor] = None
+                ) -> Tuple[Tensor, Tensor, Tensor]:
+    """Maps projected Gaussians to intersecting tiles.  Returns
+    (tiles_per_gauss int32 [C,N], isect_ids int64 [I] = cam | tile | depth bits,
+    flatten_ids int32 [I]), sorted by isect_ids (stable) when ``sort``."""
+    if packed:
+        nnz = means2d.size(0)
+        assert means2d.shape == (nnz, 2), means2d.size()
+        assert camera_ids is not None, "camera_ids is required if packed is True"
+        assert gaussian_ids is not None, "gaussian_ids is required if packed is True"
+        assert n_cameras is not None, "n_cameras is required if packed is True"
+        C = n_cameras
+    else:
+        C, N, _ = means2d.shape
+        assert means2d.shape == (C, N, 2), means2d.size()
+        assert radii.shape == (C, N), radii.size()
+        assert depths.shape == (C, N), depths.size()
+    return _make_lazy_cuda_func("isect_tiles")(means2d.contiguous(), radii.contiguous(), depths.contiguous(),
+                                               camera_ids, gaussian_ids, C, tile_size, tile_width, tile_height,
+                                               sort, True)
+
+
+@torch.no_grad()
+def isect_offset_encode(isect_ids: Tensor, n_cameras: int, tile_width: int, tile_height: int) -> Tensor:
+    """Encodes intersection ids to per-tile offsets [C, tile_height, tile_width]."""
+    return _make_lazy_cuda_func("isect_offset_encode")(isect_ids.contiguous(), n_cameras, tile_width, tile_height)
+
+
+# --------------------------------------------------------------- rasterizer
+def _pad_channels(colors: Tensor, backgrounds: Optional[Tensor]):
+    """Pad to the nearest supported channel count ([W]:1880-1902)."""
+    channels = colors.shape[-1]
+    if channels > 512 or channels == 0:
+        raise ValueError(f"Unsupported number of color channels: {channels}")
+    if channels not in (1, 2, 3, 4, 8, 16, 32, 64, 128, 256, 512):
+        padded = (1 << (channels - 1).bit_length()) - channels
+        colors = torch.cat([colors, torch.zeros(*colors.shape[:-1], padded, device=colors.device,
+                                                dtype=colors.dtype)], dim=-1)
+        if backgrounds is not None:
+            backgrounds = torch.cat([backgrounds, torch.zeros(*backgrounds.shape[:-1], padded,
+                                                              device=backgrounds.device,
+                                                              dtype=backgrounds.dtype)], dim=-1)
+    else:
+        padded = 0
+    return colors, backgrounds, padded
+
+
+PRE = "rasterize_to_pixels_"
+TGSPOST = "_textured_gaussians"
+tgs_fns = {
+    "bilinear": (f"{PRE}fwd{TGSPOST}", f"{PRE}bwd{TGSPOST}"),
+    "bilinear_bwd2": (f"{PRE}fwd{TGSPOST}", f"{PRE}bwd2{TGSPOST}"),
+    "bilinear_exact": (f"{PRE}fwd{TGSPOST}", f"{PRE}bwd_exact{TGSPOST}"),
+}
+
+
+def rasterize_to_pixels_textured_gaussians(
+        means2d: Tensor, ray_transforms: Tensor, colors: Tensor, opacities: Tensor, textures: Tensor,
+        texture_range_x: float, texture_range_y: float, normals: Tensor, densify: Tensor, image_width: int,
+        image_height: int, tile_size: int, isect_offsets: Tensor, flatten_ids: Tensor,
+        backgrounds: Optional[Tensor] = None, masks: Optional[Tensor] = None, packed: bool = False,
+        absgrad: bool = False, distloss: bool = False, gs_contrib_threshold: float = 0.0,
+        filtering: str = "bilinear", g_weight: float = 1.0):
+    """Rasterize textured 2D Gaussians to pixels.
+
+    textures [N, th, tw, 4] (RGBA, bilinear, sampled over splat coordinates
+    [-texture_range, texture_range]).  Returns (render_colors [C,H,W,D],
+    render_alphas [C,H,W,1], render_normals [C,H,W,3], render_distort
+    [C,H,W,1], render_median [C,H,W,1], gs_contrib_sum [N], gs_contrib_count [N]).
+    """
+    C = isect_offsets.size(0)
+    if packed:
+        raise NotImplementedError("rasterize_to_pixels_textured_gaussians: packed=True is not implemented")
+    N = means2d.size(1)
+    assert means2d.shape == (C, N, 2), means2d.shape
+    assert ray_transforms.shape == (C, N, 3, 3), ray_transforms.shape
+    assert colors.shape[:2] == (C, N), colors.shape
+    assert opacities.shape == (C, N), opacities.shape
+    assert textures.dim() == 4 and textures.shape[0] == N and textures.shape[-1] == 4, textures.shape
+    if backgrounds is not None:
+        assert backgrounds.shape == (C, colors.shape[-1]), backgrounds.shape
+        backgrounds = backgrounds.contiguous()
+    colors, backgrounds, padded = _pad_channels(colors, backgrounds)
+    tile_height, tile_width = isect_offsets.shape[1:3]
+    assert tile_height * tile_size >= image_height, f"Assert Failed: {tile_height} * {tile_size} >= {image_height}"
+    assert tile_width * tile_size >= image_width, f"Assert Failed: {tile_width} * {tile_size} >= {image_width}"
+    if filtering not in tgs_fns:
+        raise Exception(f"Unsupported filter type {filtering}")
+    out = _RasterizeToPixelsTexturedGaussians.apply(
+        tgs_fns[filtering][0], tgs_fns[filtering][1], means2d.contiguous(), ray_transforms.contiguous(),
+        colors.contiguous(), opacities.contiguous(), textures.contiguous(), texture_range_x, texture_range_y,
+        normals.contiguous(), densify.contiguous(), backgrounds, masks, image_width, image_height, tile_size,
+        isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad, distloss, gs_contrib_threshold, g_weight)
+    render_colors = out[0]
+    if padded > 0:
+        render_colors = render_colors[..., :-padded]
+    return (render_colors,) + tuple(out[1:])
+
+
+class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, fwd_fn, bwd_fn, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
+                texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
+                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight):
+        (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
+         gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func(fwd_fn + "_full")(
+            means2d, ray_transforms, colors, opacities, textures, texture_range_x, texture_range_y, normals,
+            backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
+            g_weight)
+        ctx.save_for_backward(means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds,
+                              masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
+                              median_ids, accum_depth)
+        ctx.bwd_fn = bwd_fn
+        ctx.texture_range = (texture_range_x, texture_range_y)
+        ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
+        ctx.absgrad, ctx.g_weight = absgrad, g_weight
+        ctx.mark_non_differentiable(gs_contrib_sum, gs_contrib_count)
+        return (render_colors, render_alphas, render_normals, render_distort, render_median, gs_contrib_sum,
+                gs_contrib_count)
+
+    @staticmethod
+    def backward(ctx, v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
+                 v_gs_contrib_sum, v_gs_contrib_count):
+        (means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds, masks, isect_offsets,
+         flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth) = ctx.saved_tensors
+        exact = ctx.bwd_fn == tgs_fns["bilinear_exact"][1]
+        (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, v_normals,
+         v_densify) = _make_lazy_cuda_func(ctx.bwd_fn)(
+            means2d, ray_transforms, colors, opacities, textures, ctx.texture_range[0], ctx.texture_range[1],
+            normals, densify, backgrounds, masks, ctx.width, ctx.height, ctx.tile_size, isect_offsets, flatten_ids,
+            ctx.g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors.contiguous(),
+            v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
+            v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None)
+        if ctx.absgrad:
+            means2d.absgrad = v_means2d_abs
+        v_backgrounds = None
+        if ctx.needs_input_grad[11]:
+            v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
+        return (None, None, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, None, None, v_normals,
+                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None)
+
+
+def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Tensor, opacities: Tensor,
+                             normals: Tensor, densify: Tensor, image_width: int, image_height: int,
+                             tile_size: int, isect_offsets: Tensor, flatten_ids: Tensor,
+                             backgrounds: Optional[Tensor] = None, masks: Optional[Tensor] = None,
+                             packed: bool = False, absgrad: bool = False, distloss: bool = False,
+                             gs_contrib_threshold: float = 0.0, grad_mode: str = "reference"):
+    """Rasterize plain 2D Gaussians to pixels (``alpha = min(.999, o exp(-sigma))``).
+    ``grad_mode="exact"`` uses the forward's own distortion accumulator in the
+    backward (the reference seeds it from ``render_colors``)."""
+    C = isect_offsets.size(0)
+    if packed:
+        raise NotImplementedError("rasterize_to_pixels_2dgs: packed=True is not implemented")
+    N = means2d.size(1)
+    assert means2d.shape == (C, N, 2), means2d.shape
+    assert ray_transforms.shape == (C, N, 3, 3), ray_transforms.shape
+    assert colors.shape[:2] == (C, N), colors.shape
+    assert opacities.shape == (C, N), opacities.shape
+    if backgrounds is not None:
+        assert backgrounds.shape == (C, colors.shape[-1]), backgrounds.shape
+        backgrounds = backgrounds.contiguous()
+    colors, backgrounds, padded = _pad_channels(colors, backgrounds)
+    tile_height, tile_width = isect_offsets.shape[1:3]
+    assert tile_height * tile_size >= image_height
+    assert tile_width * tile_size >= image_width
+    out = _RasterizeToPixels2DGS.apply(
+        means2d.contiguous(), ray_transforms.contiguous(), colors.contiguous(), opacities.contiguous(),
+        normals.contiguous(), densify.contiguous(), backgrounds, masks, image_width, image_height, tile_size,
+        isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad, distloss, gs_contrib_threshold,
+        grad_mode == "exact")
+    render_colors = out[0]
+    if padded > 0:
+        render_colors = render_colors[..., :-padded]
+    return (render_colors,) + tuple(out[1:])
+
+
+class _RasterizeToPixels2DGS(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks, width,
+                height, tile_size, isect_offsets, flatten_ids, absgrad, distloss, gs_contrib_threshold, exact):
+        (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
+         gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func("rasterize_to_pixels_fwd_2dgs_full")(
+            means2d, ray_transforms, colors, opacities, normals, backgrounds, masks, width, height, tile_size,
+            isect_offsets, flatten_ids, gs_contrib_threshold)
+        ctx.save_for_backward(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
+                              isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids,
+                              accum_depth)
+        ctx.width, ctx.height, ctx.tile_size, ctx.absgrad = width, height, tile_size, absgrad
+        ctx.exact = exact
+        ctx.mark_non_differentiable(gs_contrib_sum, gs_contrib_count)
+        return (render_colors, render_alphas, render_normals, render_distort, render_median, gs_contrib_sum,
+                gs_contrib_count)
+
+    @staticmethod
+    def backward(ctx, v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
+                 v_gs_contrib_sum, v_gs_contrib_count):
+        (means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks, isect_offsets,
+         flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth) = ctx.saved_tensors
+        (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_normals,
+         v_densify) = _make_lazy_cuda_func("rasterize_to_pixels_bwd_2dgs")(
+            means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks, ctx.width,
+            ctx.height, ctx.tile_size, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
+            median_ids, v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
+            v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad,
+            render_accum_depth=accum_depth if ctx.exact else None)
+        if ctx.absgrad:
+            means2d.absgrad = v_means2d_abs
+        v_backgrounds = None
+        if ctx.needs_input_grad[6]:
+            v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
+        return (v_means2d, v_ray_transforms, v_colors, v_opacities, v_normals, v_densify, v_backgrounds, None,
+                None, None, None, None, None, None, None, None, None)
