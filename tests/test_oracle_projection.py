@@ -8,41 +8,44 @@ import torch
 
 from oracle import projection as OP
 
-torch.set_default_dtype(torch.float64)
+
+def _t(x):
+    return torch.tensor(x, dtype=torch.float64)
+
 
 
 def _cam(fx=300.0, fy=280.0, cx=64.0, cy=48.0):
-    K = torch.tensor([[[fx, 0, cx], [0, fy, cy], [0, 0, 1.0]]])
-    return torch.eye(4)[None], K
+    K = _t([[[fx, 0, cx], [0, fy, cy], [0, 0, 1.0]]])
+    return torch.eye(4, dtype=torch.float64)[None], K
 
 
 def test_on_axis_closed_form():
     V, K = _cam()
     z, s0, s1 = 5.0, 0.05, 0.02
-    means = torch.tensor([[0.0, 0.0, z]])
-    quats = torch.tensor([[1.0, 0, 0, 0]])
-    scales = torch.tensor([[s0, s1, 1.0]])
+    means = _t([[0.0, 0.0, z]])
+    quats = _t([[1.0, 0, 0, 0]])
+    scales = _t([[s0, s1, 1.0]])
     radii, m2, dep, T, nrm = OP.project_2dgs(means, quats, scales, V, K, 128, 96)
-    assert torch.allclose(m2[0, 0], torch.tensor([64.0, 48.0]))
+    assert torch.allclose(m2[0, 0], _t([64.0, 48.0]))
     assert dep[0, 0].item() == z
-    exp_T = torch.tensor([[300 * s0, 0, 64 * z], [0, 280 * s1, 48 * z], [0, 0, z]])
+    exp_T = _t([[300 * s0, 0, 64 * z], [0, 280 * s1, 48 * z], [0, 0, z]])
     assert torch.allclose(T[0, 0], exp_T)
     assert radii[0, 0].item() == math.ceil(3 * max(300 * s0, 280 * s1) / z)
     # the normal faces the camera (-n . p > 0)
-    assert torch.allclose(nrm[0, 0], torch.tensor([0.0, 0.0, -1.0]))
+    assert torch.allclose(nrm[0, 0], _t([0.0, 0.0, -1.0]))
 
 
 def test_rotated_off_axis_layout():
     # asymmetric case: T must equal K @ [R Rq S e0, R Rq S e1, p_c] stored row-major
     g = torch.Generator().manual_seed(1)
     ang = 0.3
-    V = torch.eye(4)
-    V[:3, :3] = torch.tensor([[math.cos(ang), 0, math.sin(ang)], [0, 1, 0], [-math.sin(ang), 0, math.cos(ang)]])
-    V[:3, 3] = torch.tensor([0.2, -0.1, 0.5])
-    K = torch.tensor([[310.0, 0, 70], [0, 290, 50], [0, 0, 1]])
-    means = torch.tensor([[0.3, -0.2, 4.0]])
-    q = torch.randn(1, 4, generator=g)
-    scales = torch.tensor([[0.07, 0.03, 9.0]])
+    V = torch.eye(4, dtype=torch.float64)
+    V[:3, :3] = _t([[math.cos(ang), 0, math.sin(ang)], [0, 1, 0], [-math.sin(ang), 0, math.cos(ang)]])
+    V[:3, 3] = _t([0.2, -0.1, 0.5])
+    K = _t([[310.0, 0, 70], [0, 290, 50], [0, 0, 1]])
+    means = _t([[0.3, -0.2, 4.0]])
+    q = torch.randn(1, 4, generator=g, dtype=torch.float64)
+    scales = _t([[0.07, 0.03, 9.0]])
     radii, m2, dep, T, nrm = OP.project_2dgs(means, q, scales, V[None], K[None], 256, 256)
     # independent construction with numpy (normalised quaternion -> matrix)
     w, x, y, z = (q[0] / q[0].norm()).tolist()
@@ -68,33 +71,34 @@ def test_rotated_off_axis_layout():
 
 def test_culling():
     V, K = _cam()
-    means = torch.tensor([[0.0, 0, -1.0], [0.0, 0, 0.001], [50.0, 0, 1.0], [0.0, 0, 3.0]])
-    quats = torch.tensor([[1.0, 0, 0, 0]]).repeat(4, 1)
-    scales = torch.full((4, 3), 0.01)
+    means = _t([[0.0, 0, -1.0], [0.0, 0, 0.001], [50.0, 0, 1.0], [0.0, 0, 3.0]])
+    quats = _t([[1.0, 0, 0, 0]]).repeat(4, 1)
+    scales = torch.full((4, 3), 0.01, dtype=torch.float64)
     radii, *_ = OP.project_2dgs(means, quats, scales, V, K, 128, 96)
     assert radii[0].tolist() == [0, 0, 0, radii[0, 3].item()] and radii[0, 3] > 0
 
 
 def _scene(seed=0, N=40, C=2):
     g = torch.Generator().manual_seed(seed)
-    means = torch.randn(N, 3, generator=g) * 0.5
-    quats = torch.randn(N, 4, generator=g)
-    scales = torch.exp(torch.randn(N, 3, generator=g) * 0.3 - 2.5)
+    means = torch.randn(N, 3, generator=g, dtype=torch.float64) * 0.5
+    quats = torch.randn(N, 4, generator=g, dtype=torch.float64)
+    scales = torch.exp(torch.randn(N, 3, generator=g, dtype=torch.float64) * 0.3 - 2.5)
     Vs = []
     for c in range(C):
         a = 0.4 * c
-        V = torch.eye(4)
-        V[:3, :3] = torch.tensor([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
-        V[:3, 3] = torch.tensor([0.0, 0.0, 4.0])
+        V = torch.eye(4, dtype=torch.float64)
+        V[:3, :3] = _t([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+        V[:3, 3] = _t([0.0, 0.0, 4.0])
         Vs.append(V)
-    K = torch.tensor([[200.0, 0, 64], [0, 200, 48], [0, 0, 1]])
+    K = _t([[200.0, 0, 64], [0, 200, 48], [0, 0, 1]])
     return means, quats, scales, torch.stack(Vs), K[None].repeat(C, 1, 1)
 
 
 def _cotangents(C, N, seed=3):
     g = torch.Generator().manual_seed(seed)
-    return (torch.randn(C, N, 2, generator=g), torch.randn(C, N, generator=g), torch.randn(C, N, 3, generator=g),
-            torch.randn(C, N, 3, 3, generator=g))
+    f = torch.float64
+    return (torch.randn(C, N, 2, generator=g, dtype=f), torch.randn(C, N, generator=g, dtype=f),
+            torch.randn(C, N, 3, generator=g, dtype=f), torch.randn(C, N, 3, 3, generator=g, dtype=f))
 
 
 def _autograd_vjp(means, quats, scales, V, K, cot):
@@ -126,7 +130,7 @@ def test_reference_means2d_term_is_summandwise_derivative():
     zero = (vm2, torch.zeros_like(vd), torch.zeros_like(vn), torch.zeros_like(vT))
     radii, T, _ = _autograd_vjp(means, quats, scales, V, K, zero)
     T = T.detach()
-    sg = torch.tensor([1.0, 1.0, -1.0])
+    sg = _t([1.0, 1.0, -1.0])
     # per-entry partial derivatives of mean2d as the reference takes them
     vT_rule = torch.zeros_like(T)
     for j in range(3):

@@ -94,11 +94,48 @@ template <typename T> __device__ __forceinline__ void atomic_add(T *p, T v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sum over the 64 lanes of the wavefront; every lane must be active.
+// DPP lane permutes (gfx9 encodings): quad_perm, row_ror:n, row_bcast:15/31.
+constexpr int kDppQuad1032 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kDppQuad2301 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int kDppRowRor4 = 0x124;
+constexpr int kDppRowRor8 = 0x128;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xF> __device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+template <int CTRL, int ROW_MASK = 0xF> __device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL, int ROW_MASK = 0xF> __device__ __forceinline__ float dpp(float v) { return dpp_f32<CTRL, ROW_MASK>(v); }
+template <int CTRL, int ROW_MASK = 0xF> __device__ __forceinline__ double dpp(double v) { return dpp_f64<CTRL, ROW_MASK>(v); }
+
+__device__ __forceinline__ float readlane63(float v) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double readlane63(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of the wavefront, returned (uniform) in every lane;
+// every lane must be active.  Quad and row rotations leave each 16-lane row
+// holding its row sum; row_bcast:15 then row_bcast:31 fold the rows so lane 63
+// holds the total.
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += dpp<kDppQuad1032>(v);
+    v += dpp<kDppQuad2301>(v);
+    v += dpp<kDppRowRor4>(v);
+    v += dpp<kDppRowRor8>(v);
+    v += dpp<kDppRowBcast15, 0xA>(v);
+    v += dpp<kDppRowBcast31, 0xC>(v);
+    return readlane63(v);
 }
 template <typename T> __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll

@@ -325,7 +325,8 @@ def rasterize_to_pixels_fwd_2dgs_full(means2d, ray_transforms, colors, opacities
 def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals,
                 densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
-                v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None):
+                v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
+                v_textures_out=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -356,7 +357,12 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_means2d = torch.zeros_like(means2d)
         v_rt = torch.zeros_like(ray_transforms)
         v_colors = torch.zeros_like(colors)
-        v_textures = torch.zeros_like(textures) if textured else None
+        if textured and v_textures_out is not None:
+            # caller-provided, already zero-filled (see the textured autograd Function)
+            assert v_textures_out.shape == textures.shape and v_textures_out.dtype == textures.dtype
+            v_textures = v_textures_out
+        else:
+            v_textures = torch.zeros_like(textures) if textured else None
         v_normals = torch.zeros_like(normals)
         v_opacities = torch.zeros_like(opacities)
         v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
@@ -378,21 +384,24 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
     return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_normals, v_densify
 
 
-def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None):
+def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None):
     """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
-    return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth)
+    return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth,
+                       v_textures_out=v_textures_out)
 
 
-def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None):
+def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None):
     """Reference ``bilinear_bwd2`` rule (du/ds = (res-1)/range)."""
-    return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth)
+    return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth,
+                       v_textures_out=v_textures_out)
 
 
-def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None):
+def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None):
     """True derivative: du/ds = (res-1)/(2 range), and the forward's own
     distortion accumulator when ``render_accum_depth`` is given (no reference
     counterpart)."""
-    return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth)
+    return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth,
+                       v_textures_out=v_textures_out)
 
 
 def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
