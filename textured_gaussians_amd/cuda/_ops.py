@@ -241,6 +241,11 @@ def compute_sh_bwd(K, degrees_to_use, dirs, coeffs, masks, v_colors, compute_v_d
 
 
 # --------------------------------------------------------------- rasterizer
+def _raster_workspace(dt, C, N, dev):
+    nbytes = lib().tgs_rasterize_workspace_bytes(dt, C, N)
+    return torch.empty((nbytes,), dtype=torch.uint8, device=dev)
+
+
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
                 g_weight):
@@ -280,6 +285,7 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         csum = torch.zeros((N,), dtype=fdt, device=dev)
         ccnt = torch.zeros((N,), dtype=fdt, device=dev)
         accd = torch.empty((C, H, W), dtype=fdt, device=dev)
+        ws = _raster_workspace(dt, C, N, dev)
         with _timed("raster_fwd", means2d):
             check(lib().tgs_rasterize_fwd(
               dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
@@ -287,7 +293,7 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
               _p(masks.view(torch.uint8)) if masks is not None else None, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
-              _p(csum), _p(ccnt), _p(accd), _stream(means2d)), "rasterize_to_pixels_fwd")
+              _p(csum), _p(ccnt), _p(accd), _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_fwd")
     return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd
 
 
@@ -367,6 +373,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_opacities = torch.zeros_like(opacities)
         v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
         v_densify = torch.zeros_like(densify)
+        ws = _raster_workspace(dt, C, N, dev)
         with _timed("raster_bwd", means2d):
             check(lib().tgs_rasterize_bwd(
               dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
@@ -378,7 +385,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
               _p(v_render_alphas), _p(v_render_normals), _p(v_render_distort), _p(v_render_median), uv_mode,
               _p(v_means2d_abs), _p(v_means2d), _p(v_rt), _p(v_colors), _p(v_opacities), _p(v_textures),
               _p(v_normals), _p(v_densify) if densify.shape == means2d.shape else None,
-              _stream(means2d)), "rasterize_to_pixels_bwd")
+              _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_bwd")
     if textured:
         return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_textures, v_normals, v_densify
     return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_normals, v_densify
