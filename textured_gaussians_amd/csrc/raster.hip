@@ -10,17 +10,32 @@
 // tile (256 for 16x16 = four wave64s, each owning 4 pixel rows).  Every lane of
 // a wavefront processes the same list entry at the same time, so the entry's
 // Gaussian is wave-uniform data: it is read with scalar loads from an 80-byte
-// packed record (mean2d, opacity, ray transform, colour, normal), built by a
-// small packing kernel, straight into SGPRs that the vector ALUs use as
-// operands.  There is no LDS staging of Gaussian batches and no workgroup
-// barrier: each wave walks the tile's depth-sorted list on its own and stops as
-// soon as a ballot shows all its pixels finished.  Finished lanes stay in the
-// loop predicated off so wave reductions (per-Gaussian gradient sums,
-// contribution counters) run with a full exec mask.  The texture of the
-// current Gaussian is addressed as an SGPR base plus 32-bit per-lane texel
-// offsets; one RGBA texel is one 16-byte load.  Workgroups are remapped so each
-// XCD renders one contiguous band of tiles (neighbouring tiles share
-// Gaussians, hence texels, in that XCD's L2).
+// packed record, built by a small packing kernel, straight into SGPRs that the
+// vector ALUs use as operands.  There is no LDS staging of Gaussian batches and
+// no workgroup barrier: each wave walks the tile's depth-sorted list on its own
+// and stops as soon as a ballot shows all its pixels finished.  Finished lanes
+// stay in the loop predicated off so wave reductions (per-Gaussian gradient
+// sums, contribution counters) run with a full exec mask.  Workgroups are
+// remapped so each XCD renders one contiguous band of tiles (neighbouring
+// tiles share Gaussians, hence texels, in that XCD's L2).
+//
+// The loop is bound by vector-ALU issue, and most (pixel, entry) pairs it
+// visits contribute nothing (alpha < 1/255), so the per-pair geometry is kept
+// to a handful of instructions and everything texture-related runs only on
+// lanes inside the texture range:
+//  * the splat-plane cross product of the reference, cr = hu x hv with
+//    hu = px T2 - T0 and hv = py T2 - T1, is affine in the pixel position:
+//    cr = px (T1 x T2) + py (T2 x T0) + (T0 x T1).  The packing kernel stores
+//    the three cross products, so cr costs six FMAs per pixel; the backward
+//    accumulates gradients with respect to those nine coefficients and a
+//    post-pass turns them into ray-transform gradients (cross-product VJP).
+//  * s = cr.xy / cr.z uses one reciprocal; the texel coordinate
+//    u = (s.x + range) / (2 range) (res - 1) is one FMA with host constants.
+//  * texels are addressed as the Gaussian's texture base (SGPRs) plus a 32-bit
+//    per-lane byte offset; one RGBA texel is one 16-byte load.
+// The forward and the backward evaluate a pair through the same helpers with FP
+// contraction disabled and explicit FMAs, so both take identical keep/skip
+// decisions.
 #include "common.h"
 
 namespace tgs {
@@ -30,31 +45,48 @@ namespace tgs {
 template <typename T> using cptr = const __attribute__((address_space(4))) T *;
 template <typename T> __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(uintptr_t)p; }
 
+template <typename S> __device__ __forceinline__ S tfma(S a, S b, S c);
+template <> __device__ __forceinline__ float tfma<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double tfma<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <typename S> __device__ __forceinline__ S trcp(S x);
+template <> __device__ __forceinline__ float trcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double trcp<double>(double x) { return 1.0 / x; }
+
 template <typename S> struct Texel {
     S c[4];
 };
-template <typename S> __device__ __forceinline__ Texel<S> load_texel(const S *p);
-template <> __device__ __forceinline__ Texel<float> load_texel<float>(const float *p) {
-    const float4 v = *(const float4 *)p;
+// RGBA texel at byte offset `off` from the texture base `tb`
+template <typename S> __device__ __forceinline__ Texel<S> load_texel(const S *tb, uint32_t off);
+template <> __device__ __forceinline__ Texel<float> load_texel<float>(const float *tb, uint32_t off) {
+    const float4 v = *(const float4 *)((const char *)tb + off);
     return {{v.x, v.y, v.z, v.w}};
 }
-template <> __device__ __forceinline__ Texel<double> load_texel<double>(const double *p) {
-    const double2 a = *(const double2 *)p;
-    const double2 b = *(const double2 *)(p + 2);
+template <> __device__ __forceinline__ Texel<double> load_texel<double>(const double *tb, uint32_t off) {
+    const double2 a = *(const double2 *)((const char *)tb + off);
+    const double2 b = *(const double2 *)((const char *)tb + off + 16);
     return {{a.x, a.y, b.x, b.y}};
 }
 
-// Packed per-(camera, Gaussian) record: mean2d(2) opacity(1) T(9) colour(4, K used) normal(3) pad(1)
+// Packed per-(camera, Gaussian) record, geometry part then shading part:
+// mean2d(2) A(9) texture row(1) | opacity(1) colour(4, K used) normal(3), with
+// A = (T1 x T2, T2 x T0, T0 x T1) for the ray transform rows T0, T1, T2.  The
+// texture row is the Gaussian id modulo N (int32 bits in float records, an
+// exact double otherwise): textures and contribution counters are indexed per
+// Gaussian, not per flatten id (the reference's flatten-id indexing runs out of
+// bounds for cameras >= 1).
 constexpr int kRec = 20;
-constexpr int kRecOp = 2, kRecT = 3, kRecCol = 12, kRecNrm = 16;
+constexpr int kRecA = 2, kRecRow = 11, kRecOp = 12, kRecCol = 13, kRecNrm = 17;
+constexpr int kAbc = 9;  // backward scratch: gradient w.r.t. A per (camera, Gaussian)
 
 template <typename S> struct RasterArgs {
     int64_t C, N, n_isects;
     const S *means2d, *rt, *colors, *opac, *normals, *textures, *bg;
     const S *rec;  // packed records [C*N][kRec]
     const uint8_t *masks;
-    int64_t tex_h, tex_w;
-    S trx, try_, g_weight, vis_bias, thr;
+    int32_t tex_h, tex_w;
+    int64_t tex_stride;                   // elements per texture (tex_h * tex_w * 4)
+    S trx, try_, su, sv, uh, vh, umax, vmax;  // u = s.x su + uh, v = s.y sv + vh, clamped to [0, umax/vmax]
+    S g_weight, vis_bias, thr;
     int32_t W, H, ts, tw, th;
     const int32_t *offsets, *flatten_ids;
     // forward outputs
@@ -66,45 +98,75 @@ template <typename S> struct RasterArgs {
     const S *r_accd;  // optional: see out_accd; NULL = the reference's seed, render_colors[..., K-1]
     const int32_t *r_last, *r_med;
     int uv_mode;
-    // backward outputs
-    S *g_xy_abs, *g_xy, *g_rt, *g_col, *g_op, *g_tex, *g_nrm;
+    // backward outputs (g_abc: scratch, [C*N][kAbc], turned into g_rt by the post-pass)
+    S *g_xy_abs, *g_xy, *g_rt, *g_col, *g_op, *g_tex, *g_nrm, *g_abc;
+    // optional fused zero-fill of rows of a [N][zrow16 x 16 bytes] buffer with zmask[row] == zsel
+    void *zdst;
+    const uint8_t *zmask;
+    int64_t zrow16;
+    int32_t zsel;
 };
 
+template <typename S> __device__ __forceinline__ S rec_int(int32_t v);
+template <> __device__ __forceinline__ float rec_int<float>(int32_t v) { return __int_as_float(v); }
+template <> __device__ __forceinline__ double rec_int<double>(int32_t v) { return (double)v; }
+__device__ __forceinline__ int32_t rec_int_get(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int32_t rec_int_get(double v) { return (int32_t)v; }
+
 template <typename S, int K>
-__global__ __launch_bounds__(256) void pack_kernel(int64_t n, const S *__restrict__ means2d,
+__global__ __launch_bounds__(256) void pack_kernel(int64_t n, int64_t N, const S *__restrict__ means2d,
                                                    const S *__restrict__ rt, const S *__restrict__ opac,
                                                    const S *__restrict__ colors, const S *__restrict__ normals,
                                                    S *__restrict__ rec) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const v3<S> T0 = mk3(rt[i * 9 + 0], rt[i * 9 + 1], rt[i * 9 + 2]);
+    const v3<S> T1 = mk3(rt[i * 9 + 3], rt[i * 9 + 4], rt[i * 9 + 5]);
+    const v3<S> T2 = mk3(rt[i * 9 + 6], rt[i * 9 + 7], rt[i * 9 + 8]);
+    const v3<S> A[3] = {cross(T1, T2), cross(T2, T0), cross(T0, T1)};
     S r[kRec];
     r[0] = means2d[i * 2];
     r[1] = means2d[i * 2 + 1];
-    r[kRecOp] = opac[i];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) r[kRecT + k] = rt[i * 9 + k];
+    for (int k = 0; k < 3; ++k) {
+        r[kRecA + 3 * k] = A[k].x;
+        r[kRecA + 3 * k + 1] = A[k].y;
+        r[kRecA + 3 * k + 2] = A[k].z;
+    }
+    r[kRecRow] = rec_int<S>((int32_t)(i % N));
+    r[kRecOp] = opac[i];
 #pragma unroll
     for (int k = 0; k < 4; ++k) r[kRecCol + k] = k < K ? colors[i * K + k] : S(0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) r[kRecNrm + k] = normals[i * 3 + k];
-    r[kRec - 1] = 0;
     S *o = rec + i * kRec;
 #pragma unroll
     for (int k = 0; k < kRec; ++k) o[k] = r[k];
 }
 
-// The current entry's Gaussian, in SGPRs.
-template <typename S, int K> struct Gauss {
-    S mx, my, op, T[9], col[K], nrm[3];
+// A list entry's Gaussian, in SGPRs: the part the pair geometry and the texel
+// fetch need, and the part the blend needs.
+template <typename S> struct GaussGeom {
+    S mx, my, A[9];
+    int32_t row;
 };
-template <typename S, int K> __device__ __forceinline__ Gauss<S, K> load_gauss(const RasterArgs<S> &a, int32_t g) {
+template <typename S, int K> struct GaussShade {
+    S op, col[K], nrm[3];
+};
+template <typename S> __device__ __forceinline__ GaussGeom<S> load_geom(const RasterArgs<S> &a, int32_t g) {
     cptr<S> R = as_const(a.rec) + (int64_t)g * kRec;
-    Gauss<S, K> G;
+    GaussGeom<S> G;
     G.mx = R[0];
     G.my = R[1];
-    G.op = R[kRecOp];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) G.T[k] = R[kRecT + k];
+    for (int k = 0; k < 9; ++k) G.A[k] = R[kRecA + k];
+    G.row = rec_int_get(R[kRecRow]);
+    return G;
+}
+template <typename S, int K> __device__ __forceinline__ GaussShade<S, K> load_shade(const RasterArgs<S> &a, int32_t g) {
+    cptr<S> R = as_const(a.rec) + (int64_t)g * kRec;
+    GaussShade<S, K> G;
+    G.op = R[kRecOp];
 #pragma unroll
     for (int k = 0; k < K; ++k) G.col[k] = R[kRecCol + k];
 #pragma unroll
@@ -112,67 +174,38 @@ template <typename S, int K> __device__ __forceinline__ Gauss<S, K> load_gauss(c
     return G;
 }
 
-// Bilinear lookup state for one (pixel, Gaussian) pair; offsets are element
-// offsets of the four texels from the Gaussian's texture base.
-template <typename S> struct Bilerp {
-    bool in_range;
-    S wu, wv;
-    uint32_t o00, o10, o01, o11;
-};
-
-template <typename S> __device__ __forceinline__ Bilerp<S> bilerp_setup(S sx, S sy, const RasterArgs<S> &a) {
-#pragma clang fp contract(off)
-    Bilerp<S> b;
-    // inside [-range, range]^2; NaN coordinates count as outside so no texel
-    // address is ever formed from them
-    b.in_range = (sx >= -a.trx) && (sx <= a.trx) && (sy >= -a.try_) && (sy <= a.try_);
-    const S umax = S(a.tex_w - 1), vmax = S(a.tex_h - 1);
-    S u = (sx + a.trx) / (a.trx * S(2)) * umax;
-    S v = (sy + a.try_) / (a.try_ * S(2)) * vmax;
-    u = u < S(0) ? S(0) : (u > umax ? umax : u);
-    v = v < S(0) ? S(0) : (v > vmax ? vmax : v);
-    if (!b.in_range) { u = 0; v = 0; }
-    const S fu = floor(u), fv = floor(v);
-    const uint32_t u0 = (uint32_t)fu, v0 = (uint32_t)fv;
-    const uint32_t u1 = (uint32_t)ceil(u), v1 = (uint32_t)ceil(v);
-    b.wu = u - fu;
-    b.wv = v - fv;
-    const uint32_t tw = (uint32_t)a.tex_w;
-    b.o00 = (v0 * tw + u0) * 4;
-    b.o10 = (v0 * tw + u1) * 4;
-    b.o01 = (v1 * tw + u0) * 4;
-    b.o11 = (v1 * tw + u1) * 4;
-    return b;
-}
-
 // Per-(pixel, Gaussian) geometry, shared by the forward and the backward so that
-// both take identical keep/skip decisions (FP contraction disabled here).
+// both take identical keep/skip decisions.
 template <typename S> struct PairGeom {
-    v3<S> hu, hv;
-    S cz, sx, sy, w3, w2, dx, dy, sigma, ker, vis;
+    S cz, rz, sx, sy, w3, w2, dx, dy, sigma, ker, vis;
     bool okz, use3;
 };
 
 template <typename S>
-__device__ __forceinline__ PairGeom<S> pair_geom(S px, S py, const S Tm[9], S mx, S my, S vis_bias, S g_weight) {
+__device__ __forceinline__ PairGeom<S> pair_geom(S px, S py, const GaussGeom<S> &G, S vis_bias, S g_weight) {
 #pragma clang fp contract(off)
     PairGeom<S> p;
-    p.hu = mk3(px * Tm[6] - Tm[0], px * Tm[7] - Tm[1], px * Tm[8] - Tm[2]);
-    p.hv = mk3(py * Tm[6] - Tm[3], py * Tm[7] - Tm[4], py * Tm[8] - Tm[5]);
-    p.cz = p.hu.x * p.hv.y - p.hu.y * p.hv.x;
+    const S crx = tfma(px, G.A[0], tfma(py, G.A[3], G.A[6]));
+    const S cry = tfma(px, G.A[1], tfma(py, G.A[4], G.A[7]));
+    p.cz = tfma(px, G.A[2], tfma(py, G.A[5], G.A[8]));
     p.okz = p.cz != S(0);
-    const S czs = p.okz ? p.cz : S(1);
-    p.sx = (p.hu.y * p.hv.z - p.hu.z * p.hv.y) / czs;
-    p.sy = (p.hu.z * p.hv.x - p.hu.x * p.hv.z) / czs;
-    p.w3 = p.sx * p.sx + p.sy * p.sy;
-    p.dx = mx - px;
-    p.dy = my - py;
-    p.w2 = S(2) * (p.dx * p.dx + p.dy * p.dy);
+    p.rz = trcp(p.okz ? p.cz : S(1));
+    p.sx = crx * p.rz;
+    p.sy = cry * p.rz;
+    p.w3 = tfma(p.sx, p.sx, p.sy * p.sy);
+    p.dx = G.mx - px;
+    p.dy = G.my - py;
+    p.w2 = S(2) * tfma(p.dx, p.dx, p.dy * p.dy);
     p.use3 = p.w3 <= p.w2;
     p.sigma = S(0.5) * (p.use3 ? p.w3 : p.w2);
     p.ker = tgs_exp<S>(-p.sigma);
-    p.vis = vis_bias + g_weight * p.ker;
+    p.vis = tfma(g_weight, p.ker, vis_bias);
     return p;
+}
+
+// inside [-range, range]^2; NaN coordinates count as outside
+template <typename S> __device__ __forceinline__ bool in_tex_range(const PairGeom<S> &p, const RasterArgs<S> &a) {
+    return p.okz && (p.sx >= -a.trx) && (p.sx <= a.trx) && (p.sy >= -a.try_) && (p.sy <= a.try_);
 }
 
 template <typename S> __device__ __forceinline__ S pair_alpha(S op, S vis, S a_s) {
@@ -185,43 +218,130 @@ template <typename S> __device__ __forceinline__ bool pair_keep(const PairGeom<S
     return p.okz && !(p.sigma < S(0)) && !(alpha < S(1) / S(255));
 }
 
+// Bilinear lookup state for one in-range (pixel, Gaussian) pair: weights and
+// byte offsets of the four texels from the Gaussian's texture base.
+template <typename S> struct Bilerp {
+    S wu, wv;
+    uint32_t o00, du, dv;  // texel (u0,v0); +du -> (u1,v0); +dv -> (u0,v1)
+};
+
+template <typename S> __device__ __forceinline__ Bilerp<S> bilerp_setup(S sx, S sy, const RasterArgs<S> &a) {
+#pragma clang fp contract(off)
+    Bilerp<S> b;
+    S u = tfma(sx, a.su, a.uh), v = tfma(sy, a.sv, a.vh);
+    u = u < S(0) ? S(0) : (u > a.umax ? a.umax : u);
+    v = v < S(0) ? S(0) : (v > a.vmax ? a.vmax : v);
+    const S fu = floor(u), fv = floor(v);
+    b.wu = u - fu;
+    b.wv = v - fv;
+    const uint32_t tb = 4 * sizeof(S);  // bytes per texel
+    b.o00 = ((uint32_t)fv * (uint32_t)a.tex_w + (uint32_t)fu) * tb;
+    // the reference's second texel is at ceil(u): the next one unless u is integral
+    b.du = b.wu > S(0) ? tb : 0u;
+    b.dv = b.wv > S(0) ? (uint32_t)a.tex_w * tb : 0u;
+    return b;
+}
+
 template <typename S> struct Texels {
     Texel<S> t00, t10, t01, t11;
 };
 
-// Bilinear blend of the four texels: a_s (alpha channel) and the K colour channels.
-template <typename S, int K>
-__device__ __forceinline__ S bilerp_blend(const Bilerp<S> &b, const Texels<S> &t, S tex[K]) {
-#pragma clang fp contract(off)
-    const S w00 = (S(1) - b.wu) * (S(1) - b.wv), w10 = b.wu * (S(1) - b.wv);
-    const S w01 = (S(1) - b.wu) * b.wv, w11 = b.wu * b.wv;
-#pragma unroll
-    for (int k = 0; k < K; ++k) tex[k] = w00 * t.t00.c[k] + w10 * t.t10.c[k] + w01 * t.t01.c[k] + w11 * t.t11.c[k];
-    return w00 * t.t00.c[3] + w10 * t.t10.c[3] + w01 * t.t01.c[3] + w11 * t.t11.c[3];
-}
-
 template <typename S> __device__ __forceinline__ Texels<S> load_texels(const S *tb, const Bilerp<S> &b) {
     Texels<S> t;
-    t.t00 = load_texel(tb + b.o00);
-    t.t10 = load_texel(tb + b.o10);
-    t.t01 = load_texel(tb + b.o01);
-    t.t11 = load_texel(tb + b.o11);
+    t.t00 = load_texel(tb, b.o00);
+    t.t10 = load_texel(tb, b.o00 + b.du);
+    t.t01 = load_texel(tb, b.o00 + b.dv);
+    t.t11 = load_texel(tb, b.o00 + b.du + b.dv);
     return t;
 }
 
+template <typename S> struct BWeights {
+    S w00, w10, w01, w11;
+};
+template <typename S> __device__ __forceinline__ BWeights<S> bweights(const Bilerp<S> &b) {
+#pragma clang fp contract(off)
+    const S ou = S(1) - b.wu, ov = S(1) - b.wv;
+    return {ou * ov, b.wu * ov, ou * b.wv, b.wu * b.wv};
+}
+
+// Bilinear blend of the four texels: a_s (alpha channel) and the K colour channels.
+template <typename S, int K>
+__device__ __forceinline__ S bilerp_blend(const BWeights<S> &w, const Texels<S> &t, S tex[K]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        tex[k] = tfma(w.w11, t.t11.c[k], tfma(w.w01, t.t01.c[k], tfma(w.w10, t.t10.c[k], w.w00 * t.t00.c[k])));
+    return tfma(w.w11, t.t11.c[3], tfma(w.w01, t.t01.c[3], tfma(w.w10, t.t10.c[3], w.w00 * t.t00.c[3])));
+}
+
 // Texture-gradient update of one pixel for one texel row: the two horizontally
-// adjacent texels (u0, v) and (u1, v), RGBA each, 8 values; du = element
-// distance from the first texel to the second (0 when u0 == u1).
+// adjacent texels (u0, v) and (u1, v), RGBA each, 8 values; off = byte offset of
+// the first, du = byte distance to the second (0 when u0 == u1).
 template <typename S> struct TexRec {
-    uint32_t off;
-    int32_t du;
+    uint32_t off, du;
     S v[8];
 };
 
-// Gaussian id -> texture row (the reference indexes textures with the flatten
-// id, which is out of bounds for cameras >= 1).
-template <typename S> __device__ __forceinline__ int64_t gauss_of(const RasterArgs<S> &a, int32_t g) {
-    return a.C == 1 ? (int64_t)g : (int64_t)g % a.N;
+// Fused zero-fill of buffer rows, spread over a kernel's main loop: each wave owns
+// rows w, w + W, w + 2W, ... (w = global wave index, W = waves in the grid) and,
+// per step(), issues one 1 KB store instruction (16 bytes per lane,
+// non-temporal) into the current selected row.  Which of its rows are selected
+// is read 64 at a time (one byte load per lane, a ballot) into a uniform bit
+// mask, so finding the next row is scalar bit arithmetic.  drain() finishes
+// the wave's rows; every wave calls it before exiting.
+typedef uint32_t z32x4 __attribute__((ext_vector_type(4)));
+struct RowZeroer {
+    z32x4 *dst;
+    const uint8_t *mask;
+    uint64_t bits;
+    int32_t N, row16, stride, rb;  // rb: row of bit 0 of `bits`
+    int32_t o, sel;
+    __device__ __forceinline__ void reload(int32_t lane) {
+        // next chunk of 64 rows with at least one selected row (bounded by N / stride chunks)
+        while (bits == 0 && rb < N) {
+            const int64_t r = (int64_t)rb + (int64_t)lane * stride;
+            const bool s = r < N && (int32_t)mask[r] == sel;
+            bits = __ballot(s);
+            if (bits == 0) rb = (int32_t)min((int64_t)N, (int64_t)rb + 64 * (int64_t)stride);
+        }
+    }
+    __device__ __forceinline__ void init(void *d, const uint8_t *m, int64_t n, int64_t r16, int32_t selv, int32_t wave,
+                                         int32_t waves, int32_t lane) {
+        dst = (z32x4 *)d;
+        mask = m;
+        N = d ? (int32_t)n : 0;
+        row16 = (int32_t)r16;
+        stride = waves;
+        rb = wave;
+        bits = 0;
+        o = 0;
+        sel = selv;
+        reload(lane);
+    }
+    __device__ __forceinline__ void step(int32_t lane) {
+        if (bits == 0) return;
+        const int64_t r = (int64_t)rb + (int64_t)__builtin_ctzll(bits) * stride;
+        const int32_t k = o + lane;
+        if (k < row16) __builtin_nontemporal_store(z32x4{0, 0, 0, 0}, dst + r * row16 + k);
+        o += kWave;
+        if (o >= row16) {
+            o = 0;
+            bits &= bits - 1;
+            if (bits == 0) {
+                rb = (int32_t)min((int64_t)N, (int64_t)rb + 64 * (int64_t)stride);
+                reload(lane);
+            }
+        }
+    }
+    __device__ __forceinline__ void drain(int32_t lane) {
+        while (bits != 0) step(lane);
+    }
+};
+
+template <typename S> __device__ __forceinline__ void zeroer_init(RowZeroer &z, const RasterArgs<S> &a, int lane) {
+    const int32_t wpb = blockDim.x / kWave;
+    z.init(a.zdst, a.zmask, a.N, a.zrow16, a.zsel, (int32_t)(blockIdx.x * wpb + threadIdx.x / kWave),
+           (int32_t)(gridDim.x * wpb), lane);
 }
 
 // ------------------------------------------------------------------ forward
@@ -238,6 +358,8 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
     const bool inside = i < a.H && j < a.W;
     const int64_t pix = (c * a.H + i) * a.W + j;
     const S px = S(j) + S(0.5), py = S(i) + S(0.5);
+    RowZeroer zr;
+    zeroer_init(zr, a, lane);
 
     if (a.masks && !as_const(a.masks)[tile_lin]) {
         if (inside) {
@@ -252,9 +374,11 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
             a.median_ids[pix] = 0;
             if (a.out_accd) a.out_accd[pix] = 0;
         }
+        zr.drain(lane);
         return;
     }
 
+    cptr<int32_t> ids = as_const(a.flatten_ids);
     const int32_t start = as_const(a.offsets)[tile_lin];
     const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : as_const(a.offsets)[tile_lin + 1];
 
@@ -267,48 +391,45 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 
     for (int32_t idx = start; idx < end; ++idx) {
         if (__ballot(!done) == 0) break;  // every pixel of this wave is finished
-        const int32_t g = as_const(a.flatten_ids)[idx];
-        const Gauss<S, K> G = load_gauss<S, K>(a, g);
+        zr.step(lane);
+        const int32_t g = ids[idx];
+        const GaussGeom<S> G = load_geom<S>(a, g);
+        const GaussShade<S, K> H = load_shade<S, K>(a, g);
+        const PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
+        S a_s = 1, tex[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) tex[k] = 0;
+        if constexpr (TEX) {
+            if (!done && in_tex_range(p, a)) {
+                const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
+                a_s = bilerp_blend<S, K>(bweights(bl), load_texels(a.textures + (int64_t)G.row * a.tex_stride, bl), tex);
+            }
+        }
+        const S alpha = pair_alpha(H.op, p.vis, a_s);
         S contrib = 0;
         bool counted = false;
-        if (!done) {
-            const PairGeom<S> p = pair_geom<S>(px, py, G.T, G.mx, G.my, a.vis_bias, a.g_weight);
-            if (p.okz) {
-                S a_s = 1, tex[K];
+        if (!done && pair_keep(p, alpha)) {
+            const S nT = T * (S(1) - alpha);
+            if (nT <= S(1e-4)) {
+                done = true;  // this entry is excluded
+            } else {
+                const S w = alpha * T;
 #pragma unroll
-                for (int k = 0; k < K; ++k) tex[k] = 0;
-                if constexpr (TEX) {
-                    const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
-                    if (bl.in_range) {
-                        const S *tb = a.textures + gauss_of(a, g) * a.tex_h * a.tex_w * 4;
-                        a_s = bilerp_blend<S, K>(bl, load_texels(tb, bl), tex);
-                    }
+                for (int k = 0; k < K; ++k) pc[k] += (H.col[k] + tex[k]) * w;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pn[k] += H.nrm[k] * w;
+                const S depth = H.col[K - 1];
+                distort += S(2) * (w * depth * (S(1) - T) - w * acc_vd);
+                acc_vd += w * depth;
+                if (T > S(0.5)) {
+                    median = depth;
+                    median_id = idx;
                 }
-                const S alpha = pair_alpha(G.op, p.vis, a_s);
-                if (pair_keep(p, alpha)) {
-                    const S nT = T * (S(1) - alpha);
-                    if (nT <= S(1e-4)) {
-                        done = true;  // this entry is excluded
-                    } else {
-                        const S w = alpha * T;
-#pragma unroll
-                        for (int k = 0; k < K; ++k) pc[k] += (G.col[k] + tex[k]) * w;
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) pn[k] += G.nrm[k] * w;
-                        const S depth = G.col[K - 1];
-                        distort += S(2) * (w * depth * (S(1) - T) - w * acc_vd);
-                        acc_vd += w * depth;
-                        if (T > S(0.5)) {
-                            median = depth;
-                            median_id = idx;
-                        }
-                        cur = idx;
-                        T = nT;
-                        if (alpha > a.thr) {
-                            contrib = alpha;
-                            counted = true;
-                        }
-                    }
+                cur = idx;
+                T = nT;
+                if (alpha > a.thr) {
+                    contrib = alpha;
+                    counted = true;
                 }
             }
         }
@@ -317,12 +438,12 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
         if (cm) {
             const S sum = wave_sum(contrib);
             if (lane == 0) {
-                const int64_t n = gauss_of(a, g);
-                atomic_add(a.csum + n, sum);
-                atomic_add(a.ccnt + n, S(__popcll(cm)));
+                atomic_add(a.csum + G.row, sum);
+                atomic_add(a.ccnt + G.row, S(__popcll(cm)));
             }
         }
     }
+    zr.drain(lane);
     if (inside) {
         a.out_a[pix] = S(1) - T;
 #pragma unroll
@@ -348,15 +469,21 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
     const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t c = tile_lin / tiles_per_cam;
     const int64_t tid = tile_lin % tiles_per_cam;
-    if (a.masks && !as_const(a.masks)[tile_lin]) return;
-    const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
     const int r = threadIdx.x;
     const int lane = r & (kWave - 1);
+    RowZeroer zr;
+    zeroer_init(zr, a, lane);
+    if (a.masks && !as_const(a.masks)[tile_lin]) {
+        zr.drain(lane);
+        return;
+    }
+    const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
     const int i = ty * a.ts + r / a.ts, j = tx * a.ts + r % a.ts;
     const bool inside = i < a.H && j < a.W;
     const int64_t pix = inside ? (c * a.H + i) * a.W + j : (c * a.H + a.H - 1) * a.W + a.W - 1;
     const S px = S(j) + S(0.5), py = S(i) + S(0.5);
 
+    cptr<int32_t> ids = as_const(a.flatten_ids);
     const int32_t start = as_const(a.offsets)[tile_lin];
     const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : as_const(a.offsets)[tile_lin + 1];
 
@@ -394,49 +521,52 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
     }
     // the wave starts at the last entry any of its pixels used
     const int32_t wave_last = __builtin_amdgcn_readfirstlane(wave_max(inside ? bin_final : int32_t(-1)));
-    S du_ds = 0, dv_ds = 0;
+    S ds_u = 0, ds_v = 0;  // d(u)/d(s.x), d(v)/d(s.y) under the selected rule
     if (TEX && a.uv_mode == TGS_UV_EXACT) {
-        du_ds = S(a.tex_w - 1) / (S(2) * a.trx);
-        dv_ds = S(a.tex_h - 1) / (S(2) * a.try_);
+        ds_u = a.su;
+        ds_v = a.sv;
     } else if (TEX && a.uv_mode == TGS_UV_DOUBLE) {
-        du_ds = S(a.tex_w - 1) / a.trx;
-        dv_ds = S(a.tex_h - 1) / a.try_;
+        ds_u = S(2) * a.su;
+        ds_v = S(2) * a.sv;
     }
     const bool uv_term = TEX && a.uv_mode != TGS_UV_NONE;
     const int32_t first = min(end - 1, wave_last);
 
     for (int32_t idx = first; idx >= start; --idx) {
-        const int32_t g = as_const(a.flatten_ids)[idx];
-        const Gauss<S, K> G = load_gauss<S, K>(a, g);
-        const PairGeom<S> p = pair_geom<S>(px, py, G.T, G.mx, G.my, a.vis_bias, a.g_weight);
+        zr.step(lane);
+        const int32_t g = ids[idx];
+        const GaussGeom<S> G = load_geom<S>(a, g);
+        const GaussShade<S, K> H = load_shade<S, K>(a, g);
+        const PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
         bool valid = inside && idx <= bin_final && p.okz;
         S a_s = 1, tex[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) tex[k] = 0;
         Bilerp<S> bl;
+        BWeights<S> bw;
         Texels<S> tt;
-        bl.in_range = false;
+        bool inr = false;
         const S *tb = nullptr;
         if constexpr (TEX) {
-            tb = a.textures + gauss_of(a, g) * a.tex_h * a.tex_w * 4;
-            if (valid) {
+            tb = a.textures + (int64_t)G.row * a.tex_stride;
+            inr = valid && in_tex_range(p, a);
+            if (inr) {
                 bl = bilerp_setup<S>(p.sx, p.sy, a);
-                if (bl.in_range) {
-                    tt = load_texels(tb, bl);
-                    a_s = bilerp_blend<S, K>(bl, tt, tex);
-                }
+                bw = bweights(bl);
+                tt = load_texels(tb, bl);
+                a_s = bilerp_blend<S, K>(bw, tt, tex);
             }
         }
-        const S alpha = pair_alpha(G.op, p.vis, a_s);
+        const S alpha = pair_alpha(H.op, p.vis, a_s);
         valid = valid && pair_keep(p, alpha);
         if (__ballot(valid) == 0) continue;  // wave-uniform
 
         bool tex_live = false;
-        S g_col[K], g_n[3] = {0, 0, 0}, g_T[9], g_xy[2] = {0, 0}, g_abs[2] = {0, 0}, g_op = 0;
+        S g_col[K], g_n[3] = {0, 0, 0}, g_A[9], g_xy[2] = {0, 0}, g_abs[2] = {0, 0}, g_op = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) g_col[k] = 0;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) g_T[k] = 0;
+        for (int k = 0; k < 9; ++k) g_A[k] = 0;
         if (valid) {
             if (idx == median_idx) g_col[K - 1] += vmed;
             const S ra = S(1) / (S(1) - alpha);
@@ -446,17 +576,17 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 g_col[k] += fac * vc[k];
-                v_alpha += ((G.col[k] + tex[k]) * T - buf[k] * ra) * vc[k];
+                v_alpha += ((H.col[k] + tex[k]) * T - buf[k] * ra) * vc[k];
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 g_n[k] = fac * vn[k];
-                v_alpha += (G.nrm[k] * T - bufn[k] * ra) * vn[k];
+                v_alpha += (H.nrm[k] * T - bufn[k] * ra) * vn[k];
             }
             v_alpha += T_final * ra * va;
             v_alpha += -T_final * ra * bg_dot;
             if (has_d) {
-                const S depth = G.col[K - 1];
+                const S depth = H.col[K - 1];
                 const S dl_dw = S(2) * (S(2) * (depth * acc_w_buf - acc_d_buf) + (acc_d - depth * acc_w));
                 v_alpha += (dl_dw * T - dist_buf * ra) * vdist;
                 acc_d_buf -= fac * depth;
@@ -464,10 +594,10 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
                 dist_buf += dl_dw * fac;
                 g_col[K - 1] += S(2) * fac * (S(2) - S(2) * T - acc_w + fac) * vdist;
             }
-            const bool unclamped = G.op * p.vis * a_s <= S(0.999);
+            const bool unclamped = H.op * p.vis * a_s <= S(0.999);
             S vsx = 0, vsy = 0;
             if (unclamped) {
-                const S v_G = G.op * v_alpha * a_s;
+                const S v_G = H.op * v_alpha * a_s;
                 const S dk = -a.g_weight * p.ker;  // d(vis)/d(sigma)
                 if (p.use3) {
                     vsx = v_G * dk * p.sx;
@@ -481,12 +611,10 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
                 g_op = p.vis * v_alpha * a_s;
             }
             if constexpr (TEX) {
-                if (bl.in_range) {
-                    const S w00 = (S(1) - bl.wu) * (S(1) - bl.wv), w10 = bl.wu * (S(1) - bl.wv);
-                    const S w01 = (S(1) - bl.wu) * bl.wv, w11 = bl.wu * bl.wv;
+                if (inr) {
                     // d(loss)/d(sample) per channel: colour channels k < K, alpha channel 3
                     // (for K = 4 channel 3 carries both, as in the forward's sum)
-                    const S v_as = unclamped ? p.vis * G.op * v_alpha : S(0);
+                    const S v_as = unclamped ? p.vis * H.op * v_alpha : S(0);
                     S q[4] = {0, 0, 0, 0};
 #pragma unroll
                     for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
@@ -494,15 +622,15 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
                     TexRec<S> &r0 = stage[lane];
                     TexRec<S> &r1 = stage[kWave + lane];
                     r0.off = bl.o00;
-                    r0.du = (int32_t)(bl.o10 - bl.o00);
-                    r1.off = bl.o01;
-                    r1.du = (int32_t)(bl.o11 - bl.o01);
+                    r0.du = bl.du;
+                    r1.off = bl.o00 + bl.dv;
+                    r1.du = bl.du;
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        r0.v[k] = w00 * q[k];
-                        r0.v[4 + k] = w10 * q[k];
-                        r1.v[k] = w01 * q[k];
-                        r1.v[4 + k] = w11 * q[k];
+                        r0.v[k] = bw.w00 * q[k];
+                        r0.v[4 + k] = bw.w10 * q[k];
+                        r1.v[k] = bw.w01 * q[k];
+                        r1.v[4 + k] = bw.w11 * q[k];
                     }
                     tex_live = true;
                     if (uv_term) {
@@ -517,39 +645,35 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
                         }
                         gu += v_as * ((S(1) - bl.wv) * (tt.t10.c[3] - tt.t00.c[3]) + bl.wv * (tt.t11.c[3] - tt.t01.c[3]));
                         gv += v_as * ((S(1) - bl.wu) * (tt.t01.c[3] - tt.t00.c[3]) + bl.wu * (tt.t11.c[3] - tt.t10.c[3]));
-                        vsx += gu * du_ds;
-                        vsy += gv * dv_ds;
+                        vsx += gu * ds_u;
+                        vsy += gv * ds_v;
                     }
                 }
             }
             if (vsx != S(0) || vsy != S(0)) {
-                // s = (cr.x, cr.y) / cr.z, cr = hu x hv, hu = px T2 - T0, hv = py T2 - T1
-                const S ax = vsx / p.cz, ay = vsy / p.cz;
-                const v3<S> vcr = mk3(ax, ay, -(ax * p.sx + ay * p.sy));
-                const v3<S> vhu = cross(p.hv, vcr);
-                const v3<S> vhv = cross(vcr, p.hu);
-                g_T[0] = -vhu.x; g_T[1] = -vhu.y; g_T[2] = -vhu.z;
-                g_T[3] = -vhv.x; g_T[4] = -vhv.y; g_T[5] = -vhv.z;
-                g_T[6] = px * vhu.x + py * vhv.x;
-                g_T[7] = px * vhu.y + py * vhv.y;
-                g_T[8] = px * vhu.z + py * vhv.z;
+                // s = cr.xy / cr.z with cr = px A0 + py A1 + A2
+                const S ax = vsx * p.rz, ay = vsy * p.rz;
+                const S az = -(ax * p.sx + ay * p.sy);
+                g_A[0] = px * ax; g_A[1] = px * ay; g_A[2] = px * az;
+                g_A[3] = py * ax; g_A[4] = py * ay; g_A[5] = py * az;
+                g_A[6] = ax; g_A[7] = ay; g_A[8] = az;
             }
 #pragma unroll
-            for (int k = 0; k < K; ++k) buf[k] += (tex[k] + G.col[k]) * fac;
+            for (int k = 0; k < K; ++k) buf[k] += (tex[k] + H.col[k]) * fac;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) bufn[k] += G.nrm[k] * fac;
+            for (int k = 0; k < 3; ++k) bufn[k] += H.nrm[k] * fac;
         }
         if constexpr (TEX) {
             // Texture-gradient flush, transposed through LDS: in each round 8 consecutive
             // lanes add the 8 values of one pixel's texel row (two adjacent RGBA texels,
-            // 32 contiguous bytes), so one atomic instruction touches 8 such runs instead
-            // of 64 scattered dwords.
+            // contiguous), so one atomic instruction touches 8 such runs instead of 64
+            // scattered values.
             const uint64_t live = __ballot(tex_live);
             if (live) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                S *gtb = a.g_tex + gauss_of(a, g) * a.tex_h * a.tex_w * 4;
+                S *gtb = a.g_tex + (int64_t)G.row * a.tex_stride;
                 const int sub = lane >> 3, e = lane & 7;
 #pragma unroll 1
                 for (int grp = 0; grp < 8; ++grp) {
@@ -560,7 +684,8 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
                         for (int row = 0; row < 2; ++row) {
                             const TexRec<S> &rr = stage[row * kWave + src];
                             const S val = rr.v[e];
-                            if (val != S(0)) atomic_add(gtb + rr.off + (e >> 2) * rr.du + (e & 3), val);
+                            const uint32_t off = rr.off + (e >> 2) * rr.du + (e & 3) * (uint32_t)sizeof(S);
+                            if (val != S(0)) atomic_add((S *)((char *)gtb + off), val);
                         }
                     }
                 }
@@ -573,7 +698,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
 #pragma unroll
         for (int k = 0; k < 3; ++k) g_n[k] = wave_sum(g_n[k]);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) g_T[k] = wave_sum(g_T[k]);
+        for (int k = 0; k < 9; ++k) g_A[k] = wave_sum(g_A[k]);
         g_xy[0] = wave_sum(g_xy[0]);
         g_xy[1] = wave_sum(g_xy[1]);
         g_op = wave_sum(g_op);
@@ -586,8 +711,8 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
             const int64_t gg = g;
             S v = 0;
 #pragma unroll
-            for (int f = 0; f < 9; ++f) v = lane == f ? g_T[f] : v;
-            if (lane < 9 && v != S(0)) atomic_add(a.g_rt + gg * 9 + lane, v);
+            for (int f = 0; f < 9; ++f) v = lane == f ? g_A[f] : v;
+            if (lane < 9 && v != S(0)) atomic_add(a.g_abc + gg * kAbc + lane, v);
             v = 0;
 #pragma unroll
             for (int f = 0; f < K; ++f) v = lane == f ? g_col[f] : v;
@@ -603,27 +728,53 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
             if (lane == 0 && g_op != S(0)) atomic_add(a.g_op + gg, g_op);
         }
     }
+    zr.drain(lane);
 }
 
 template <typename S>
-__global__ __launch_bounds__(256) void densify_kernel(int64_t n, const S *__restrict__ rt, const S *__restrict__ g_rt,
-                                                      S *__restrict__ v_densify) {
+__global__ __launch_bounds__(256) void zero_kernel(int64_t n, S *__restrict__ p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Post-pass of the backward: the gradient with respect to A = (T1 x T2, T2 x T0,
+// T0 x T1) becomes the ray-transform gradient (for z = x X y: dL/dx = y X dL/dz,
+// dL/dy = dL/dz X x), added to v_ray_transforms; then the densification gradient
+// v_densify = (dL/dT[0][2], dL/dT[1][2]) * T[2][2] from the settled value.
+template <typename S>
+__global__ __launch_bounds__(256) void rt_grad_kernel(int64_t n, const S *__restrict__ rt, const S *__restrict__ g_abc,
+                                                      S *__restrict__ g_rt, S *__restrict__ v_densify) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const S depth = rt[i * 9 + 8];
-    v_densify[i * 2] = g_rt[i * 9 + 2] * depth;
-    v_densify[i * 2 + 1] = g_rt[i * 9 + 5] * depth;
+    const S *T = rt + i * 9;
+    const S *q = g_abc + i * kAbc;
+    const v3<S> T0 = mk3(T[0], T[1], T[2]), T1 = mk3(T[3], T[4], T[5]), T2 = mk3(T[6], T[7], T[8]);
+    const v3<S> va = mk3(q[0], q[1], q[2]), vb = mk3(q[3], q[4], q[5]), vc = mk3(q[6], q[7], q[8]);
+    const v3<S> v0 = cross(vb, T2) + cross(T1, vc);
+    const v3<S> v1 = cross(T2, va) + cross(vc, T0);
+    const v3<S> v2 = cross(va, T1) + cross(T0, vb);
+    S *o = g_rt + i * 9;
+    const S d[9] = {v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] += d[k];
+    if (v_densify) {
+        v_densify[i * 2] = o[2] * T[8];
+        v_densify[i * 2 + 1] = o[5] * T[8];
+    }
 }
 
 template <typename S> static size_t rec_bytes(int64_t C, int64_t N) {
     return (((size_t)(C * N) * kRec * sizeof(S)) + 255) / 256 * 256;
 }
+template <typename S> static size_t abc_bytes(int64_t C, int64_t N) {
+    return (((size_t)(C * N) * kAbc * sizeof(S)) + 255) / 256 * 256;
+}
 
 template <typename S, int K>
 static void launch_pack(const RasterArgs<S> &a, hipStream_t st) {
     const int64_t n = a.C * a.N;
-    pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.means2d, a.rt, a.opac, a.colors, a.normals,
-                                                                    (S *)a.rec);
+    pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.N, a.means2d, a.rt, a.opac, a.colors,
+                                                                    a.normals, (S *)a.rec);
 }
 template <typename S, int K, bool TEX> static void launch_fwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
     launch_pack<S, K>(a, st);
@@ -639,7 +790,9 @@ template <typename S, bool BWD>
 static int dispatch(const RasterArgs<S> &a, int K, bool tex, hipStream_t st) {
     const unsigned grid = (unsigned)(a.C * a.tw * a.th);
     const int bs = a.ts * a.ts;
-    if (grid == 0) return TGS_OK;
+    if (grid == 0) {  // no kernel to fuse the row fill into
+        return a.zdst ? tgs_zero_fill_rows(a.zdst, (size_t)a.zrow16 * 16, a.N, a.zmask, a.zsel, st) : TGS_OK;
+    }
 #define TGS_CASE(KK)                                                   \
     case KK:                                                           \
         if (BWD) {                                                     \
@@ -669,15 +822,28 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
                         const void *colors, const void *opac, const void *normals, const void *textures,
                         int64_t tex_h, int64_t tex_w, double trx, double try_, const void *bg, const uint8_t *masks,
                         int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th, const int32_t *offsets,
-                        const int32_t *flatten_ids, int64_t n_isects, double g_weight, void *workspace) {
+                        const int32_t *flatten_ids, int64_t n_isects, double g_weight, void *workspace,
+                        void *zdst, const uint8_t *zmask, size_t zrow_bytes, int zsel) {
     memset(&a, 0, sizeof(a));
+    a.zdst = zdst;
+    a.zmask = zmask;
+    a.zrow16 = (int64_t)(zrow_bytes / 16);
+    a.zsel = zsel;
     a.C = C; a.N = N; a.n_isects = n_isects;
     a.means2d = (const S *)means2d; a.rt = (const S *)rt; a.colors = (const S *)colors;
     a.opac = (const S *)opac; a.normals = (const S *)normals; a.textures = (const S *)textures;
     a.bg = (const S *)bg; a.masks = masks;
     a.rec = (const S *)workspace;
-    a.tex_h = tex_h; a.tex_w = tex_w; a.trx = (S)trx; a.try_ = (S)try_;
+    a.g_abc = (S *)((char *)workspace + rec_bytes<S>(C, N));
     if (textures) {
+        a.tex_h = (int32_t)tex_h; a.tex_w = (int32_t)tex_w;
+        a.tex_stride = tex_h * tex_w * 4;
+        a.trx = (S)trx; a.try_ = (S)try_;
+        a.umax = (S)(tex_w - 1); a.vmax = (S)(tex_h - 1);
+        a.su = (S)((double)(tex_w - 1) / (2.0 * trx));
+        a.sv = (S)((double)(tex_h - 1) / (2.0 * try_));
+        a.uh = (S)(0.5 * (double)(tex_w - 1));
+        a.vh = (S)(0.5 * (double)(tex_h - 1));
         a.g_weight = (S)g_weight;
         a.vis_bias = S(0.998) - (S)g_weight;
     } else {
@@ -689,20 +855,29 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
 }
 
 static int check_common(int64_t C, int64_t N, int32_t K, const void *textures, int64_t tex_h, int64_t tex_w,
-                        int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th, int64_t n_isects) {
+                        double trx, double try_, int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th,
+                        int64_t n_isects) {
     TGS_CHECK_ARG(C >= 0 && N > 0 && C * N < (int64_t(1) << 31), "rasterize: bad sizes C=%lld N=%lld",
                   (long long)C, (long long)N);
     TGS_CHECK_ARG(K >= 1 && K <= 4, "rasterize: Unsupported number of channels: %d", K);
     TGS_CHECK_ARG(ts == 8 || ts == 16, "rasterize: tile_size %d unsupported (8 or 16)", ts);
     TGS_CHECK_ARG((int64_t)tw * ts >= W && (int64_t)th * ts >= H, "rasterize: tile grid does not cover the image");
     TGS_CHECK_ARG(n_isects >= 0 && n_isects < (int64_t(1) << 31), "rasterize: bad n_isects");
-    TGS_CHECK_ARG(!textures || (tex_h >= 1 && tex_w >= 1 && tex_h * tex_w * 4 < (int64_t(1) << 31)),
+    TGS_CHECK_ARG(!textures || (tex_h >= 1 && tex_w >= 1 && tex_h * tex_w * 4 * 8 < (int64_t(1) << 31)),
                   "rasterize: bad texture size");
+    TGS_CHECK_ARG(!textures || (trx > 0 && try_ > 0), "rasterize: texture ranges must be positive");
+    return TGS_OK;
+}
+
+static int check_zero_rows(const void *dst, const uint8_t *mask, size_t row_bytes) {
+    TGS_CHECK_ARG(!dst || (mask && ((uintptr_t)dst % 16) == 0 && row_bytes > 0 && row_bytes % 16 == 0),
+                  "rasterize: zero_rows_dst needs a mask, 16-byte alignment and rows of a multiple of 16 bytes");
     return TGS_OK;
 }
 
 static size_t workspace_bytes(int dtype, int64_t C, int64_t N) {
-    return dtype == TGS_F64 ? rec_bytes<double>(C, N) : rec_bytes<float>(C, N);
+    return dtype == TGS_F64 ? rec_bytes<double>(C, N) + abc_bytes<double>(C, N)
+                            : rec_bytes<float>(C, N) + abc_bytes<float>(C, N);
 }
 
 }  // namespace tgs
@@ -720,20 +895,24 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const int32_t *flatten_ids, int64_t n_isects, double gs_contrib_threshold,
                                  double g_weight, void *render_colors, void *render_alphas, void *render_normals,
                                  void *render_distort, void *render_median, int32_t *last_ids, int32_t *median_ids,
-                                 void *contrib_sum, void *contrib_count, void *render_accum_depth, void *workspace,
-                                 size_t workspace_size, tgs_stream_t stream) {
-    int rc = check_common(C, N, K, textures, tex_h, tex_w, width, height, tile_size, tile_width, tile_height,
-                          n_isects);
+                                 void *contrib_sum, void *contrib_count, void *render_accum_depth,
+                                 void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
+                                 void *workspace, size_t workspace_size, tgs_stream_t stream) {
+    int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
+                          tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
     TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, C, N),
                   "rasterize_fwd: workspace of %zu bytes required", workspace_bytes(dtype, C, N));
+    rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
 #define TGS_FWD(S)                                                                                              \
     {                                                                                                           \
         RasterArgs<S> a;                                                                                        \
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
-                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace);      \
+                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
+                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 1);                                       \
         a.thr = (S)gs_contrib_threshold;                                                                        \
         a.out_c = (S *)render_colors; a.out_a = (S *)render_alphas; a.out_n = (S *)render_normals;              \
         a.out_d = (S *)render_distort; a.out_m = (S *)render_median; a.last_ids = last_ids;                     \
@@ -761,22 +940,26 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const void *v_render_normals, const void *v_render_distort,
                                  const void *v_render_median, int uv_grad_mode, void *v_means2d_abs, void *v_means2d,
                                  void *v_ray_transforms, void *v_colors, void *v_opacities, void *v_textures,
-                                 void *v_normals, void *v_densify, void *workspace, size_t workspace_size,
-                                 tgs_stream_t stream) {
-    int rc = check_common(C, N, K, textures, tex_h, tex_w, width, height, tile_size, tile_width, tile_height,
-                          n_isects);
+                                 void *v_normals, void *v_densify, void *zero_rows_dst,
+                                 const uint8_t *zero_rows_mask, size_t zero_row_bytes, void *workspace,
+                                 size_t workspace_size, tgs_stream_t stream) {
+    int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
+                          tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
     TGS_CHECK_ARG(uv_grad_mode >= 0 && uv_grad_mode <= 2, "rasterize_bwd: bad uv_grad_mode %d", uv_grad_mode);
     TGS_CHECK_ARG(!textures || v_textures, "rasterize_bwd: v_textures required with textures");
     TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, C, N),
                   "rasterize_bwd: workspace of %zu bytes required", workspace_bytes(dtype, C, N));
+    rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
 #define TGS_BWD(S)                                                                                              \
     {                                                                                                           \
         RasterArgs<S> a;                                                                                        \
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
-                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace);      \
+                       tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
+                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 0);                                       \
         a.r_c = (const S *)render_colors; a.r_a = (const S *)render_alphas; a.r_last = last_ids;                \
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \
@@ -784,17 +967,19 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.v_m = (const S *)v_render_median; a.uv_mode = uv_grad_mode;                                           \
         a.g_xy_abs = (S *)v_means2d_abs; a.g_xy = (S *)v_means2d; a.g_rt = (S *)v_ray_transforms;              \
         a.g_col = (S *)v_colors; a.g_op = (S *)v_opacities; a.g_tex = (S *)v_textures; a.g_nrm = (S *)v_normals; \
+        const int64_t n = C * N;                                                                                \
+        const unsigned nb = (unsigned)ceil_div(n, 256);                                                         \
+        zero_kernel<S><<<(unsigned)ceil_div(n * kAbc, 256), 256, 0, st>>>(n * kAbc, a.g_abc);                   \
+        TGS_CHECK_LAUNCH("rasterize_bwd zero");                                                                 \
         if (n_isects > 0) {                                                                                     \
             rc = dispatch<S, true>(a, K, textures != nullptr, st);                                              \
-            if (rc) return rc;                                                                                  \
+        } else if (zero_rows_dst) {                                                                             \
+            rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, 0, st);                   \
         }                                                                                                       \
-        if (v_densify) {                                                                                        \
-            const int64_t n = C * N;                                                                            \
-            densify_kernel<S><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, (const S *)ray_transforms,         \
-                                                                          (const S *)v_ray_transforms,          \
-                                                                          (S *)v_densify);                      \
-            TGS_CHECK_LAUNCH("rasterize_bwd densify");                                                          \
-        }                                                                                                       \
+        if (rc) return rc;                                                                                      \
+        rt_grad_kernel<S><<<nb, 256, 0, st>>>(n, (const S *)ray_transforms, a.g_abc, (S *)v_ray_transforms,     \
+                                              (S *)v_densify);                                                  \
+        TGS_CHECK_LAUNCH("rasterize_bwd ray-transform gradient");                                               \
         return TGS_OK;                                                                                          \
     }
     if (dtype == TGS_F32) TGS_BWD(float)

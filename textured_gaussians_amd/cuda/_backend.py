@@ -29,6 +29,9 @@ vp = ctypes.c_void_p
 _PROTOS = {
     "tgs_last_error": (ctypes.c_char_p, []),
     "tgs_abi_version": (c_int, []),
+    "tgs_zero_fill": (c_int, [vp, c_size, vp]),
+    "tgs_mark_rows": (c_int, [c_i64, vp, c_i64, vp, vp]),
+    "tgs_zero_fill_rows": (c_int, [vp, c_size, c_i64, vp, c_int, vp]),
     "tgs_projection_2dgs_fwd": (c_int, [c_int, c_i64, c_i64, vp, vp, vp, vp, vp, c_i32, c_i32, c_dbl, c_dbl,
                                         c_dbl, vp, vp, vp, vp, vp, vp]),
     "tgs_projection_2dgs_bwd": (c_int, [c_int, c_i64, c_i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
@@ -43,12 +46,12 @@ _PROTOS = {
     "tgs_sh_bwd": (c_int, [c_int, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
     "tgs_rasterize_fwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
                                   vp, vp, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl, c_dbl,
-                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp]),
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
     "tgs_rasterize_bwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
                                   vp, vp, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
-                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp]),
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),
     "tgs_selective_adam_update": (c_int, [c_i64, c_i64, vp, vp, vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, vp]),
 }
 

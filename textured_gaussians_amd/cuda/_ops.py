@@ -246,9 +246,20 @@ def _raster_workspace(dt, C, N, dev):
     return torch.empty((nbytes,), dtype=torch.uint8, device=dev)
 
 
+def _zero_rows_args(zero_rows, N):
+    """(dst ptr, mask ptr, row bytes) for the fused row fill, or nulls."""
+    if zero_rows is None:
+        return None, None, 0
+    dst, mask = zero_rows
+    _check_input(dst, "zero_rows dst")
+    _check_input(mask, "zero_rows mask")
+    assert dst.shape[0] == N and mask.numel() >= N and mask.dtype == torch.uint8
+    return _p(dst), _p(mask), dst[0].numel() * dst.element_size()
+
+
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
-                g_weight):
+                g_weight, zero_rows=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, tile_offsets=tile_offsets, flatten_ids=flatten_ids)
     if textured:
@@ -286,6 +297,7 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         ccnt = torch.zeros((N,), dtype=fdt, device=dev)
         accd = torch.empty((C, H, W), dtype=fdt, device=dev)
         ws = _raster_workspace(dt, C, N, dev)
+        zd, zm, zb = _zero_rows_args(zero_rows, N)
         with _timed("raster_fwd", means2d):
             check(lib().tgs_rasterize_fwd(
               dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
@@ -293,7 +305,8 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
               _p(masks.view(torch.uint8)) if masks is not None else None, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
-              _p(csum), _p(ccnt), _p(accd), _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_fwd")
+              _p(csum), _p(ccnt), _p(accd), zd, zm, zb, _p(ws), ws.numel(), _stream(means2d)),
+              "rasterize_to_pixels_fwd")
     return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd
 
 
@@ -306,10 +319,12 @@ def rasterize_to_pixels_fwd_textured_gaussians(means2d, ray_transforms, colors, 
                        tile_offsets, flatten_ids, gs_contrib_threshold, g_weight)[:9]
 
 
-def rasterize_to_pixels_fwd_textured_gaussians_full(*args):
+def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None):
     """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
-    per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W]."""
-    return _raster_fwd(True, *args)
+    per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W].
+    ``zero_rows`` = (buffer [N, ...], uint8 mask [N]): rows with mask 1 are
+    zero-filled by the forward kernel (see ``mark_rows``)."""
+    return _raster_fwd(True, *args, zero_rows=zero_rows)
 
 
 def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
@@ -332,7 +347,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
                 densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
-                v_textures_out=None):
+                v_textures_out=None, zero_rows=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -374,6 +389,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
         v_densify = torch.zeros_like(densify)
         ws = _raster_workspace(dt, C, N, dev)
+        zd, zm, zb = _zero_rows_args(zero_rows, N)
         with _timed("raster_bwd", means2d):
             check(lib().tgs_rasterize_bwd(
               dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
@@ -385,30 +401,30 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
               _p(v_render_alphas), _p(v_render_normals), _p(v_render_distort), _p(v_render_median), uv_mode,
               _p(v_means2d_abs), _p(v_means2d), _p(v_rt), _p(v_colors), _p(v_opacities), _p(v_textures),
               _p(v_normals), _p(v_densify) if densify.shape == means2d.shape else None,
-              _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_bwd")
+              zd, zm, zb, _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_bwd")
     if textured:
         return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_textures, v_normals, v_densify
     return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_normals, v_densify
 
 
-def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None):
+def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None):
     """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
     return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows)
 
 
-def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None):
+def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None):
     """Reference ``bilinear_bwd2`` rule (du/ds = (res-1)/range)."""
     return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows)
 
 
-def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None):
+def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None):
     """True derivative: du/ds = (res-1)/(2 range), and the forward's own
     distortion accumulator when ``render_accum_depth`` is given (no reference
     counterpart)."""
     return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows)
 
 
 def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
@@ -434,6 +450,40 @@ def selective_adam_update(param, param_grad, exp_avg, exp_avg_sq, tiles_touched,
         check(lib().tgs_selective_adam_update(N, M, _p(param), _p(param_grad), _p(exp_avg), _p(exp_avg_sq),
                                               _p(tiles_touched.view(torch.uint8)), float(lr), float(b1),
                                               float(b2), float(eps), _stream(param)), "selective_adam_update")
+
+
+def zero_fill_(t: Tensor) -> Tensor:
+    """Zero-fills a contiguous device tensor in place with one native launch on
+    the current stream (no reference counterpart; used for the texture
+    gradient, which the backward accumulates into)."""
+    _check_input(t, "t")
+    with torch.cuda.device(t.device):
+        check(lib().tgs_zero_fill(_p(t), t.numel() * t.element_size(), _stream(t)), "zero_fill")
+    return t
+
+
+def mark_rows(flatten_ids: Tensor, N: int) -> Tensor:
+    """uint8 mask [N]: 1 for each Gaussian id % N present in ``flatten_ids``
+    (the texture rows the backward can touch), on the current stream."""
+    _check_input(flatten_ids, "flatten_ids")
+    mask = torch.empty((N,), dtype=torch.uint8, device=flatten_ids.device)
+    with torch.cuda.device(flatten_ids.device):
+        check(lib().tgs_mark_rows(flatten_ids.numel(), _p(flatten_ids), N, _p(mask), _stream(flatten_ids)),
+              "mark_rows")
+    return mask
+
+
+def zero_fill_rows_(t: Tensor, mask: Tensor, select: int) -> Tensor:
+    """Zero-fills the rows t[n] (t contiguous, first dim N) with mask[n] == select."""
+    _check_input(t, "t")
+    _check_input(mask, "mask")
+    row_bytes = t[0].numel() * t.element_size() if t.shape[0] > 0 else 0
+    if t.shape[0] == 0:
+        return t
+    with torch.cuda.device(t.device):
+        check(lib().tgs_zero_fill_rows(_p(t), row_bytes, t.shape[0], _p(mask), int(select), _stream(t)),
+              "zero_fill_rows")
+    return t
 
 
 class _OpTable:

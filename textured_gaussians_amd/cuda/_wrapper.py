@@ -23,34 +23,6 @@ from torch import Tensor
 
 from ._ops import _C
 
-_SIDE_STREAMS = {}
-
-
-def _side_stream(device: torch.device) -> torch.cuda.Stream:
-    """One extra stream per device (default priority) for work that overlaps
-    the main stream."""
-    key = device.index if device.index is not None else torch.cuda.current_device()
-    if key not in _SIDE_STREAMS:
-        _SIDE_STREAMS[key] = torch.cuda.Stream(device=device)
-    return _SIDE_STREAMS[key]
-
-
-def _zeros_async(like: Tensor):
-    """A zero-filled tensor shaped like ``like``, filled on the side stream so
-    the fill overlaps what the current stream runs next; returns (tensor,
-    event) - wait on the event before using the tensor."""
-    cur = torch.cuda.current_stream(like.device)
-    side = _side_stream(like.device)
-    buf = torch.empty_like(like)
-    side.wait_stream(cur)
-    with torch.cuda.stream(side):
-        buf.zero_()
-        ev = torch.cuda.Event()
-        ev.record(side)
-    buf.record_stream(side)
-    return buf, ev
-
-
 def _make_lazy_cuda_func(name: str) -> Callable:
     def call(*args, **kwargs):
         return getattr(_C, name)(*args, **kwargs)
@@ -269,16 +241,22 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
     def forward(ctx, fwd_fn, bwd_fn, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                 texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
                 flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight):
-        # the texture gradient is as large as the textures (40 GB at 1M x 50x50): zero it now,
-        # on the side stream, so the fill overlaps this forward instead of preceding the backward
+        # The texture gradient is as large as the textures (40 GB at 1M x 50x50).  Its zero-fill
+        # is fused into the rasterizer kernels, which are compute-bound: the forward zeroes the
+        # rows of Gaussians present in the tile lists (the rows the backward accumulates
+        # into), the backward zeroes the rest.
         ctx.v_tex = None
+        zero_rows = None
         if ctx.needs_input_grad[6] and textures.is_cuda:
-            ctx.v_tex = _zeros_async(textures)
+            v_tex = torch.empty_like(textures)
+            rows = _C.mark_rows(flatten_ids, textures.shape[0])
+            ctx.v_tex = (v_tex, rows)
+            zero_rows = ctx.v_tex
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func(fwd_fn + "_full")(
             means2d, ray_transforms, colors, opacities, textures, texture_range_x, texture_range_y, normals,
             backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
-            g_weight)
+            g_weight, zero_rows=zero_rows)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds,
                               masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
                               median_ids, accum_depth)
@@ -296,10 +274,10 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
         (means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds, masks, isect_offsets,
          flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth) = ctx.saved_tensors
         exact = ctx.bwd_fn == tgs_fns["bilinear_exact"][1]
-        v_tex_buf = None
+        v_tex_buf, zero_rows = None, None
         if ctx.v_tex is not None:
-            v_tex_buf, ev = ctx.v_tex
-            torch.cuda.current_stream(v_tex_buf.device).wait_event(ev)
+            v_tex_buf = ctx.v_tex[0]
+            zero_rows = ctx.v_tex
             ctx.v_tex = None
         (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, v_normals,
          v_densify) = _make_lazy_cuda_func(ctx.bwd_fn)(
@@ -308,7 +286,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             ctx.g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors.contiguous(),
             v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
             v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None,
-            v_textures_out=v_tex_buf)
+            v_textures_out=v_tex_buf, zero_rows=zero_rows)
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
         v_backgrounds = None

@@ -94,11 +94,10 @@ def _finish(render_colors, render_alphas, render_normals, render_median, render_
     if render_mode in ["RGB+ED", "RGB+D"]:
         depth_for_normal = render_colors[..., -1:] if depth_mode == "expected" else render_median
         render_normals_from_depth = depth_to_normal(depth_for_normal, torch.linalg.inv(viewmats), Ks).squeeze(0)
-    # camera-frame normals -> world frame: one [H*W,3] x [3,3] product per camera
-    C, H, W = render_normals.shape[:3]
+    # camera-frame normals -> world frame, n_w[i] = sum_j R[i, j] n[j]: a broadcast product and a
+    # sum over j (two elementwise passes; a [H*W,3] x [3,3] GEMM runs far below bandwidth)
     R = torch.linalg.inv(viewmats)[..., :3, :3]
-    render_normals = torch.bmm(render_normals.reshape(C, H * W, 3),
-                               R.transpose(1, 2).contiguous()).reshape(C, H, W, 3)
+    render_normals = (render_normals[..., None, :] * R[:, None, None]).sum(-1)
     return render_colors, render_normals, render_normals_from_depth
 
 
