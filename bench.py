@@ -144,7 +144,7 @@ def cpu_baseline(sc, meta_gpu, n_tiles_sample):
     from oracle.raster import rasterize_to_pixels
     from oracle.sh import spherical_harmonics
 
-    threads = os.cpu_count() or 1
+    threads = min(os.cpu_count() or 1, 16)  # the host-core share of one GPU
     torch.set_num_threads(threads)
     f = torch.float32
     cpu = lambda t: t.detach().cpu().to(f) if t.is_floating_point() else t.detach().cpu()  # noqa: E731
