@@ -212,6 +212,30 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
                       void *workspace, size_t workspace_size, tgs_stream_t stream);
 
+/* ------------------------------------------------ neural-texture decoder
+ * No reference counterpart (its closest relative is the coordinate-MLP
+ * implicit texture, examples/texture_models/mlp.py:6-18).  The decoder maps a
+ * per-Gaussian latent through 32 -> 128 -> 128 -> 1024 (ReLU, ReLU, sigmoid) to
+ * a 16x16 RGBA texture; these are its building blocks.
+ *
+ * tgs_gemm_bf16: C[M,N] = epilogue(A[M,K] B[K,N] + bias[N]) on the matrix cores
+ * (v_mfma_f32_16x16x32_bf16): operands are float32 in memory, rounded to
+ * bfloat16 (nearest even) when staged, accumulated in float32.  A element
+ * (m,k) is A[m*lda+k] (trans_a 0) or A[k*lda+m] (trans_a 1); B element (k,n)
+ * is B[k*ldb+n] (trans_b 0) or B[n*ldb+k] (trans_b 1).  Epilogues: NONE, RELU,
+ * SIGMOID, RELU_MASK (keep where mask[m*ldm+n] > 0, for ReLU backward), ATOMIC
+ * (C += A B with float atomics; no bias; k_splits > 1 splits K over
+ * workgroups, C zeroed by the caller).
+ * tgs_sigmoid_bwd: g = g_out * out * (1 - out).
+ * tgs_colsum: db[n] += sum_m G[m*ldg+n] (float32).                         */
+enum { TGS_EPI_NONE = 0, TGS_EPI_RELU = 1, TGS_EPI_SIGMOID = 2, TGS_EPI_RELU_MASK = 3, TGS_EPI_ATOMIC = 4 };
+int tgs_gemm_bf16(int trans_a, int trans_b, int epilogue, int64_t M, int64_t N, int64_t K,
+                  const float *A, int64_t lda, const float *B, int64_t ldb, const float *bias,
+                  float *C, int64_t ldc, const float *mask, int64_t ldm, int32_t k_splits,
+                  tgs_stream_t stream);
+int tgs_sigmoid_bwd(int64_t n, const float *g_out, const float *out, float *g, tgs_stream_t stream);
+int tgs_colsum(int64_t M, int64_t N, const float *G, int64_t ldg, float *db, tgs_stream_t stream);
+
 /* ------------------------------------------------------------ fused Adam
  * Replaces `selective_adam_update` (ext.cpp:73, bindings.h:486-497, adam.cu):
  * rows g < N with visibility[g] != 0 are updated element-wise,

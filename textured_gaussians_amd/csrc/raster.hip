@@ -25,9 +25,12 @@
 // lanes inside the texture range:
 //  * the splat-plane cross product of the reference, cr = hu x hv with
 //    hu = px T2 - T0 and hv = py T2 - T1, is affine in the pixel position:
-//    cr = px (T1 x T2) + py (T2 x T0) + (T0 x T1).  The packing kernel stores
-//    the three cross products, so cr costs six FMAs per pixel; the backward
-//    accumulates gradients with respect to those nine coefficients and a
+//    cr = px (T1 x T2) + py (T2 x T0) + (T0 x T1).  Written around the
+//    Gaussian's 2D mean m, cr = C0 - dx A0 - dy A1 with dx = mx - px,
+//    A0 = T1 x T2, A1 = T2 x T0 and C0 = cr(m), which the packing kernel
+//    computes in float64 (its terms are pixel-sized multiples of A that cancel).
+//    cr then costs six FMAs per pixel with the conditioning of the reference's
+//    hu x hv; the backward accumulates the matching centred sums and a float64
 //    post-pass turns them into ray-transform gradients (cross-product VJP).
 //  * s = cr.xy / cr.z uses one reciprocal; the texel coordinate
 //    u = (s.x + range) / (2 range) (res - 1) is one FMA with host constants.
@@ -69,14 +72,14 @@ template <> __device__ __forceinline__ Texel<double> load_texel<double>(const do
 
 // Packed per-(camera, Gaussian) record, geometry part then shading part:
 // mean2d(2) A(9) texture row(1) | opacity(1) colour(4, K used) normal(3), with
-// A = (T1 x T2, T2 x T0, T0 x T1) for the ray transform rows T0, T1, T2.  The
+// A = (T1 x T2, T2 x T0, C0 = cr at the mean) for the ray transform rows T0, T1, T2.  The
 // texture row is the Gaussian id modulo N (int32 bits in float records, an
 // exact double otherwise): textures and contribution counters are indexed per
 // Gaussian, not per flatten id (the reference's flatten-id indexing runs out of
 // bounds for cameras >= 1).
 constexpr int kRec = 20;
 constexpr int kRecA = 2, kRecRow = 11, kRecOp = 12, kRecCol = 13, kRecNrm = 17;
-constexpr int kAbc = 9;  // backward scratch: gradient w.r.t. A per (camera, Gaussian)
+constexpr int kAbc = 9;  // backward scratch: centred sums of dL/dcr per (camera, Gaussian)
 
 template <typename S> struct RasterArgs {
     int64_t C, N, n_isects;
@@ -120,18 +123,22 @@ __global__ __launch_bounds__(256) void pack_kernel(int64_t n, int64_t N, const S
                                                    S *__restrict__ rec) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const v3<S> T0 = mk3(rt[i * 9 + 0], rt[i * 9 + 1], rt[i * 9 + 2]);
-    const v3<S> T1 = mk3(rt[i * 9 + 3], rt[i * 9 + 4], rt[i * 9 + 5]);
-    const v3<S> T2 = mk3(rt[i * 9 + 6], rt[i * 9 + 7], rt[i * 9 + 8]);
-    const v3<S> A[3] = {cross(T1, T2), cross(T2, T0), cross(T0, T1)};
+    // float64 whatever S is: C0 sums terms of size |pixel| |A| that cancel to a small value
+    const v3<double> T0 = mk3<double>(rt[i * 9 + 0], rt[i * 9 + 1], rt[i * 9 + 2]);
+    const v3<double> T1 = mk3<double>(rt[i * 9 + 3], rt[i * 9 + 4], rt[i * 9 + 5]);
+    const v3<double> T2 = mk3<double>(rt[i * 9 + 6], rt[i * 9 + 7], rt[i * 9 + 8]);
+    const double mx = means2d[i * 2], my = means2d[i * 2 + 1];
+    const v3<double> A0 = cross(T1, T2), A1 = cross(T2, T0);
+    const v3<double> C0 = mx * A0 + my * A1 + cross(T0, T1);
+    const v3<double> A[3] = {A0, A1, C0};
     S r[kRec];
-    r[0] = means2d[i * 2];
-    r[1] = means2d[i * 2 + 1];
+    r[0] = (S)mx;
+    r[1] = (S)my;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        r[kRecA + 3 * k] = A[k].x;
-        r[kRecA + 3 * k + 1] = A[k].y;
-        r[kRecA + 3 * k + 2] = A[k].z;
+        r[kRecA + 3 * k] = (S)A[k].x;
+        r[kRecA + 3 * k + 1] = (S)A[k].y;
+        r[kRecA + 3 * k + 2] = (S)A[k].z;
     }
     r[kRecRow] = rec_int<S>((int32_t)(i % N));
     r[kRecOp] = opac[i];
@@ -185,16 +192,17 @@ template <typename S>
 __device__ __forceinline__ PairGeom<S> pair_geom(S px, S py, const GaussGeom<S> &G, S vis_bias, S g_weight) {
 #pragma clang fp contract(off)
     PairGeom<S> p;
-    const S crx = tfma(px, G.A[0], tfma(py, G.A[3], G.A[6]));
-    const S cry = tfma(px, G.A[1], tfma(py, G.A[4], G.A[7]));
-    p.cz = tfma(px, G.A[2], tfma(py, G.A[5], G.A[8]));
+    // cr = C0 - dx A0 - dy A1, centred on the Gaussian's mean (dx = mx - px)
+    p.dx = G.mx - px;
+    p.dy = G.my - py;
+    const S crx = tfma(-p.dx, G.A[0], tfma(-p.dy, G.A[3], G.A[6]));
+    const S cry = tfma(-p.dx, G.A[1], tfma(-p.dy, G.A[4], G.A[7]));
+    p.cz = tfma(-p.dx, G.A[2], tfma(-p.dy, G.A[5], G.A[8]));
     p.okz = p.cz != S(0);
     p.rz = trcp(p.okz ? p.cz : S(1));
     p.sx = crx * p.rz;
     p.sy = cry * p.rz;
     p.w3 = tfma(p.sx, p.sx, p.sy * p.sy);
-    p.dx = G.mx - px;
-    p.dy = G.my - py;
     p.w2 = S(2) * tfma(p.dx, p.dx, p.dy * p.dy);
     p.use3 = p.w3 <= p.w2;
     p.sigma = S(0.5) * (p.use3 ? p.w3 : p.w2);
@@ -651,11 +659,12 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
                 }
             }
             if (vsx != S(0) || vsy != S(0)) {
-                // s = cr.xy / cr.z with cr = px A0 + py A1 + A2
+                // s = cr.xy / cr.z with cr = C0 - dx A0 - dy A1
                 const S ax = vsx * p.rz, ay = vsy * p.rz;
                 const S az = -(ax * p.sx + ay * p.sy);
-                g_A[0] = px * ax; g_A[1] = px * ay; g_A[2] = px * az;
-                g_A[3] = py * ax; g_A[4] = py * ay; g_A[5] = py * az;
+                // centred sums: (-dx, -dy, 1) x dL/dcr; the post-pass restores dL/dA
+                g_A[0] = -p.dx * ax; g_A[1] = -p.dx * ay; g_A[2] = -p.dx * az;
+                g_A[3] = -p.dy * ax; g_A[4] = -p.dy * ay; g_A[5] = -p.dy * az;
                 g_A[6] = ax; g_A[7] = ay; g_A[8] = az;
             }
 #pragma unroll
@@ -737,26 +746,36 @@ __global__ __launch_bounds__(256) void zero_kernel(int64_t n, S *__restrict__ p)
     if (i < n) p[i] = 0;
 }
 
-// Post-pass of the backward: the gradient with respect to A = (T1 x T2, T2 x T0,
-// T0 x T1) becomes the ray-transform gradient (for z = x X y: dL/dx = y X dL/dz,
-// dL/dy = dL/dz X x), added to v_ray_transforms; then the densification gradient
-// v_densify = (dL/dT[0][2], dL/dT[1][2]) * T[2][2] from the settled value.
+// Post-pass of the backward, in float64: the kernel accumulated the centred sums
+// S_x = sum (px - mx) dL/dcr, S_y = sum (py - my) dL/dcr, S_0 = sum dL/dcr, so the
+// gradient with respect to A = (T1 x T2, T2 x T0, T0 x T1) is (S_x + mx S_0,
+// S_y + my S_0, S_0) (the mean only recentres cr: its terms cancel).  The
+// cross-product VJP (for z = x X y: dL/dx = y X dL/dz, dL/dy = dL/dz X x) turns
+// that into the ray-transform gradient, added to v_ray_transforms; then the
+// densification gradient v_densify = (dL/dT[0][2], dL/dT[1][2]) * T[2][2] from
+// the settled value.
 template <typename S>
-__global__ __launch_bounds__(256) void rt_grad_kernel(int64_t n, const S *__restrict__ rt, const S *__restrict__ g_abc,
-                                                      S *__restrict__ g_rt, S *__restrict__ v_densify) {
+__global__ __launch_bounds__(256) void rt_grad_kernel(int64_t n, const S *__restrict__ rt, const S *__restrict__ means2d,
+                                                      const S *__restrict__ g_abc, S *__restrict__ g_rt,
+                                                      S *__restrict__ v_densify) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const S *T = rt + i * 9;
     const S *q = g_abc + i * kAbc;
-    const v3<S> T0 = mk3(T[0], T[1], T[2]), T1 = mk3(T[3], T[4], T[5]), T2 = mk3(T[6], T[7], T[8]);
-    const v3<S> va = mk3(q[0], q[1], q[2]), vb = mk3(q[3], q[4], q[5]), vc = mk3(q[6], q[7], q[8]);
-    const v3<S> v0 = cross(vb, T2) + cross(T1, vc);
-    const v3<S> v1 = cross(T2, va) + cross(vc, T0);
-    const v3<S> v2 = cross(va, T1) + cross(T0, vb);
+    const v3<double> T0 = mk3<double>(T[0], T[1], T[2]), T1 = mk3<double>(T[3], T[4], T[5]);
+    const v3<double> T2 = mk3<double>(T[6], T[7], T[8]);
+    const double mx = means2d[i * 2], my = means2d[i * 2 + 1];
+    const v3<double> s0 = mk3<double>(q[6], q[7], q[8]);
+    const v3<double> va = mk3<double>(q[0], q[1], q[2]) + mx * s0;
+    const v3<double> vb = mk3<double>(q[3], q[4], q[5]) + my * s0;
+    const v3<double> vc = s0;
+    const v3<double> v0 = cross(vb, T2) + cross(T1, vc);
+    const v3<double> v1 = cross(T2, va) + cross(vc, T0);
+    const v3<double> v2 = cross(va, T1) + cross(T0, vb);
     S *o = g_rt + i * 9;
-    const S d[9] = {v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z};
+    const double d[9] = {v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z};
 #pragma unroll
-    for (int k = 0; k < 9; ++k) o[k] += d[k];
+    for (int k = 0; k < 9; ++k) o[k] += (S)d[k];
     if (v_densify) {
         v_densify[i * 2] = o[2] * T[8];
         v_densify[i * 2 + 1] = o[5] * T[8];
@@ -977,7 +996,8 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
             rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, 0, st);                   \
         }                                                                                                       \
         if (rc) return rc;                                                                                      \
-        rt_grad_kernel<S><<<nb, 256, 0, st>>>(n, (const S *)ray_transforms, a.g_abc, (S *)v_ray_transforms,     \
+        rt_grad_kernel<S><<<nb, 256, 0, st>>>(n, (const S *)ray_transforms, (const S *)means2d, a.g_abc,        \
+                                              (S *)v_ray_transforms,                                            \
                                               (S *)v_densify);                                                  \
         TGS_CHECK_LAUNCH("rasterize_bwd ray-transform gradient");                                               \
         return TGS_OK;                                                                                          \

@@ -52,6 +52,10 @@ _PROTOS = {
                                   vp, vp, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),
+    "tgs_gemm_bf16": (c_int, [c_int, c_int, c_int, c_i64, c_i64, c_i64, vp, c_i64, vp, c_i64, vp, vp, c_i64, vp,
+                              c_i64, c_i32, vp]),
+    "tgs_sigmoid_bwd": (c_int, [c_i64, vp, vp, vp, vp]),
+    "tgs_colsum": (c_int, [c_i64, c_i64, vp, c_i64, vp, vp]),
     "tgs_selective_adam_update": (c_int, [c_i64, c_i64, vp, vp, vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, vp]),
 }
 

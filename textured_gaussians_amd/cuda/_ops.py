@@ -486,6 +486,68 @@ def zero_fill_rows_(t: Tensor, mask: Tensor, select: int) -> Tensor:
     return t
 
 
+# ------------------------------------------------- neural-texture decoder
+EPI_NONE, EPI_RELU, EPI_SIGMOID, EPI_RELU_MASK, EPI_ATOMIC = 0, 1, 2, 3, 4
+
+
+def _f32_2d(t: Tensor, name: str) -> None:
+    _check_input(t, name)
+    if t.dtype != torch.float32 or t.dim() != 2:
+        raise RuntimeError(f"{name} must be a 2-D float32 tensor")
+
+
+def gemm_bf16(a: Tensor, b: Tensor, trans_a: bool = False, trans_b: bool = False, epilogue: int = EPI_NONE,
+              bias: Optional[Tensor] = None, mask: Optional[Tensor] = None, out: Optional[Tensor] = None,
+              k_splits: int = 1) -> Tensor:
+    """C = epilogue(op(a) @ op(b) + bias) on the matrix cores, bfloat16 operands and
+    float32 accumulation (op = transpose when trans_*).  With ``EPI_ATOMIC`` the
+    product is added into ``out`` (which must be given), optionally with K split
+    over ``k_splits`` workgroup slices."""
+    _f32_2d(a, "a")
+    _f32_2d(b, "b")
+    M, K = (a.shape[1], a.shape[0]) if trans_a else (a.shape[0], a.shape[1])
+    Kb, N = (b.shape[1], b.shape[0]) if trans_b else (b.shape[0], b.shape[1])
+    if K != Kb:
+        raise RuntimeError(f"gemm_bf16: inner sizes differ ({K} vs {Kb})")
+    if bias is not None:
+        _check_input(bias, "bias")
+        assert bias.dtype == torch.float32 and bias.numel() == N
+    if mask is not None:
+        _f32_2d(mask, "mask")
+        assert mask.shape == (M, N)
+    if out is None:
+        if epilogue == EPI_ATOMIC:
+            raise RuntimeError("gemm_bf16: the atomic epilogue accumulates into a given out")
+        out = torch.empty((M, N), dtype=torch.float32, device=a.device)
+    else:
+        _f32_2d(out, "out")
+        assert out.shape == (M, N)
+    with torch.cuda.device(a.device):
+        check(lib().tgs_gemm_bf16(int(trans_a), int(trans_b), int(epilogue), M, N, K, _p(a), a.stride(0), _p(b),
+                                  b.stride(0), _p(bias), _p(out), out.stride(0), _p(mask),
+                                  mask.stride(0) if mask is not None else 0, int(k_splits), _stream(a)),
+              "gemm_bf16")
+    return out
+
+
+def sigmoid_bwd(g_out: Tensor, out: Tensor) -> Tensor:
+    _check_input(g_out, "g_out")
+    _check_input(out, "out")
+    g = torch.empty_like(out)
+    with torch.cuda.device(out.device):
+        check(lib().tgs_sigmoid_bwd(out.numel(), _p(g_out), _p(out), _p(g), _stream(out)), "sigmoid_bwd")
+    return g
+
+
+def colsum_(g: Tensor, db: Tensor) -> Tensor:
+    """db += g.sum(0) for a 2-D float32 g (float32 accumulation)."""
+    _f32_2d(g, "g")
+    _check_input(db, "db")
+    with torch.cuda.device(g.device):
+        check(lib().tgs_colsum(g.shape[0], g.shape[1], _p(g), g.stride(0), _p(db), _stream(g)), "colsum")
+    return db
+
+
 class _OpTable:
     """Attribute access by op name, like the reference's ``_C`` module."""
 

@@ -97,7 +97,8 @@ def make_scene(n: int, width: int, height: int, fov_deg: float, cam_positions, t
     target = torch.rand(C, height, width, 3, generator=tg2)
     sc = Scene(means, quats, scales, opacities, colors, None, viewmats, Ks, width, height, sh_degree, backgrounds,
                target).to(device, dtype)
-    sc.textures = textures if textures is None else textures.to(device)
+    # textures drawn on texture_device stay there (tens of GB at the large configs)
+    sc.textures = textures if textures is None or texture_device is not None else textures.to(device)
     return sc
 
 
@@ -126,3 +127,69 @@ def config3(**kw) -> Scene:
                 sh_sigma=0.2)
     args.update(kw)
     return make_scene(**args)
+
+
+def smooth_textures(n: int, res: int = 16, seed: int = 7, max_cycles: float = 0.75) -> torch.Tensor:
+    """Procedural ground-truth textures [n, res, res, 4] in (0, 1): per texture a base
+    colour plus one low-frequency cosine wave per channel group (RGB shares a wave,
+    alpha has its own), at most ``max_cycles`` cycles across the texture."""
+    g = torch.Generator().manual_seed(seed)
+    u = (torch.arange(res) + 0.5) / res
+    uu, vv = torch.meshgrid(u, u, indexing="xy")  # [res(v), res(u)]
+    def wave():
+        f = (torch.rand(n, 2, generator=g) * 2 - 1) * max_cycles
+        ph = torch.rand(n, generator=g) * 2 * math.pi
+        return torch.cos(2 * math.pi * (f[:, 0, None, None] * uu + f[:, 1, None, None] * vv) + ph[:, None, None])
+    base = torch.rand(n, 3, generator=g) * 0.6 + 0.2
+    amp = torch.rand(n, 3, generator=g) * 0.2
+    rgb = base[:, None, None, :] + amp[:, None, None, :] * wave()[..., None]
+    alpha = 0.6 + 0.35 * wave()
+    return torch.cat([rgb, alpha[..., None]], -1).clamp(0.01, 0.99)
+
+
+def atom_textures(n: int, res: int = 16, seed: int = 7, n_atoms: int = 24, per_texture: int = 2,
+                  cycles=(1.5, 3.0)) -> torch.Tensor:
+    """Ground-truth textures [n, res, res, 4] in (0, 1) that share structure, as the
+    textures of one scene do: a dictionary of ``n_atoms`` RGBA patterns (each a
+    sum of two cosine waves of ``cycles`` cycles across the texture), and per
+    texture a base colour/alpha plus ``per_texture`` atoms with random weights."""
+    g = torch.Generator().manual_seed(seed)
+    u = (torch.arange(res) + 0.5) / res
+    uu, vv = torch.meshgrid(u, u, indexing="xy")
+    atoms = torch.zeros(n_atoms, res, res, 4)
+    for k in range(n_atoms):
+        for _ in range(2):
+            ang = torch.rand(1, generator=g) * 2 * math.pi
+            f = cycles[0] + torch.rand(1, generator=g) * (cycles[1] - cycles[0])
+            ph = torch.rand(1, generator=g) * 2 * math.pi
+            wave = torch.cos(2 * math.pi * f * (torch.cos(ang) * uu + torch.sin(ang) * vv) + ph)
+            atoms[k] += wave[..., None] * (torch.rand(4, generator=g) * 2 - 1) * 0.5
+    idx = torch.stack([torch.randperm(n_atoms, generator=g)[:per_texture] for _ in range(n)])
+    wts = torch.rand(n, per_texture, generator=g) * 0.3 + 0.1
+    base = torch.cat([torch.rand(n, 3, generator=g) * 0.5 + 0.25, torch.rand(n, 1, generator=g) * 0.3 + 0.6], 1)
+    tex = base[:, None, None, :] + (atoms[idx] * wts[:, :, None, None, None]).sum(1)
+    return tex.clamp(0.01, 0.99)
+
+
+def sphere_cameras(n_cams: int, radius: float = 4.0, max_elev_deg: float = 50.0, offset: float = 0.0):
+    """``n_cams`` camera centres spread over a band of the sphere (golden-angle spiral in
+    azimuth, elevations evenly within +-max_elev_deg), all looking at the origin."""
+    cams = []
+    for k in range(n_cams):
+        el = math.radians(max_elev_deg) * (2 * (k + 0.5) / n_cams - 1)
+        az = k * math.pi * (3 - math.sqrt(5)) + offset
+        cams.append((radius * math.cos(el) * math.cos(az), radius * math.cos(el) * math.sin(az),
+                     radius * math.sin(el)))
+    return cams
+
+
+def neural_scene(n: int = 10_000, size: int = 256, n_cams: int = 16, seed: int = 0) -> Scene:
+    """Config-1-style Gaussians seen by ``n_cams`` cameras around them (fov 40, radius 4),
+    with dim base colours (the textures carry the colour) and ground-truth textures
+    that share structure (``atom_textures``): the scene the neural-texture
+    decoder's training textures are refined on."""
+    cams = sphere_cameras(n_cams)
+    sc = make_scene(n, size, size, 40.0, cams, 16, sh_degree=None, seed=seed)
+    sc.colors = torch.full_like(sc.colors, 0.05)
+    sc.textures = atom_textures(n, 16, seed + 7).to(sc.means.device)
+    return sc
