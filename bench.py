@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=60, help="tiles in the CPU oracle sample")
+    ap.add_argument("--no-neural", action="store_true", help="skip the neural-texture leg")
     return ap.parse_args()
 
 
@@ -218,6 +219,76 @@ def cpu_baseline(sc, meta_gpu, n_tiles_sample):
                         f"({t_sample:.1f}s) scaled by sum(I_tile*256); fp32 torch oracle; cpu: {model}"))
 
 
+DECODER_FLOP = 2 * (32 * 128 + 128 * 128 + 128 * 1024)  # per latent
+
+
+def neural_leg(tg, n, steps, warmup, dev):
+    """The neural-texture form at scene scale: the headline scene's N Gaussians with
+    textures decoded from per-Gaussian 32-d latents (N(0,1)) by the 32 -> 128 -> 128
+    -> 1024 decoder (16x16 RGBA).  Times (a) the fused inference kernel alone, HIP
+    events on its stream, and (b) one training step: decode (matrix-core GEMMs under
+    autograd) -> textured render -> L1 loss -> backward into latents, decoder and
+    Gaussians."""
+    from textured_gaussians_amd import neural, scenes
+    from textured_gaussians_amd.cuda import _ops
+    sc = scenes.config2(n=n, tex_res=None).to(dev)
+    z = torch.randn(n, 32, generator=torch.Generator().manual_seed(2)).to(dev)
+    dec = neural.NeuralTextureDecoder(device=dev, seed=3)
+    packed = _ops.decoder_pack(dec.W1.detach(), dec.W2.detach(), dec.W3.detach())
+    bias = [dec.b1.detach(), dec.b2.detach(), dec.b3.detach()]
+    tex = torch.empty(n, 16, 16, 4, device=dev)
+    for _ in range(2):
+        _ops.decode_textures(z, packed, *bias, out=tex)
+    torch.cuda.synchronize()
+    reps = 10
+    st = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        _ops.decode_textures(z, packed, *bias, out=tex)
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_dec = e0.elapsed_time(e1) / reps
+    dec_bytes = n * (32 * 4 + 1024 * 4) + packed.numel() * 2 + 4 * (128 + 128 + 1024)
+    del tex
+
+    zp = z.clone().requires_grad_(True)
+    gauss = dict(means=sc.means.clone().requires_grad_(True), quats=sc.quats.clone().requires_grad_(True),
+                 scales=sc.scales.clone().requires_grad_(True),
+                 opacities_logit=torch.logit(sc.opacities.clamp(1e-6, 1 - 1e-6)).requires_grad_(True),
+                 sh=sc.colors.clone().requires_grad_(True))
+    train = [zp, *dec.parameters(), *gauss.values()]
+
+    def nstep():
+        for p in train:
+            p.grad = None
+        textures = dec(zp)
+        out = tg.rasterization_textured_gaussians(
+            gauss["means"], gauss["quats"], gauss["scales"], torch.sigmoid(gauss["opacities_logit"]), gauss["sh"],
+            textures, sc.viewmats, sc.Ks, sc.width, sc.height, sh_degree=3, backgrounds=sc.backgrounds)
+        loss = (out[0] - sc.target).abs().mean()
+        loss.backward()
+
+    for _ in range(warmup):
+        nstep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nstep()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / steps * 1e3
+    achieved = dec_bytes / (t_dec * 1e-3) / 1e9
+    return {
+        "workload": f"N={n} Gaussians of the headline scene, 32-d latents -> 16x16x4 f32 textures, 1297x840",
+        "decode_ms": round(t_dec, 4), "decode_tflops": round(n * DECODER_FLOP / (t_dec * 1e-3) / 1e12, 1),
+        "decode_roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "decode_kernel",
+                            "algorithmic_bytes": dec_bytes},
+        "train_step_ms": round(t_step, 3),
+        "texture_bytes": {"explicit_16x16_f32": n * 4096, "latents_bf16_plus_decoder": n * 64 + 151_680 * 2},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -280,6 +351,11 @@ def main():
             line["cpu_baseline"] = cpu_baseline(sc, meta, args.cpu_tiles)
         except MemoryError as e:  # pragma: no cover
             line["cpu_baseline"] = {"error": str(e)}
+    if not args.no_neural:
+        print("[bench] neural-texture leg", file=sys.stderr, flush=True)
+        del sc, params
+        torch.cuda.empty_cache()
+        line["neural"] = neural_leg(tg, args.n, max(3, args.steps // 2), 1, dev)
     print(json.dumps(line), flush=True)
 
 

@@ -236,6 +236,20 @@ int tgs_gemm_bf16(int trans_a, int trans_b, int epilogue, int64_t M, int64_t N, 
 int tgs_sigmoid_bwd(int64_t n, const float *g_out, const float *out, float *g, tgs_stream_t stream);
 int tgs_colsum(int64_t M, int64_t N, const float *G, int64_t ldg, float *db, tgs_stream_t stream);
 
+/* Fused decoder inference: all three layers in one kernel, activations kept in
+ * registers, with the GEMM path's numerics (bfloat16 operands, float32
+ * accumulation, bias and activations in float32).
+ * tgs_decoder_pack: W1 [32,128], W2 [128,128], W3 [128,1024] (row-major
+ * [in, out], float32) -> tgs_decoder_pack_bytes() bytes of bfloat16 MFMA
+ * fragments at `packed` (16-byte aligned).  Repack after every weight change.
+ * tgs_decode_textures: out[i, :] = sigmoid(W3^T relu(W2^T relu(W1^T z[i] + b1)
+ * + b2) + b3) for i < n; z [n,32], out [n,1024] (the [n,16,16,4] RGBA textures);
+ * z, packed, biases and out 16-byte aligned.                                */
+size_t tgs_decoder_pack_bytes(void);
+int tgs_decoder_pack(const float *W1, const float *W2, const float *W3, void *packed, tgs_stream_t stream);
+int tgs_decode_textures(int64_t n, const float *z, const void *packed, const float *b1, const float *b2,
+                        const float *b3, float *out, tgs_stream_t stream);
+
 /* ------------------------------------------------------------ fused Adam
  * Replaces `selective_adam_update` (ext.cpp:73, bindings.h:486-497, adam.cu):
  * rows g < N with visibility[g] != 0 are updated element-wise,

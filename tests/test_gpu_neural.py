@@ -91,6 +91,40 @@ def test_decoder_forward_backward_match_oracle():
         assert e < 2e-3, (name, e)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 300, 4096 + 17])
+def test_fused_decode_matches_oracle_and_gemm_path(n):
+    dec = neural.NeuralTextureDecoder(device=DEV, seed=5)
+    g = torch.Generator().manual_seed(6)
+    z = (torch.randn(n, 32, generator=g) * 2).to(DEV)
+    packed = _ops.decoder_pack(dec.W1.detach(), dec.W2.detach(), dec.W3.detach())
+    out = _ops.decode_textures(z, packed, dec.b1.detach(), dec.b2.detach(), dec.b3.detach())
+    assert out.shape == (n, 16, 16, 4)
+    P = [t.detach().double().cpu() for t in (z, dec.W1, dec.b1, dec.W2, dec.b2, dec.W3, dec.b3)]
+    ref, _, _ = mlp_forward(*P)
+    # The kernel accumulates in float32, the oracle in float64: a hidden activation within
+    # ~1e-7 of a bfloat16 rounding boundary rounds to the neighbouring value (one bfloat16
+    # ulp, 2^-8 relative), which moves that latent's outputs by up to ~1e-3.  The other
+    # latents agree to float32 accumulation error.
+    d = (out.reshape(n, -1).double().cpu() - ref).abs()
+    assert float(d.max()) < 2e-3 and float(d.mean()) < 1e-6
+    assert float((d.amax(1) > 1e-5).double().mean()) < 0.02
+    with torch.no_grad():
+        gemm = neural.decode_mlp(z, dec.W1, dec.b1, dec.W2, dec.b2, dec.W3, dec.b3)
+    dg = (out.reshape(n, -1) - gemm).abs()
+    assert float(dg.max()) < 2e-3 and float((dg.amax(1) > 1e-5).float().mean()) < 0.02
+    # inference through the module (no autograd) takes the fused kernel
+    with torch.no_grad():
+        assert torch.equal(dec(z), out)
+
+
+def test_fused_decode_writes_only_its_rows():
+    dec = neural.NeuralTextureDecoder(device=DEV, seed=7)
+    z = torch.randn(100, 32, device=DEV)
+    buf = torch.full((101, 16, 16, 4), -7.0, device=DEV)
+    dec.decode(z, out=buf[:100])
+    assert float(buf[100].max()) == -7.0 and float(buf[:100].min()) > 0.0
+
+
 def test_decoder_memory_and_output_range():
     dec = neural.NeuralTextureDecoder(device=DEV)
     assert sum(p.numel() for p in dec.parameters()) == 32 * 128 + 128 + 128 * 128 + 128 + 128 * 1024 + 1024

@@ -28,13 +28,11 @@ namespace tgs {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 bits, round to nearest even (NaN stays NaN)
-__device__ __forceinline__ uint16_t bf16_bits(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// float -> bfloat16 bits, round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t bf16_bits(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// logistic function; the reciprocal is the hardware's (1 ulp)
+__device__ __forceinline__ float sigmoidf(float v) { return __builtin_amdgcn_rcpf(1.f + __expf(-v)); }
 
 enum { EPI_NONE = 0, EPI_RELU = 1, EPI_SIGMOID = 2, EPI_RELU_MASK = 3, EPI_ATOMIC = 4 };
 
@@ -117,7 +115,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
                 }
                 if (bias) v += bias[gn];
                 if (EPI == EPI_RELU) v = v > 0.f ? v : 0.f;
-                if (EPI == EPI_SIGMOID) v = 1.f / (1.f + __expf(-v));
+                if (EPI == EPI_SIGMOID) v = sigmoidf(v);
                 if (EPI == EPI_RELU_MASK) v = mask[(int64_t)gm * ldm + gn] > 0.f ? v : 0.f;
                 *dst = v;
             }
@@ -148,6 +146,209 @@ __global__ __launch_bounds__(256) void colsum_kernel(int M, int N, const float *
     part[rq][c] = s;
     __syncthreads();
     if (rq == 0 && n < N) atomic_add(db + n, part[0][c] + part[1][c] + part[2][c] + part[3][c]);
+}
+
+// ------------------------------------------------------------ fused decode
+// Inference of the whole decoder in one kernel: latents in, float32 textures out,
+// activations never leave registers.  Each layer is computed transposed,
+// H_out^T = W^T H_in^T, so a 16x16 MFMA tile holds 16 output features (rows) x
+// 16 latents (columns) and lane l holds features 4(l/16)..+3 of latent l%16.
+// Rounded to bfloat16, those 4 values of two consecutive feature tiles are exactly
+// the lane's 8 B-operand values of the next layer's K step, provided that layer
+// visits K in the order the lanes hold it (kperm); the weights are packed once per
+// call into 1 KB MFMA A-fragments in that order, so no activation is exchanged
+// between lanes.  A wave owns 4 column tiles (64 latents) and reuses each weight
+// fragment (read from L2, 296 KB in all) across them.  The output row of a latent
+// is its 1024 features in order, i.e. the [16, 16, 4] RGBA texture, written through
+// an LDS slab in 512-byte pieces.  The kernel is bound by those 4 KB-per-latent
+// writes; the matrix-core work is a small fraction of its time.
+constexpr int kDecIn = 32, kDecHid = 128, kDecOut = 1024;
+constexpr int kDecNT = 4;                                      // 16-latent column tiles per wave
+constexpr int kDecRows = 4 * kDecNT * 16;                      // latents per 256-thread workgroup
+constexpr int kF1 = 0, kF2 = 8, kF3 = 8 + 32, kFN = 8 + 32 + 256;  // fragment offsets per layer
+
+// input feature held in k-slot s (0..31) of K step kt, for layers fed by a D-tile pair
+__device__ __forceinline__ int kperm(int kt, int s) {
+    const int g = s / 8, j = s % 8;
+    return 32 * kt + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+
+// fragment f, lane l, slot j -> bf16(W^T[row][k]) with row = 16 mt + l%16, k per layer
+__global__ __launch_bounds__(256) void decoder_pack_kernel(const float *__restrict__ W1, const float *__restrict__ W2,
+                                                           const float *__restrict__ W3, uint16_t *__restrict__ frag) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= kFN * 512) return;
+    const int f = e / 512, lane = (e / 8) % 64, j = e % 8;
+    const int g = lane / 16, c = lane % 16;
+    float v;
+    if (f < kF2) {
+        const int mt = f;
+        v = W1[(8 * g + j) * kDecHid + 16 * mt + c];
+    } else if (f < kF3) {
+        const int mt = (f - kF2) / 4, kt = (f - kF2) % 4;
+        v = W2[kperm(kt, 8 * g + j) * kDecHid + 16 * mt + c];
+    } else {
+        const int mt = (f - kF3) / 4, kt = (f - kF3) % 4;
+        v = W3[kperm(kt, 8 * g + j) * kDecOut + 16 * mt + c];
+    }
+    frag[e] = bf16_bits(v);
+}
+
+__device__ __forceinline__ bf16x8 ld_frag(const u32x4v *__restrict__ frag, int f, int lane) {
+    return __builtin_bit_cast(bf16x8, frag[f * 64 + lane]);
+}
+
+// two D tiles (features 4g..+3 of tiles 2kt and 2kt+1) -> one bfloat16 B fragment
+__device__ __forceinline__ bf16x8 to_bfrag(const f32x4 &lo, const f32x4 &hi) {
+    return bf16x8{(__bf16)lo[0], (__bf16)lo[1], (__bf16)lo[2], (__bf16)lo[3],
+                  (__bf16)hi[0], (__bf16)hi[1], (__bf16)hi[2], (__bf16)hi[3]};
+}
+
+__device__ __forceinline__ f32x4 ld4(const float *p) { return *(const f32x4 *)p; }
+
+// Layer 3 of the fused decode: 64 tiles of 16 features, each fragment group and
+// bias prefetched one tile ahead.  Finished tiles are staged in the wave's LDS
+// slab (64 latents x 128 features); every 8 tiles the wave writes them out with
+// non-temporal stores, two latents x 512 contiguous bytes per instruction (direct
+// 64-byte pieces per tile reach about half the write bandwidth).  FULL (every
+// latent of the wave exists) keeps the stores unconditional, so the wait for the
+// prefetch never waits for them.
+constexpr int kDecGroup = 8;                         // tiles per staged group
+constexpr int kDecPitch = 16 * kDecGroup + 4;        // floats per staged latent row
+constexpr int kDecSlab = 16 * kDecNT * kDecPitch;    // floats per wave
+
+template <bool FULL>
+__device__ __forceinline__ void decode_out(const u32x4v *__restrict__ frag, const bf16x8 (&h2)[4][kDecNT],
+                                           const float *__restrict__ b3, float *__restrict__ out, int64_t base,
+                                           int64_t n, int lane, float *__restrict__ slab) {
+    const int g = lane / 16, c = lane % 16;
+    const f32x4 zero = {0, 0, 0, 0};
+    bf16x8 A[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) A[kt] = ld_frag(frag, kF3 + kt, lane);
+    f32x4 bias = ld4(b3 + 4 * g);
+    auto tile = [&](int mt) {
+        f32x4 d[kDecNT];
+#pragma unroll
+        for (int t = 0; t < kDecNT; ++t) d[t] = zero;
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+            for (int t = 0; t < kDecNT; ++t) d[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kt], h2[kt][t], d[t], 0, 0, 0);
+        const f32x4 bc = bias;
+        if (mt + 1 < kDecOut / 16) {
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) A[kt] = ld_frag(frag, kF3 + (mt + 1) * 4 + kt, lane);
+            bias = ld4(b3 + 16 * (mt + 1) + 4 * g);
+        }
+#pragma unroll
+        for (int t = 0; t < kDecNT; ++t) {
+            f32x4 o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o[r] = sigmoidf(d[t][r] + bc[r]);
+            *(f32x4 *)(slab + (16 * t + c) * kDecPitch + 16 * (mt % kDecGroup) + 4 * g) = o;
+        }
+        if (mt % kDecGroup == kDecGroup - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            constexpr int kLanesPer = 4 * kDecGroup;  // lanes per latent row piece
+            constexpr int kLpi = kWave / kLanesPer;   // latents per store instruction
+            const int col = 4 * (lane % kLanesPer);
+#pragma unroll
+            for (int i = 0; i < 16 * kDecNT / kLpi; ++i) {
+                const int L = kLpi * i + lane / kLanesPer;
+                const f32x4 v = *(const f32x4 *)(slab + L * kDecPitch + col);
+                if (FULL || base + L < n)
+                    __builtin_nontemporal_store(
+                        v, (f32x4 *)(out + (base + L) * kDecOut + 16 * kDecGroup * (mt / kDecGroup) + col));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    };
+    // tile 0 peeled: every entry into the loop then has the prefetch followed by a
+    // tile's work in flight, and the wait for the prefetch leaves the stores alone
+    tile(0);
+#pragma unroll 1
+    for (int mt = 1; mt < kDecOut / 16; ++mt) tile(mt);
+}
+
+__global__ __launch_bounds__(256) void decode_kernel(int64_t n, const float *__restrict__ z,
+                                                     const u32x4v *__restrict__ frag, const float *__restrict__ b1,
+                                                     const float *__restrict__ b2, const float *__restrict__ b3,
+                                                     float *__restrict__ out) {
+    __shared__ __align__(16) float slabs[4 * kDecSlab];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t base = (int64_t)blockIdx.x * kDecRows + (int64_t)wave * (16 * kDecNT);
+    if (base >= n) return;  // wave-uniform; the waves share no LDS and no barrier
+    const int g = lane / 16, c = lane % 16;
+    // layer-1 B operand: latent base + 16 t + c, input features 8g..8g+7
+    bf16x8 bz[kDecNT];
+#pragma unroll
+    for (int t = 0; t < kDecNT; ++t) {
+        const int64_t row = base + 16 * t + c;
+        f32x4 lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+        if (row < n) {
+            lo = ld4(z + row * kDecIn + 8 * g);
+            hi = ld4(z + row * kDecIn + 8 * g + 4);
+        }
+        bz[t] = to_bfrag(lo, hi);
+    }
+    const f32x4 zero = {0, 0, 0, 0};
+    // ---- layer 1: 32 -> 128, ReLU
+    bf16x8 h1[4][kDecNT];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) {
+        f32x4 d[2][kDecNT];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int mt = 2 * kt + h;
+            const bf16x8 A = ld_frag(frag, kF1 + mt, lane);
+            const f32x4 bias = ld4(b1 + 16 * mt + 4 * g);
+#pragma unroll
+            for (int t = 0; t < kDecNT; ++t) {
+                d[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, bz[t], zero, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) d[h][t][r] = fmaxf(d[h][t][r] + bias[r], 0.f);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kDecNT; ++t) h1[kt][t] = to_bfrag(d[0][t], d[1][t]);
+    }
+    // ---- layer 2: 128 -> 128, ReLU
+    bf16x8 h2[4][kDecNT];
+#pragma unroll
+    for (int kt2 = 0; kt2 < 4; ++kt2) {
+        f32x4 d[2][kDecNT];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int mt = 2 * kt2 + h;
+#pragma unroll
+            for (int t = 0; t < kDecNT; ++t) d[h][t] = zero;
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) {
+                const bf16x8 A = ld_frag(frag, kF2 + mt * 4 + kt, lane);
+#pragma unroll
+                for (int t = 0; t < kDecNT; ++t)
+                    d[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, h1[kt][t], d[h][t], 0, 0, 0);
+            }
+            const f32x4 bias = ld4(b2 + 16 * mt + 4 * g);
+#pragma unroll
+            for (int t = 0; t < kDecNT; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) d[h][t][r] = fmaxf(d[h][t][r] + bias[r], 0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < kDecNT; ++t) h2[kt2][t] = to_bfrag(d[0][t], d[1][t]);
+    }
+    // ---- layer 3: 128 -> 1024, sigmoid, streamed to the output tile by tile
+    float *slab = slabs + wave * kDecSlab;
+    if (base + 16 * kDecNT <= n) decode_out<true>(frag, h2, b3, out, base, n, lane, slab);
+    else decode_out<false>(frag, h2, b3, out, base, n, lane, slab);
 }
 
 template <bool TA, bool TB, int EPI>
@@ -195,6 +396,32 @@ extern "C" int tgs_gemm_bf16(int trans_a, int trans_b, int epilogue, int64_t M, 
 #undef TGS_GE
 #undef TGS_G
     TGS_CHECK_LAUNCH("gemm_bf16");
+    return TGS_OK;
+}
+
+extern "C" size_t tgs_decoder_pack_bytes(void) { return (size_t)kFN * 512 * sizeof(uint16_t); }
+
+extern "C" int tgs_decoder_pack(const float *W1, const float *W2, const float *W3, void *packed,
+                                tgs_stream_t stream) {
+    TGS_CHECK_ARG(W1 && W2 && W3 && packed, "decoder_pack: null argument");
+    TGS_CHECK_ARG(((uintptr_t)packed & 15) == 0, "decoder_pack: packed weights must be 16-byte aligned");
+    decoder_pack_kernel<<<(unsigned)ceil_div(kFN * 512, 256), 256, 0, (hipStream_t)stream>>>(W1, W2, W3,
+                                                                                           (uint16_t *)packed);
+    TGS_CHECK_LAUNCH("decoder_pack");
+    return TGS_OK;
+}
+
+extern "C" int tgs_decode_textures(int64_t n, const float *z, const void *packed, const float *b1, const float *b2,
+                                   const float *b3, float *out, tgs_stream_t stream) {
+    TGS_CHECK_ARG(n >= 0 && n < (int64_t(1) << 40), "decode_textures: bad n=%lld", (long long)n);
+    if (n == 0) return TGS_OK;
+    TGS_CHECK_ARG(z && packed && b1 && b2 && b3 && out, "decode_textures: null argument");
+    TGS_CHECK_ARG(((uintptr_t)z & 15) == 0 && ((uintptr_t)packed & 15) == 0 && ((uintptr_t)out & 15) == 0 &&
+                      ((uintptr_t)b1 & 15) == 0 && ((uintptr_t)b2 & 15) == 0 && ((uintptr_t)b3 & 15) == 0,
+                  "decode_textures: z, packed weights, biases and out must be 16-byte aligned");
+    decode_kernel<<<(unsigned)ceil_div(n, kDecRows), 256, 0, (hipStream_t)stream>>>(
+        n, z, (const u32x4v *)packed, b1, b2, b3, out);
+    TGS_CHECK_LAUNCH("decode_textures");
     return TGS_OK;
 }
 

@@ -56,6 +56,9 @@ _PROTOS = {
                               c_i64, c_i32, vp]),
     "tgs_sigmoid_bwd": (c_int, [c_i64, vp, vp, vp, vp]),
     "tgs_colsum": (c_int, [c_i64, c_i64, vp, c_i64, vp, vp]),
+    "tgs_decoder_pack_bytes": (c_size, []),
+    "tgs_decoder_pack": (c_int, [vp, vp, vp, vp, vp]),
+    "tgs_decode_textures": (c_int, [c_i64, vp, vp, vp, vp, vp, vp, vp]),
     "tgs_selective_adam_update": (c_int, [c_i64, c_i64, vp, vp, vp, vp, vp, c_dbl, c_dbl, c_dbl, c_dbl, vp]),
 }
 

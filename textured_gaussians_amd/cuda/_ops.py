@@ -548,6 +548,46 @@ def colsum_(g: Tensor, db: Tensor) -> Tensor:
     return db
 
 
+DECODER_SIZES = (32, 128, 128, 1024)
+
+
+def decoder_pack(W1: Tensor, W2: Tensor, W3: Tensor, packed: Optional[Tensor] = None) -> Tensor:
+    """bfloat16 MFMA fragments of the decoder weights ([in, out] float32) for
+    :func:`decode_textures`; repack after every weight change."""
+    i, h1, h2, o = DECODER_SIZES
+    for w, shape, name in ((W1, (i, h1), "W1"), (W2, (h1, h2), "W2"), (W3, (h2, o), "W3")):
+        _f32_2d(w, name)
+        if tuple(w.shape) != shape:
+            raise RuntimeError(f"decoder_pack: {name} must be {shape}, got {tuple(w.shape)}")
+    nbytes = lib().tgs_decoder_pack_bytes()
+    if packed is None:
+        packed = torch.empty(nbytes // 2, dtype=torch.int16, device=W1.device)
+    assert packed.numel() * packed.element_size() >= nbytes and packed.is_contiguous()
+    with torch.cuda.device(W1.device):
+        check(lib().tgs_decoder_pack(_p(W1), _p(W2), _p(W3), _p(packed), _stream(W1)), "decoder_pack")
+    return packed
+
+
+def decode_textures(z: Tensor, packed: Tensor, b1: Tensor, b2: Tensor, b3: Tensor,
+                    out: Optional[Tensor] = None) -> Tensor:
+    """Fused decoder inference: [n, 32] latents -> [n, 16, 16, 4] float32 textures."""
+    _f32_2d(z, "z")
+    if z.shape[1] != DECODER_SIZES[0]:
+        raise RuntimeError(f"decode_textures: latents must be [n, {DECODER_SIZES[0]}]")
+    for b, k, name in ((b1, DECODER_SIZES[1], "b1"), (b2, DECODER_SIZES[2], "b2"), (b3, DECODER_SIZES[3], "b3")):
+        _check_input(b, name)
+        if b.dtype != torch.float32 or b.numel() != k:
+            raise RuntimeError(f"decode_textures: {name} must be {k} float32 values")
+    n = z.shape[0]
+    if out is None:
+        out = torch.empty((n, 16, 16, 4), dtype=torch.float32, device=z.device)
+    assert out.is_contiguous() and out.dtype == torch.float32 and out.numel() == n * DECODER_SIZES[3]
+    with torch.cuda.device(z.device):
+        check(lib().tgs_decode_textures(n, _p(z), _p(packed), _p(b1), _p(b2), _p(b3), _p(out), _stream(z)),
+              "decode_textures")
+    return out
+
+
 class _OpTable:
     """Attribute access by op name, like the reference's ``_C`` module."""
 

@@ -10,11 +10,13 @@ frame, and the explicit-texture rasterizer is used unchanged.
 Decoder: z (32) -> Linear 128 -> ReLU -> Linear 128 -> ReLU -> Linear 1024 ->
 sigmoid, read as [16, 16, 4] (RGBA innermost, the rasterizer's texture layout).
 
-Every product runs on the matrix cores through ``tgs_gemm_bf16``:
-- bfloat16 operands, float32 accumulation;
-- bias and activation fused into the GEMM epilogue;
-- the backward's ReLU masks fused into the data-gradient GEMMs;
-- the weight gradients are split-K GEMMs with atomic accumulation.
+Every product runs on the matrix cores, with bfloat16 operands and float32
+accumulation:
+- training (autograd) through ``tgs_gemm_bf16``: bias and activation fused into
+  the GEMM epilogue, the backward's ReLU masks fused into the data-gradient
+  GEMMs, the weight gradients as split-K GEMMs with atomic accumulation;
+- inference (no autograd) through ``tgs_decode_textures``: the whole network in
+  one kernel, activations kept in registers, textures written once.
 
 The fallback decision, the memory accounting and the training loops are below.
 Training uses the fused Adam kernel (``selective_adam_update``).
@@ -91,8 +93,21 @@ class NeuralTextureDecoder(torch.nn.Module):
         self.W3, self.b3 = lin(hidden, out_dim)
 
     def forward(self, z: Tensor) -> Tensor:
+        if not torch.is_grad_enabled():
+            return self.decode(z)
         out = decode_mlp(z, self.W1, self.b1, self.W2, self.b2, self.W3, self.b3)
         return out.reshape(z.shape[0], self.tex_res, self.tex_res, 4)
+
+    @torch.no_grad()
+    def decode(self, z: Tensor, out: Optional[Tensor] = None) -> Tensor:
+        """Inference with the fused kernel (all layers in one launch); the weights are
+        packed into matrix-core fragments on every call, so it always sees the current
+        parameters."""
+        if self.tex_res != TEX_RES or self.W1.shape[0] != LATENT_DIM or self.W1.shape[1] != HIDDEN:
+            raise RuntimeError("the fused decode kernel is built for 32 -> 128 -> 128 -> 16x16x4")
+        packed = _ops.decoder_pack(self.W1.detach(), self.W2.detach(), self.W3.detach())
+        return _ops.decode_textures(z.detach().float().contiguous(), packed, self.b1.detach(), self.b2.detach(),
+                                    self.b3.detach(), out=out)
 
     def param_bytes(self, bytes_per_param: int = 2) -> int:
         return sum(p.numel() for p in self.parameters()) * bytes_per_param
