@@ -150,6 +150,12 @@ int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs
  * render_accum_depth[C,H,W] (may be NULL; no reference counterpart): the
  * forward's distortion accumulator sum_i w_i colors[i, K-1], for the exact
  * distortion gradient.
+ * live_entries (may be NULL; no reference counterpart): uint8[waves][n_isects]
+ * with waves = ceil(tile_size^2 / 64), the wavefronts of a tile; the forward
+ * sets byte [w][i] to 1 where some pixel of wave w of entry i's tile
+ * composited entry i, 0 where none did, for every i up to the last entry
+ * wave w reached.  Given to tgs_rasterize_bwd, the backward visits only those
+ * entries (the others contribute nothing to any gradient).
  * workspace: device scratch of at least tgs_rasterize_workspace_bytes(dtype,
  * C, N) bytes (no reference counterpart: the reference reads Gaussian
  * attributes in place); it holds packed per-Gaussian records for the call.
@@ -175,8 +181,9 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *render_alphas, void *render_normals, void *render_distort,
                       void *render_median, int32_t *last_ids, int32_t *median_ids,
                       void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                      void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
-                      void *workspace, size_t workspace_size, tgs_stream_t stream);
+                      uint8_t *live_entries, void *zero_rows_dst, const uint8_t *zero_rows_mask,
+                      size_t zero_row_bytes, void *workspace, size_t workspace_size,
+                      tgs_stream_t stream);
 
 /* Replaces `rasterize_to_pixels_bwd_textured_gaussians` /
  * `rasterize_to_pixels_bwd2_textured_gaussians` (ext.cpp:89-94,
@@ -186,6 +193,8 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
  * NULL selects the reference's distortion rule, which seeds the depth
  * accumulator with render_colors[..., K-1] (that includes the texture colour
  * and T*background of the channel); non-NULL gives the true gradient.
+ * live_entries: NULL (visit every entry) or the array the forward of the same
+ * inputs wrote.
  * Accumulated outputs (caller zero-fills): v_means2d_abs[C,N,2] (may be
  * NULL), v_means2d[C,N,2], v_ray_transforms[C,N,3,3], v_colors[C,N,K],
  * v_opacities[C,N], v_textures[N,tex_h,tex_w,4] (NULL if textures NULL),
@@ -203,7 +212,8 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const int32_t *flatten_ids, int64_t n_isects, double g_weight,
                       const void *render_colors, const void *render_alphas,
                       const int32_t *last_ids, const int32_t *median_ids,
-                      const void *render_accum_depth, const void *v_render_colors,
+                      const void *render_accum_depth, const uint8_t *live_entries,
+                      const void *v_render_colors,
                       const void *v_render_alphas,
                       const void *v_render_normals, const void *v_render_distort,
                       const void *v_render_median, int uv_grad_mode, void *v_means2d_abs,

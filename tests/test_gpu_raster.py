@@ -220,3 +220,61 @@ def test_raster_bwd_plain(dtype, rule):
 @pytest.mark.parametrize("K", [1, 4])
 def test_raster_bwd_channels_nobg(K):
     _bwd_case(torch.float64, "exact", K=K, use_bg=False, seed=3)
+
+
+@pytest.mark.parametrize("textured", [True, False])
+@pytest.mark.parametrize("tile_size", [16, 8])
+def test_live_entries_skip_only_unused_entries(textured, tile_size):
+    """The forward records, per wavefront and list entry, whether any of the wave's
+    pixels composited the entry; the backward then visits only those entries.  The
+    bytes are 0/1, every pixel's last composited entry is marked for its wave, and the
+    backward over live entries gives the full walk's gradients (float64, so only the
+    atomics' summation order differs)."""
+    dtype = torch.float64
+    sc = scenes.config1(n=2500, width=80, height=64, tex_res=12, seed=3, sh_degree=None)
+    st = gpu_stages(sc, dtype, tile_size=tile_size)
+    s = st["scene"]
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(4)
+    colors = torch.rand(C, N, 3, generator=g, dtype=torch.float64).to(DEV)
+    opac = sc.opacities.double()[None].repeat(C, 1).to(DEV)
+    tex = sc.textures.to(DEV, dtype) if textured else None
+    flat = st["flatten_ids"]
+    waves = (tile_size * tile_size + 63) // 64
+    live = torch.zeros(waves, flat.numel(), dtype=torch.uint8, device=DEV)
+    if textured:
+        fwd = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(
+            st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], None, None, s.width, s.height,
+            tile_size, st["offsets"], flat, 0.0, 1.0, live_entries=live)
+    else:
+        fwd = _ops.rasterize_to_pixels_fwd_2dgs_full(st["means2d"], st["rt"], colors, opac, st["normals"], None,
+                                                     None, s.width, s.height, tile_size, st["offsets"], flat, 0.0,
+                                                     live_entries=live)
+    lv = live.cpu()
+    assert int(lv.max()) <= 1 and 0 < int(lv.sum()) < lv.numel()
+    alpha, last = fwd[1][0, ..., 0].cpu(), fwd[5][0].cpu()
+    ii, jj = torch.meshgrid(torch.arange(s.height), torch.arange(s.width), indexing="ij")
+    wave = ((ii % tile_size) * tile_size + jj % tile_size) // 64
+    hit = alpha > 0
+    assert hit.any() and bool((lv[wave[hit], last[hit].long()] == 1).all())
+
+    gen = torch.Generator().manual_seed(5)
+    H, W = s.height, s.width
+    vc = torch.randn(C, H, W, 3, generator=gen, dtype=torch.float64).to(DEV)
+    va = torch.randn(C, H, W, 1, generator=gen, dtype=torch.float64).to(DEV)
+    vn = torch.randn(C, H, W, 3, generator=gen, dtype=torch.float64).to(DEV)
+    vd = torch.randn(C, H, W, 1, generator=gen, dtype=torch.float64).to(DEV)
+    vm = torch.randn(C, H, W, 1, generator=gen, dtype=torch.float64).to(DEV)
+    densify = torch.zeros_like(st["means2d"])
+    args = [st["means2d"], st["rt"], colors, opac]
+    if textured:
+        args += [tex, 3.0, 3.0]
+    args += [st["normals"], densify, None, None, W, H, tile_size, st["offsets"], flat]
+    if textured:
+        args += [1.0]
+    args += [fwd[0], fwd[1], fwd[5], fwd[6], vc, va, vn, vd, vm, False]
+    bwd = _ops.rasterize_to_pixels_bwd_exact_textured_gaussians if textured else _ops.rasterize_to_pixels_bwd_2dgs
+    full = bwd(*args, render_accum_depth=fwd[9])
+    part = bwd(*args, render_accum_depth=fwd[9], live_entries=live)
+    for a, b in zip(full[1:], part[1:]):
+        assert rel_err(b, a) < 1e-12

@@ -79,6 +79,15 @@ template <> __device__ __forceinline__ Texel<double> load_texel<double>(const do
 // bounds for cameras >= 1).
 constexpr int kRec = 20;
 constexpr int kRecA = 2, kRecRow = 11, kRecOp = 12, kRecCol = 13, kRecNrm = 17;
+#ifndef TGS_BWD_ZERO_STEPS
+#define TGS_BWD_ZERO_STEPS 2
+#endif
+// fused zero-fill stores per visited entry in the backward (it visits only live entries)
+constexpr int kBwdZeroSteps = TGS_BWD_ZERO_STEPS;
+#ifndef TGS_FWD_ZERO_STEPS
+#define TGS_FWD_ZERO_STEPS 1
+#endif
+constexpr int kFwdZeroSteps = TGS_FWD_ZERO_STEPS;
 constexpr int kAbc = 9;  // backward scratch: centred sums of dL/dcr per (camera, Gaussian)
 
 template <typename S> struct RasterArgs {
@@ -96,6 +105,9 @@ template <typename S> struct RasterArgs {
     S *out_c, *out_a, *out_n, *out_d, *out_m, *csum, *ccnt;
     S *out_accd;  // optional: sum_i w_i depth_i per pixel (the forward's distortion accumulator)
     int32_t *last_ids, *median_ids;
+    // optional: per wave of a tile, one byte per list entry, nonzero where some pixel of
+    // the wave composited the entry ([waves per tile][n_isects]; forward writes, backward reads)
+    uint8_t *live;
     // backward inputs
     const S *r_c, *r_a, *v_c, *v_a, *v_n, *v_d, *v_m;
     const S *r_accd;  // optional: see out_accd; NULL = the reference's seed, render_colors[..., K-1]
@@ -396,10 +408,16 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 #pragma unroll
     for (int k = 0; k < K; ++k) pc[k] = 0;
     bool done = !inside;
+    // live-entry bytes of this wave: bit i of lw is entry lcb + i, written out per 64 entries
+    uint8_t *live = a.live ? a.live + (int64_t)(threadIdx.x / kWave) * a.n_isects : nullptr;
+    uint64_t lw = 0;
+    int32_t lcb = start;
 
-    for (int32_t idx = start; idx < end; ++idx) {
+    int32_t idx = start;
+    for (; idx < end; ++idx) {
         if (__ballot(!done) == 0) break;  // every pixel of this wave is finished
-        zr.step(lane);
+#pragma unroll
+        for (int zs = 0; zs < kFwdZeroSteps; ++zs) zr.step(lane);
         const int32_t g = ids[idx];
         const GaussGeom<S> G = load_geom<S>(a, g);
         const GaussShade<S, K> H = load_shade<S, K>(a, g);
@@ -415,12 +433,13 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
         }
         const S alpha = pair_alpha(H.op, p.vis, a_s);
         S contrib = 0;
-        bool counted = false;
+        bool counted = false, used = false;
         if (!done && pair_keep(p, alpha)) {
             const S nT = T * (S(1) - alpha);
             if (nT <= S(1e-4)) {
                 done = true;  // this entry is excluded
             } else {
+                used = true;
                 const S w = alpha * T;
 #pragma unroll
                 for (int k = 0; k < K; ++k) pc[k] += (H.col[k] + tex[k]) * w;
@@ -450,7 +469,18 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
                 atomic_add(a.ccnt + G.row, S(__popcll(cm)));
             }
         }
+        if (live) {
+            lw |= (uint64_t)(__ballot(used) != 0) << (idx - lcb);
+            if (idx - lcb == kWave - 1) {
+                live[lcb + lane] = (uint8_t)((lw >> lane) & 1);
+                lcb += kWave;
+                lw = 0;
+            }
+        }
     }
+    // entries [lcb, idx) not yet written; the backward reads no entry past the wave's
+    // last composited one, which is below idx
+    if (live && lane < idx - lcb) live[lcb + lane] = (uint8_t)((lw >> lane) & 1);
     zr.drain(lane);
     if (inside) {
         a.out_a[pix] = S(1) - T;
@@ -468,7 +498,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 
 // ----------------------------------------------------------------- backward
 template <typename S, int K, bool TEX>
-__global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
+__global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
     extern __shared__ __align__(16) char smem[];
     // per-wave staging of texture-gradient updates (2 texel rows x 64 pixels)
     TexRec<S> *stage = (TexRec<S> *)smem + (threadIdx.x / kWave) * (2 * kWave);
@@ -539,9 +569,41 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
     }
     const bool uv_term = TEX && a.uv_mode != TGS_UV_NONE;
     const int32_t first = min(end - 1, wave_last);
+    const uint8_t *live = a.live ? a.live + (int64_t)(threadIdx.x / kWave) * a.n_isects : nullptr;
+    // per-Gaussian gradient fields in reduce-scatter order: dL/dA (9), colour (K),
+    // normal (3), mean2d (2), opacity, |mean2d| (2); this lane adds field `slot`
+    constexpr int kGradFields = 17 + K;
+    S *gdst = nullptr;
+    int32_t gstride = 0;
+    {
+        const int slot = wave_rs_slot<kGradFields>(lane);
+        if (slot >= 0 && slot < 9) { gdst = a.g_abc + slot; gstride = kAbc; }
+        else if (slot >= 9 && slot < 9 + K) { gdst = a.g_col + (slot - 9); gstride = K; }
+        else if (slot >= 9 + K && slot < 12 + K) { gdst = a.g_nrm + (slot - 9 - K); gstride = 3; }
+        else if (slot >= 12 + K && slot < 14 + K) { gdst = a.g_xy + (slot - 12 - K); gstride = 2; }
+        else if (slot == 14 + K) { gdst = a.g_op; gstride = 1; }
+        else if (slot >= 15 + K && slot < 17 + K && a.g_xy_abs) { gdst = a.g_xy_abs + (slot - 15 - K); gstride = 2; }
+    }
 
-    for (int32_t idx = first; idx >= start; --idx) {
-        zr.step(lane);
+    // back to front in chunks of 64 entries; with the forward's live bytes only the
+    // entries some pixel of this wave composited are visited
+    for (int32_t hi = first; hi >= start; hi -= kWave) {
+        const int32_t lo = max(start, hi - (kWave - 1));
+        uint64_t todo;
+        if (live) {
+            const int32_t e = lo + lane;
+            todo = __ballot(e <= hi && live[e] != 0);
+        } else {
+            const int32_t cnt = hi - lo + 1;
+            todo = cnt == kWave ? ~0ull : ((1ull << cnt) - 1);
+        }
+    while (todo) {
+        const int bit = 63 - __builtin_clzll(todo);
+        todo ^= 1ull << bit;
+        // wave-uniform (todo is a ballot); stated so the list and record reads stay scalar
+        const int32_t idx = __builtin_amdgcn_readfirstlane(lo + bit);
+#pragma unroll
+        for (int zs = 0; zs < kBwdZeroSteps; ++zs) zr.step(lane);
         const int32_t g = ids[idx];
         const GaussGeom<S> G = load_geom<S>(a, g);
         const GaussShade<S, K> H = load_shade<S, K>(a, g);
@@ -701,41 +763,25 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 3 : 1) void raster_bwd_kernel
                 __builtin_amdgcn_wave_barrier();
             }
         }
-        // per-Gaussian sums over the wavefront; lane f then adds field f
-#pragma unroll
-        for (int k = 0; k < K; ++k) g_col[k] = wave_sum(g_col[k]);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) g_n[k] = wave_sum(g_n[k]);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) g_A[k] = wave_sum(g_A[k]);
-        g_xy[0] = wave_sum(g_xy[0]);
-        g_xy[1] = wave_sum(g_xy[1]);
-        g_op = wave_sum(g_op);
-        if (a.g_xy_abs) {
-            g_abs[0] = wave_sum(g_abs[0]);
-            g_abs[1] = wave_sum(g_abs[1]);
-        }
+        // per-Gaussian sums over the wavefront, reduce-scattered: the lane holding a
+        // field's total adds it (one atomic instruction for all fields)
         {
-            // lane f adds field f of each per-Gaussian gradient array (one instruction per array)
-            const int64_t gg = g;
-            S v = 0;
+            S f[kGradFields];
 #pragma unroll
-            for (int f = 0; f < 9; ++f) v = lane == f ? g_A[f] : v;
-            if (lane < 9 && v != S(0)) atomic_add(a.g_abc + gg * kAbc + lane, v);
-            v = 0;
+            for (int k = 0; k < 9; ++k) f[k] = g_A[k];
 #pragma unroll
-            for (int f = 0; f < K; ++f) v = lane == f ? g_col[f] : v;
-            if (lane < K && v != S(0)) atomic_add(a.g_col + gg * K + lane, v);
-            v = lane == 0 ? g_n[0] : (lane == 1 ? g_n[1] : g_n[2]);
-            if (lane < 3 && v != S(0)) atomic_add(a.g_nrm + gg * 3 + lane, v);
-            v = lane == 0 ? g_xy[0] : g_xy[1];
-            if (lane < 2 && v != S(0)) atomic_add(a.g_xy + gg * 2 + lane, v);
-            if (a.g_xy_abs) {
-                v = lane == 0 ? g_abs[0] : g_abs[1];
-                if (lane < 2 && v != S(0)) atomic_add(a.g_xy_abs + gg * 2 + lane, v);
-            }
-            if (lane == 0 && g_op != S(0)) atomic_add(a.g_op + gg, g_op);
+            for (int k = 0; k < K; ++k) f[9 + k] = g_col[k];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) f[9 + K + k] = g_n[k];
+            f[12 + K] = g_xy[0];
+            f[13 + K] = g_xy[1];
+            f[14 + K] = g_op;
+            f[15 + K] = g_abs[0];
+            f[16 + K] = g_abs[1];
+            const S v = wave_reduce_scatter(f, lane);
+            if (gdst && v != S(0)) atomic_add(gdst + (int64_t)g * gstride, v);
         }
+    }
     }
     zr.drain(lane);
 }
@@ -915,8 +961,8 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  double g_weight, void *render_colors, void *render_alphas, void *render_normals,
                                  void *render_distort, void *render_median, int32_t *last_ids, int32_t *median_ids,
                                  void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                                 void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
-                                 void *workspace, size_t workspace_size, tgs_stream_t stream) {
+                                 uint8_t *live_entries, void *zero_rows_dst, const uint8_t *zero_rows_mask,
+                                 size_t zero_row_bytes, void *workspace, size_t workspace_size, tgs_stream_t stream) {
     int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
                           tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
@@ -937,6 +983,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.out_d = (S *)render_distort; a.out_m = (S *)render_median; a.last_ids = last_ids;                     \
         a.median_ids = median_ids; a.csum = (S *)contrib_sum; a.ccnt = (S *)contrib_count;                      \
         a.out_accd = (S *)render_accum_depth;                                                                   \
+        a.live = live_entries;                                                                                  \
         return dispatch<S, false>(a, K, textures != nullptr, st);                                               \
     }
     if (dtype == TGS_F32) TGS_FWD(float)
@@ -955,7 +1002,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const int32_t *flatten_ids, int64_t n_isects, double g_weight,
                                  const void *render_colors, const void *render_alphas, const int32_t *last_ids,
                                  const int32_t *median_ids, const void *render_accum_depth,
-                                 const void *v_render_colors, const void *v_render_alphas,
+                                 const uint8_t *live_entries, const void *v_render_colors, const void *v_render_alphas,
                                  const void *v_render_normals, const void *v_render_distort,
                                  const void *v_render_median, int uv_grad_mode, void *v_means2d_abs, void *v_means2d,
                                  void *v_ray_transforms, void *v_colors, void *v_opacities, void *v_textures,
@@ -982,6 +1029,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.r_c = (const S *)render_colors; a.r_a = (const S *)render_alphas; a.r_last = last_ids;                \
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \
+        a.live = (uint8_t *)live_entries;                                                                       \
         a.v_n = (const S *)v_render_normals; a.v_d = (const S *)v_render_distort;                               \
         a.v_m = (const S *)v_render_median; a.uv_mode = uv_grad_mode;                                           \
         a.g_xy_abs = (S *)v_means2d_abs; a.g_xy = (S *)v_means2d; a.g_rt = (S *)v_ray_transforms;              \

@@ -259,7 +259,7 @@ def _zero_rows_args(zero_rows, N):
 
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
-                g_weight, zero_rows=None):
+                g_weight, zero_rows=None, live_entries=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, tile_offsets=tile_offsets, flatten_ids=flatten_ids)
     if textured:
@@ -305,7 +305,8 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
               _p(masks.view(torch.uint8)) if masks is not None else None, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
-              _p(csum), _p(ccnt), _p(accd), zd, zm, zb, _p(ws), ws.numel(), _stream(means2d)),
+              _p(csum), _p(ccnt), _p(accd), _live_arg(live_entries, flatten_ids, tile_size), zd, zm, zb, _p(ws),
+              ws.numel(), _stream(means2d)),
               "rasterize_to_pixels_fwd")
     return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd
 
@@ -319,12 +320,31 @@ def rasterize_to_pixels_fwd_textured_gaussians(means2d, ray_transforms, colors, 
                        tile_offsets, flatten_ids, gs_contrib_threshold, g_weight)[:9]
 
 
-def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None):
+def live_entries_buffer(flatten_ids: Tensor, tile_size: int) -> Tensor:
+    """uint8 [waves per tile, n_isects] for the forward to record which list entries
+    each wavefront composited (no reference counterpart)."""
+    waves = (tile_size * tile_size + 63) // 64
+    return torch.empty((waves, flatten_ids.numel()), dtype=torch.uint8, device=flatten_ids.device)
+
+
+def _live_arg(live_entries, flatten_ids, tile_size):
+    if live_entries is None:
+        return None
+    _check_input(live_entries, "live_entries")
+    waves = (tile_size * tile_size + 63) // 64
+    if live_entries.dtype != torch.uint8 or live_entries.numel() != waves * flatten_ids.numel():
+        raise RuntimeError(f"live_entries must be uint8 [{waves}, {flatten_ids.numel()}]")
+    return _p(live_entries)
+
+
+def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None):
     """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
     per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W].
     ``zero_rows`` = (buffer [N, ...], uint8 mask [N]): rows with mask 1 are
-    zero-filled by the forward kernel (see ``mark_rows``)."""
-    return _raster_fwd(True, *args, zero_rows=zero_rows)
+    zero-filled by the forward kernel (see ``mark_rows``).  ``live_entries``
+    (from ``live_entries_buffer``) receives the entries each wavefront composited,
+    for the backward to visit only those."""
+    return _raster_fwd(True, *args, zero_rows=zero_rows, live_entries=live_entries)
 
 
 def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
@@ -337,17 +357,17 @@ def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, nor
 
 def rasterize_to_pixels_fwd_2dgs_full(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
                                       image_width, image_height, tile_size, tile_offsets, flatten_ids,
-                                      gs_contrib_threshold):
+                                      gs_contrib_threshold, live_entries=None):
     return _raster_fwd(False, means2d, ray_transforms, colors, opacities, None, 1.0, 1.0, normals, backgrounds,
                        masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
-                       gs_contrib_threshold, 1.0)
+                       gs_contrib_threshold, 1.0, live_entries=live_entries)
 
 
 def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals,
                 densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
-                v_textures_out=None, zero_rows=None):
+                v_textures_out=None, zero_rows=None, live_entries=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -397,7 +417,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
               _p(masks.view(torch.uint8)) if masks is not None else None, image_width, image_height, tile_size,
               tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(g_weight),
               _p(render_colors), _p(render_alphas), _p(last_ids), _p(median_ids), _p(render_accum_depth),
-              _p(v_render_colors),
+              _live_arg(live_entries, flatten_ids, tile_size), _p(v_render_colors),
               _p(v_render_alphas), _p(v_render_normals), _p(v_render_distort), _p(v_render_median), uv_mode,
               _p(v_means2d_abs), _p(v_means2d), _p(v_rt), _p(v_colors), _p(v_opacities), _p(v_textures),
               _p(v_normals), _p(v_densify) if densify.shape == means2d.shape else None,
@@ -407,36 +427,39 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
     return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_normals, v_densify
 
 
-def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None):
+def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
+                                               live_entries=None):
     """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
     return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries)
 
 
-def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None):
+def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
+                                                live_entries=None):
     """Reference ``bilinear_bwd2`` rule (du/ds = (res-1)/range)."""
     return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries)
 
 
-def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None):
+def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
+                                                     live_entries=None):
     """True derivative: du/ds = (res-1)/(2 range), and the forward's own
     distortion accumulator when ``render_accum_depth`` is given (no reference
     counterpart)."""
     return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries)
 
 
 def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
                                  image_width, image_height, tile_size, tile_offsets, flatten_ids, render_colors,
                                  render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                                  v_render_normals, v_render_distort, v_render_median, absgrad,
-                                 render_accum_depth=None):
+                                 render_accum_depth=None, live_entries=None):
     return _raster_bwd(False, UV_NONE, means2d, ray_transforms, colors, opacities, None, 1.0, 1.0, normals,
                        densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets,
                        flatten_ids, 1.0, render_colors, render_alphas, last_ids, median_ids, v_render_colors,
                        v_render_alphas, v_render_normals, v_render_distort, v_render_median, absgrad,
-                       render_accum_depth=render_accum_depth)
+                       render_accum_depth=render_accum_depth, live_entries=live_entries)
 
 
 # ------------------------------------------------------------------ Adam

@@ -252,11 +252,14 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             rows = _C.mark_rows(flatten_ids, textures.shape[0])
             ctx.v_tex = (v_tex, rows)
             zero_rows = ctx.v_tex
+        # the forward records which list entries each wavefront composited; the backward
+        # visits only those
+        ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func(fwd_fn + "_full")(
             means2d, ray_transforms, colors, opacities, textures, texture_range_x, texture_range_y, normals,
             backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
-            g_weight, zero_rows=zero_rows)
+            g_weight, zero_rows=zero_rows, live_entries=ctx.live)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds,
                               masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
                               median_ids, accum_depth)
@@ -286,7 +289,8 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             ctx.g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors.contiguous(),
             v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
             v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None,
-            v_textures_out=v_tex_buf, zero_rows=zero_rows)
+            v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live)
+        ctx.live = None
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
         v_backgrounds = None
@@ -335,10 +339,11 @@ class _RasterizeToPixels2DGS(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks, width,
                 height, tile_size, isect_offsets, flatten_ids, absgrad, distloss, gs_contrib_threshold, exact):
+        ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func("rasterize_to_pixels_fwd_2dgs_full")(
             means2d, ray_transforms, colors, opacities, normals, backgrounds, masks, width, height, tile_size,
-            isect_offsets, flatten_ids, gs_contrib_threshold)
+            isect_offsets, flatten_ids, gs_contrib_threshold, live_entries=ctx.live)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
                               isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids,
                               accum_depth)
@@ -359,7 +364,8 @@ class _RasterizeToPixels2DGS(torch.autograd.Function):
             ctx.height, ctx.tile_size, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
             median_ids, v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
             v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad,
-            render_accum_depth=accum_depth if ctx.exact else None)
+            render_accum_depth=accum_depth if ctx.exact else None, live_entries=ctx.live)
+        ctx.live = None
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
         v_backgrounds = None
