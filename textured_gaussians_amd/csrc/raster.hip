@@ -631,7 +631,13 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         valid = valid && pair_keep(p, alpha);
         if (__ballot(valid) == 0) continue;  // wave-uniform
 
-        bool tex_live = false;
+        // lanes with texture-gradient updates are staged compactly: this lane at tpos
+        uint64_t tl = 0;
+        int tpos = 0;
+        if constexpr (TEX) {
+            tl = __ballot(valid && inr);
+            tpos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(tl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tl, 0u));
+        }
         S g_col[K], g_n[3] = {0, 0, 0}, g_A[9], g_xy[2] = {0, 0}, g_abs[2] = {0, 0}, g_op = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) g_col[k] = 0;
@@ -689,8 +695,8 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
 #pragma unroll
                     for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
                     q[3] += v_as;
-                    TexRec<S> &r0 = stage[lane];
-                    TexRec<S> &r1 = stage[kWave + lane];
+                    TexRec<S> &r0 = stage[tpos];
+                    TexRec<S> &r1 = stage[kWave + tpos];
                     r0.off = bl.o00;
                     r0.du = bl.du;
                     r1.off = bl.o00 + bl.dv;
@@ -702,7 +708,6 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
                         r1.v[k] = bw.w01 * q[k];
                         r1.v[4 + k] = bw.w11 * q[k];
                     }
-                    tex_live = true;
                     if (uv_term) {
                         // d(sample)/du and d(sample)/dv of the bilinear blend, against the
                         // sample's cotangent (colour channels, plus alpha when unclamped)
@@ -738,19 +743,18 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
             // Texture-gradient flush, transposed through LDS: in each round 8 consecutive
             // lanes add the 8 values of one pixel's texel row (two adjacent RGBA texels,
             // contiguous), so one atomic instruction touches 8 such runs instead of 64
-            // scattered values.
-            const uint64_t live = __ballot(tex_live);
-            if (live) {
+            // scattered values; the n staged pixels take ceil(n / 8) rounds.
+            const int n_tl = __popcll(tl);
+            if (n_tl) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 S *gtb = a.g_tex + (int64_t)G.row * a.tex_stride;
                 const int sub = lane >> 3, e = lane & 7;
 #pragma unroll 1
-                for (int grp = 0; grp < 8; ++grp) {
-                    if (((live >> (grp * 8)) & 0xffull) == 0) continue;
+                for (int grp = 0; grp * 8 < n_tl; ++grp) {
                     const int src = grp * 8 + sub;
-                    if ((live >> src) & 1ull) {
+                    if (src < n_tl) {
 #pragma unroll
                         for (int row = 0; row < 2; ++row) {
                             const TexRec<S> &rr = stage[row * kWave + src];
