@@ -156,6 +156,10 @@ int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs
  * composited entry i, 0 where none did, for every i up to the last entry
  * wave w reached.  Given to tgs_rasterize_bwd, the backward visits only those
  * entries (the others contribute nothing to any gradient).
+ * tex_counts int32[N] (may be NULL; no reference counterpart; accumulated,
+ * caller zero-fills): per texture row, the number of (pixel, entry) pairs
+ * whose backward adds texture gradient (composited and inside the texture
+ * range).  Given to tgs_rasterize_bwd, it sizes the texture-gradient records.
  * workspace: device scratch of at least tgs_rasterize_workspace_bytes(dtype,
  * C, N) bytes (no reference counterpart: the reference reads Gaussian
  * attributes in place); it holds packed per-Gaussian records for the call.
@@ -168,6 +172,12 @@ int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs
  * the rows the backward accumulates into are zero when it starts, the other
  * rows when it ends.                                                      */
 size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
+/* Bytes of one texture-gradient record (see tgs_rasterize_bwd). */
+size_t tgs_tex_record_bytes(int dtype);
+/* inclusive[i] = sum_{j <= i} counts[j] (int64); temp of tgs_isect_count_temp_bytes(n)
+ * bytes.  No reference counterpart (sizes the texture-gradient records). */
+int tgs_prefix_sum_i32(int64_t n, const int32_t *counts, int64_t *inclusive, void *temp,
+                       size_t temp_bytes, tgs_stream_t stream);
 
 int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
                       const void *ray_transforms, const void *colors, const void *opacities,
@@ -181,9 +191,9 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *render_alphas, void *render_normals, void *render_distort,
                       void *render_median, int32_t *last_ids, int32_t *median_ids,
                       void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                      uint8_t *live_entries, void *zero_rows_dst, const uint8_t *zero_rows_mask,
-                      size_t zero_row_bytes, void *workspace, size_t workspace_size,
-                      tgs_stream_t stream);
+                      uint8_t *live_entries, int32_t *tex_counts, void *zero_rows_dst,
+                      const uint8_t *zero_rows_mask, size_t zero_row_bytes, void *workspace,
+                      size_t workspace_size, tgs_stream_t stream);
 
 /* Replaces `rasterize_to_pixels_bwd_textured_gaussians` /
  * `rasterize_to_pixels_bwd2_textured_gaussians` (ext.cpp:89-94,
@@ -195,6 +205,16 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
  * and T*background of the channel); non-NULL gives the true gradient.
  * live_entries: NULL (visit every entry) or the array the forward of the same
  * inputs wrote.
+ * Texture-gradient records (all NULL, or all given; no reference counterpart):
+ * tex_counts from the forward of the same inputs, tex_rec_incl int64[N] its
+ * inclusive prefix sum (tgs_prefix_sum_i32), tex_rec_cursor int32[N] zeroed,
+ * tex_records of tex_rec_incl[N-1] * tgs_tex_record_bytes(dtype) bytes.  The
+ * backward then writes one record per pair instead of adding to v_textures in
+ * L2, and a gather kernel sums each texture row's records in LDS and writes the
+ * row once: rows with zero_rows_mask == 1 are written in full (so they need no
+ * zero-fill beforehand; the rows with mask 0 are zero-filled by this call), or,
+ * with zero_rows_dst NULL, the sums are added to v_textures.  Rows larger than
+ * 64 KB fall back to direct accumulation.
  * Accumulated outputs (caller zero-fills): v_means2d_abs[C,N,2] (may be
  * NULL), v_means2d[C,N,2], v_ray_transforms[C,N,3,3], v_colors[C,N,K],
  * v_opacities[C,N], v_textures[N,tex_h,tex_w,4] (NULL if textures NULL),
@@ -213,7 +233,8 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const void *render_colors, const void *render_alphas,
                       const int32_t *last_ids, const int32_t *median_ids,
                       const void *render_accum_depth, const uint8_t *live_entries,
-                      const void *v_render_colors,
+                      const int32_t *tex_counts, const int64_t *tex_rec_incl,
+                      int32_t *tex_rec_cursor, void *tex_records, const void *v_render_colors,
                       const void *v_render_alphas,
                       const void *v_render_normals, const void *v_render_distort,
                       const void *v_render_median, int uv_grad_mode, void *v_means2d_abs,

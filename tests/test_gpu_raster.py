@@ -278,3 +278,43 @@ def test_live_entries_skip_only_unused_entries(textured, tile_size):
     part = bwd(*args, render_accum_depth=fwd[9], live_entries=live)
     for a, b in zip(full[1:], part[1:]):
         assert rel_err(b, a) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,tex_res", [(torch.float32, 12), (torch.float64, 12), (torch.float32, 130)])
+def test_texture_gradient_records_match_direct_accumulation(dtype, tex_res):
+    """Texture gradients through per-pair records and the per-row LDS gather equal
+    direct accumulation (the forward's pair counts size the records); rows absent
+    from the tile lists end zero.  130x130 float32 rows (270 KB) exceed the gather's
+    LDS and take the direct path inside the same call."""
+    sc = scenes.config1(n=2500, width=80, height=64, tex_res=tex_res, seed=6, sh_degree=None)
+    st = gpu_stages(sc, dtype)
+    s = st["scene"]
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(8)
+    colors = torch.rand(C, N, 3, generator=g, dtype=torch.float64).to(DEV, dtype)
+    opac = sc.opacities.double()[None].repeat(C, 1).to(DEV, dtype)
+    tex = sc.textures.to(DEV, dtype)
+    flat = st["flatten_ids"]
+    H, W = s.height, s.width
+    fargs = (st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], None, None, W, H, 16,
+             st["offsets"], flat, 0.0, 1.0)
+    cnt = _ops.tex_counts_buffer(tex)
+    fwd = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(*fargs, tex_counts=cnt)
+    assert int(cnt.sum()) > 0
+    vc = torch.randn(C, H, W, 3, generator=g, dtype=torch.float64).to(DEV, dtype)
+    va = torch.randn(C, H, W, 1, generator=g, dtype=torch.float64).to(DEV, dtype)
+    z3 = torch.zeros(C, H, W, 3, device=DEV, dtype=dtype)
+    z1 = torch.zeros(C, H, W, 1, device=DEV, dtype=dtype)
+    bargs = (st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], torch.zeros_like(st["means2d"]),
+             None, None, W, H, 16, st["offsets"], flat, 1.0, fwd[0], fwd[1], fwd[5], fwd[6], vc, va, z3, z1, z1,
+             False)
+    direct = _ops.rasterize_to_pixels_bwd_exact_textured_gaussians(*bargs)
+    mask = _ops.mark_rows(flat, N)
+    vt = torch.full_like(tex, float("nan"))  # every row must be written by the call
+    rec = _ops.rasterize_to_pixels_bwd_exact_textured_gaussians(*bargs, v_textures_out=vt, zero_rows=(vt, mask),
+                                                                tex_counts=cnt)
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    assert bool(torch.isfinite(rec[5]).all())
+    for a, b in zip(direct[1:], rec[1:]):
+        assert rel_err(b, a) < tol
+    assert float(rec[5][mask == 0].abs().max()) == 0.0

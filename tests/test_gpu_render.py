@@ -154,6 +154,33 @@ def test_texture_gradient_rows_are_filled_by_the_kernels():
     assert g.cpu()[listed].abs().sum() > 0
 
 
+@pytest.mark.parametrize("records", [False, True])
+def test_texture_gradient_paths_through_the_api(records):
+    """Both texture-gradient paths of the autograd Function (float atomics into rows the
+    forward zero-fills, or per-pair records summed per row in LDS) start from
+    uninitialised memory and give the same gradients."""
+    import textured_gaussians_amd.cuda._wrapper as W
+    sc = _scene(n=400, tex=20, seed=12).to(dtype=torch.float32)
+    sc.means[:50] = torch.tensor([3.5, 0.3, 0.4]) * 3.0  # behind the camera: in no tile list
+    grads = {}
+    saved = W.TEXTURE_GRAD_RECORDS
+    try:
+        for mode in (False, records):
+            W.TEXTURE_GRAD_RECORDS = mode
+            L = _leaves(sc, DEV)
+            junk = torch.full(sc.textures.shape, float("nan"), device=DEV)
+            del junk  # the gradient buffer reuses this NaN block
+            out = _gpu_render(sc, L)
+            (out[0].sum() + out[1].sum()).backward()
+            grads[mode] = {k: v.grad.detach().clone() for k, v in L.items() if v.grad is not None}
+    finally:
+        W.TEXTURE_GRAD_RECORDS = saved
+    for k, a in grads[False].items():
+        b = grads[records][k]
+        assert torch.isfinite(b).all(), k
+        assert rel_err(b, a) < 1e-5, k
+
+
 def test_masks_render_background():
     sc = _scene(n=300, seed=5).to(dtype=torch.float32)
     s = sc.to(DEV)

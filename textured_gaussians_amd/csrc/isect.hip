@@ -322,6 +322,13 @@ extern "C" int tgs_isect_tiles_count(int64_t C, int64_t N, const float *means2d,
                                          st);
 }
 
+extern "C" int tgs_prefix_sum_i32(int64_t n, const int32_t *counts, int64_t *inclusive, void *temp,
+                                  size_t temp_bytes, tgs_stream_t stream) {
+    TGS_CHECK_ARG(n >= 0 && (n == 0 || (counts && inclusive)), "prefix_sum: bad arguments");
+    if (n == 0) return TGS_OK;
+    return device_scan<int32_t, int64_t>(counts, inclusive, n, true, (char *)temp, temp_bytes, (hipStream_t)stream);
+}
+
 extern "C" int tgs_isect_tiles_fill(int64_t C, int64_t N, const float *means2d, const int32_t *radii,
                                     const float *depths, const int64_t *cum_tiles_per_gauss, int32_t tile_size,
                                     int32_t tile_width, int32_t tile_height, int64_t *isect_ids,

@@ -108,6 +108,15 @@ template <typename S> struct RasterArgs {
     // optional: per wave of a tile, one byte per list entry, nonzero where some pixel of
     // the wave composited the entry ([waves per tile][n_isects]; forward writes, backward reads)
     uint8_t *live;
+    // optional texture-gradient records: the forward counts per texture row the
+    // (pixel, entry) pairs that will add texture gradient (tex_cnt); the backward
+    // claims slots of each row's segment (rec_incl: inclusive prefix sum of the
+    // counts, rec_cur: per-row cursor) and writes one record per pair into recs
+    int32_t *tex_cnt;
+    const int64_t *rec_incl;
+    const int32_t *rec_cnt;
+    int32_t *rec_cur;
+    void *recs;
     // backward inputs
     const S *r_c, *r_a, *v_c, *v_a, *v_n, *v_d, *v_m;
     const S *r_accd;  // optional: see out_accd; NULL = the reference's seed, render_colors[..., K-1]
@@ -284,6 +293,15 @@ template <typename S> __device__ __forceinline__ BWeights<S> bweights(const Bile
     return {ou * ov, b.wu * ov, ou * b.wv, b.wu * b.wv};
 }
 
+// One texture-gradient contribution of a (pixel, Gaussian) pair: the bilinear
+// lookup (texel (u0,v0) byte offset, whether the +u / +v neighbours are distinct,
+// the fractional weights) and the cotangent of the RGBA sample.  32 bytes (f32).
+template <typename S> struct __align__(16) TexRecord {
+    uint32_t o00, flags;  // flags: bit 0 -> du != 0, bit 1 -> dv != 0
+    S wu, wv;
+    S q[4];
+};
+
 // Bilinear blend of the four texels: a_s (alpha channel) and the K colour channels.
 template <typename S, int K>
 __device__ __forceinline__ S bilerp_blend(const BWeights<S> &w, const Texels<S> &t, S tex[K]) {
@@ -425,8 +443,10 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
         S a_s = 1, tex[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) tex[k] = 0;
+        bool inr = false;
         if constexpr (TEX) {
-            if (!done && in_tex_range(p, a)) {
+            inr = !done && in_tex_range(p, a);
+            if (inr) {
                 const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
                 a_s = bilerp_blend<S, K>(bweights(bl), load_texels(a.textures + (int64_t)G.row * a.tex_stride, bl), tex);
             }
@@ -467,6 +487,13 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
             if (lane == 0) {
                 atomic_add(a.csum + G.row, sum);
                 atomic_add(a.ccnt + G.row, S(__popcll(cm)));
+            }
+        }
+        if constexpr (TEX) {
+            if (a.tex_cnt) {
+                // pairs whose backward adds texture gradient: composited and in range
+                const uint64_t tu = __ballot(used && inr);
+                if (tu && lane == 0) atomicAdd(a.tex_cnt + G.row, (int32_t)__popcll(tu));
             }
         }
         if (live) {
@@ -634,9 +661,14 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         // lanes with texture-gradient updates are staged compactly: this lane at tpos
         uint64_t tl = 0;
         int tpos = 0;
+        int32_t rbase = 0;
         if constexpr (TEX) {
             tl = __ballot(valid && inr);
             tpos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(tl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tl, 0u));
+            if (a.recs && tl) {
+                // claim this entry's slots in the row's record segment now; used at the end
+                if (lane == 0) rbase = atomicAdd(a.rec_cur + G.row, (int32_t)__popcll(tl));
+            }
         }
         S g_col[K], g_n[3] = {0, 0, 0}, g_A[9], g_xy[2] = {0, 0}, g_abs[2] = {0, 0}, g_op = 0;
 #pragma unroll
@@ -695,18 +727,45 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
 #pragma unroll
                     for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
                     q[3] += v_as;
-                    TexRec<S> &r0 = stage[tpos];
-                    TexRec<S> &r1 = stage[kWave + tpos];
-                    r0.off = bl.o00;
-                    r0.du = bl.du;
-                    r1.off = bl.o00 + bl.dv;
-                    r1.du = bl.du;
+                    if (a.recs) {
+                        // lane 0 claimed the slots; read its value whatever lanes are active here
+                        const int32_t pos = __builtin_amdgcn_readlane(rbase, 0) + tpos;
+                        S *gtb = a.g_tex + (int64_t)G.row * a.tex_stride;
+                        if (pos < as_const(a.rec_cnt)[G.row]) {
+                            const int64_t slot = (G.row ? as_const(a.rec_incl)[G.row - 1] : 0) + pos;
+                            TexRecord<S> rec;
+                            rec.o00 = bl.o00;
+                            rec.flags = (bl.du ? 1u : 0u) | (bl.dv ? 2u : 0u);
+                            rec.wu = bl.wu;
+                            rec.wv = bl.wv;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        r0.v[k] = bw.w00 * q[k];
-                        r0.v[4 + k] = bw.w10 * q[k];
-                        r1.v[k] = bw.w01 * q[k];
-                        r1.v[4 + k] = bw.w11 * q[k];
+                            for (int k = 0; k < 4; ++k) rec.q[k] = q[k];
+                            ((TexRecord<S> *)a.recs)[slot] = rec;
+                        } else {
+                            // more pairs than the forward counted (not expected): add directly
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                if (q[k] == S(0)) continue;
+                                atomic_add((S *)((char *)gtb + bl.o00) + k, bw.w00 * q[k]);
+                                atomic_add((S *)((char *)gtb + bl.o00 + bl.du) + k, bw.w10 * q[k]);
+                                atomic_add((S *)((char *)gtb + bl.o00 + bl.dv) + k, bw.w01 * q[k]);
+                                atomic_add((S *)((char *)gtb + bl.o00 + bl.du + bl.dv) + k, bw.w11 * q[k]);
+                            }
+                        }
+                    } else {
+                        TexRec<S> &r0 = stage[tpos];
+                        TexRec<S> &r1 = stage[kWave + tpos];
+                        r0.off = bl.o00;
+                        r0.du = bl.du;
+                        r1.off = bl.o00 + bl.dv;
+                        r1.du = bl.du;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            r0.v[k] = bw.w00 * q[k];
+                            r0.v[4 + k] = bw.w10 * q[k];
+                            r1.v[k] = bw.w01 * q[k];
+                            r1.v[4 + k] = bw.w11 * q[k];
+                        }
                     }
                     if (uv_term) {
                         // d(sample)/du and d(sample)/dv of the bilinear blend, against the
@@ -744,7 +803,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
             // lanes add the 8 values of one pixel's texel row (two adjacent RGBA texels,
             // contiguous), so one atomic instruction touches 8 such runs instead of 64
             // scattered values; the n staged pixels take ceil(n / 8) rounds.
-            const int n_tl = __popcll(tl);
+            const int n_tl = a.recs ? 0 : __popcll(tl);
             if (n_tl) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -829,6 +888,110 @@ __global__ __launch_bounds__(256) void rt_grad_kernel(int64_t n, const S *__rest
     if (v_densify) {
         v_densify[i * 2] = o[2] * T[8];
         v_densify[i * 2 + 1] = o[5] * T[8];
+    }
+}
+
+// Texture-gradient gather.  Each 512-thread workgroup owns a contiguous range of
+// texture rows and sums each row's records in LDS (float atomics there instead of
+// in L2), then writes the row once.  The workgroup is split by role: waves 0-3
+// zero a row buffer and add the row's records into it, waves 4-7 write the
+// previous row out of the other buffer meanwhile.  On this architecture one
+// counter covers a wave's loads and stores, so a wave that loaded records after
+// issuing stores would wait for the stores; with the roles split, the record loads
+// never wait for the write-out and the write-out never waits at all.
+// overwrite: rows with mask[row] != 0 are written in full (they were not
+// zero-filled beforehand); otherwise the sums are added to the row in place and
+// rows without records are left alone.
+template <typename S>
+__global__ __launch_bounds__(512) void tex_gather_kernel(int64_t N, int64_t rows_per_block, int32_t tex_w,
+                                                         int64_t row_elems, const int64_t *__restrict__ rec_incl,
+                                                         const int32_t *__restrict__ rec_cnt,
+                                                         const int32_t *__restrict__ rec_cur,
+                                                         const TexRecord<S> *__restrict__ recs,
+                                                         const uint8_t *__restrict__ mask, int overwrite,
+                                                         S *__restrict__ g_tex) {
+    typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+    extern __shared__ __align__(16) char smem[];
+    const int64_t row_vec = row_elems * (int64_t)sizeof(S) / 16;  // 16-byte units per row
+    // two row buffers, addressed from the LDS base every time so accesses stay ds_* (a
+    // pointer kept in a register would be generic and turn them into flat accesses)
+    S *const acc = (S *)smem;
+    u32x4_t *const acc4 = (u32x4_t *)smem;
+    const int t = threadIdx.x, tt = t & 255;
+    const bool writer = t >= 256;  // uniform per wave
+    const uint32_t tb = 4 * sizeof(S);
+    cptr<int32_t> cnt = as_const(rec_cnt), cur = as_const(rec_cur);
+    cptr<int64_t> incl = as_const(rec_incl);
+
+    // writer waves: half h of row `row` out of buffer b
+    auto write_half = [&](int b, int64_t row, int h) {
+        const int64_t lo = h ? row_vec / 2 : 0, hi = h ? row_vec : row_vec / 2;
+        if (overwrite) {
+            u32x4_t *dst = (u32x4_t *)(g_tex + row * row_elems);
+            for (int64_t i = lo + tt; i < hi; i += 256) __builtin_nontemporal_store(acc4[b * row_vec + i], dst + i);
+        } else {
+            S *d = g_tex + row * row_elems;
+            const int64_t e = 16 / (int64_t)sizeof(S);
+            for (int64_t i = lo * e + tt; i < hi * e; i += 256) d[i] += acc[b * row_elems + i];
+        }
+    };
+
+    const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r_end = min(N, r_begin + rows_per_block);
+    int64_t prev = -1;
+    int k = 0;
+    for (int64_t row = r_begin; row < r_end; ++row) {
+        const int32_t nrec = min(cnt[row], cur[row]);  // records actually written
+        bool skip = nrec == 0;
+        if (overwrite) {
+            // the mask byte through a scalar load of its (4-byte aligned) word, but the last
+            // partial word, which may end the buffer
+            const uint32_t word = (row | 3) < N ? as_const((const uint32_t *)mask)[row >> 2]
+                                                : (uint32_t)mask[row] << (8 * (row & 3));
+            skip = ((word >> (8 * (row & 3))) & 0xffu) == 0;
+        }
+        if (skip) continue;  // uniform over the workgroup
+        const int bc = k & 1, bp = bc ^ 1;
+        if (writer) {
+            if (prev >= 0) write_half(bp, prev, 0);
+        } else {
+            for (int64_t i = tt; i < row_vec; i += 256) acc4[bc * row_vec + i] = u32x4_t{0, 0, 0, 0};
+        }
+        __syncthreads();
+        if (writer) {
+            if (prev >= 0) write_half(bp, prev, 1);
+        } else {
+            const TexRecord<S> *rr = recs + (row ? incl[row - 1] : 0);
+            for (int32_t r = tt; r < nrec; r += 256) {
+                const TexRecord<S> rec = rr[r];
+                Bilerp<S> bl;
+                bl.wu = rec.wu;
+                bl.wv = rec.wv;
+                bl.o00 = rec.o00;
+                bl.du = (rec.flags & 1u) ? tb : 0u;
+                bl.dv = (rec.flags & 2u) ? (uint32_t)tex_w * tb : 0u;
+                const BWeights<S> bw = bweights(bl);
+                const uint32_t e00 = bl.o00 / sizeof(S), e10 = (bl.o00 + bl.du) / sizeof(S);
+                const uint32_t e01 = (bl.o00 + bl.dv) / sizeof(S), e11 = (bl.o00 + bl.du + bl.dv) / sizeof(S);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const S qc = rec.q[c];
+                    if (qc == S(0)) continue;
+                    const int64_t o = bc * row_elems + c;
+                    atomicAdd(acc + o + e00, bw.w00 * qc);
+                    atomicAdd(acc + o + e10, bw.w10 * qc);
+                    atomicAdd(acc + o + e01, bw.w01 * qc);
+                    atomicAdd(acc + o + e11, bw.w11 * qc);
+                }
+            }
+        }
+        __syncthreads();
+        prev = row;
+        ++k;
+    }
+    if (writer && prev >= 0) {
+        write_half((k + 1) & 1, prev, 0);
+        write_half((k + 1) & 1, prev, 1);
     }
 }
 
@@ -944,6 +1107,35 @@ static int check_zero_rows(const void *dst, const uint8_t *mask, size_t row_byte
     return TGS_OK;
 }
 
+constexpr size_t kGatherMaxLds = 160 * 1024;  // LDS per workgroup: two texture rows in the gather
+
+template <typename S> static bool gather_fits(int64_t tex_h, int64_t tex_w) {
+    return 2 * (size_t)(tex_h * tex_w * 4) * sizeof(S) <= kGatherMaxLds;
+}
+
+template <typename S>
+static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterArgs<S> &a, const uint8_t *mask,
+                         hipStream_t st) {
+    const int64_t row_elems = tex_h * tex_w * 4;
+    const size_t lds = 2 * (size_t)row_elems * sizeof(S);
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)tex_gather_kernel<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess) {
+        set_error("rasterize_bwd: cannot reserve %zu bytes of LDS for the texture gather", lds);
+        return TGS_ERR_LAUNCH;
+    }
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, kGatherMaxLds / std::max<size_t>(lds, 1)));
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
+    const int64_t rows_per_block = (N + blocks - 1) / blocks;
+    tex_gather_kernel<S><<<(unsigned)blocks, 512, lds, st>>>(N, rows_per_block, (int32_t)tex_w, row_elems, a.rec_incl,
+                                                             a.rec_cnt, a.rec_cur, (const TexRecord<S> *)a.recs, mask,
+                                                             mask ? 1 : 0, a.g_tex);
+    TGS_CHECK_LAUNCH("rasterize_bwd texture gather");
+    return TGS_OK;
+}
+
 static size_t workspace_bytes(int dtype, int64_t C, int64_t N) {
     return dtype == TGS_F64 ? rec_bytes<double>(C, N) + abc_bytes<double>(C, N)
                             : rec_bytes<float>(C, N) + abc_bytes<float>(C, N);
@@ -955,6 +1147,10 @@ using namespace tgs;
 
 extern "C" size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N) { return workspace_bytes(dtype, C, N); }
 
+extern "C" size_t tgs_tex_record_bytes(int dtype) {
+    return dtype == TGS_F64 ? sizeof(TexRecord<double>) : sizeof(TexRecord<float>);
+}
+
 extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
                                  const void *ray_transforms, const void *colors, const void *opacities,
                                  const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
@@ -965,7 +1161,8 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  double g_weight, void *render_colors, void *render_alphas, void *render_normals,
                                  void *render_distort, void *render_median, int32_t *last_ids, int32_t *median_ids,
                                  void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                                 uint8_t *live_entries, void *zero_rows_dst, const uint8_t *zero_rows_mask,
+                                 uint8_t *live_entries, int32_t *tex_counts, void *zero_rows_dst,
+                                 const uint8_t *zero_rows_mask,
                                  size_t zero_row_bytes, void *workspace, size_t workspace_size, tgs_stream_t stream) {
     int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
                           tile_size, tile_width, tile_height, n_isects);
@@ -988,6 +1185,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.median_ids = median_ids; a.csum = (S *)contrib_sum; a.ccnt = (S *)contrib_count;                      \
         a.out_accd = (S *)render_accum_depth;                                                                   \
         a.live = live_entries;                                                                                  \
+        a.tex_cnt = textures ? tex_counts : nullptr;                                                            \
         return dispatch<S, false>(a, K, textures != nullptr, st);                                               \
     }
     if (dtype == TGS_F32) TGS_FWD(float)
@@ -1006,7 +1204,9 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const int32_t *flatten_ids, int64_t n_isects, double g_weight,
                                  const void *render_colors, const void *render_alphas, const int32_t *last_ids,
                                  const int32_t *median_ids, const void *render_accum_depth,
-                                 const uint8_t *live_entries, const void *v_render_colors, const void *v_render_alphas,
+                                 const uint8_t *live_entries, const int32_t *tex_counts,
+                                 const int64_t *tex_rec_incl, int32_t *tex_rec_cursor, void *tex_records,
+                                 const void *v_render_colors, const void *v_render_alphas,
                                  const void *v_render_normals, const void *v_render_distort,
                                  const void *v_render_median, int uv_grad_mode, void *v_means2d_abs, void *v_means2d,
                                  void *v_ray_transforms, void *v_colors, void *v_opacities, void *v_textures,
@@ -1034,6 +1234,18 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \
         a.live = (uint8_t *)live_entries;                                                                       \
+        const bool recs = textures && tex_records && tex_counts && tex_rec_incl && tex_rec_cursor &&           \
+                          gather_fits<S>(tex_h, tex_w) &&                                                        \
+                          (!zero_rows_dst || ((uintptr_t)zero_rows_mask % 4) == 0);                              \
+        if (recs) {                                                                                             \
+            a.rec_cnt = tex_counts; a.rec_incl = tex_rec_incl; a.rec_cur = tex_rec_cursor;                      \
+            a.recs = tex_records;                                                                               \
+        } else if (textures && tex_records && zero_rows_dst) {                                                  \
+            /* records requested but the row does not fit the gather's LDS: the rows the */                   \
+            /* forward left for the gather are zero-filled here and added to directly    */                   \
+            rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, 1, st);                   \
+            if (rc) return rc;                                                                                  \
+        }                                                                                                       \
         a.v_n = (const S *)v_render_normals; a.v_d = (const S *)v_render_distort;                               \
         a.v_m = (const S *)v_render_median; a.uv_mode = uv_grad_mode;                                           \
         a.g_xy_abs = (S *)v_means2d_abs; a.g_xy = (S *)v_means2d; a.g_rt = (S *)v_ray_transforms;              \
@@ -1048,6 +1260,10 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
             rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, 0, st);                   \
         }                                                                                                       \
         if (rc) return rc;                                                                                      \
+        if (recs) {                                                                                             \
+            rc = launch_gather<S>(N, tex_h, tex_w, a, zero_rows_dst ? zero_rows_mask : nullptr, st);             \
+            if (rc) return rc;                                                                                  \
+        }                                                                                                       \
         rt_grad_kernel<S><<<nb, 256, 0, st>>>(n, (const S *)ray_transforms, (const S *)means2d, a.g_abc,        \
                                               (S *)v_ray_transforms,                                            \
                                               (S *)v_densify);                                                  \

@@ -259,7 +259,7 @@ def _zero_rows_args(zero_rows, N):
 
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
-                g_weight, zero_rows=None, live_entries=None):
+                g_weight, zero_rows=None, live_entries=None, tex_counts=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, tile_offsets=tile_offsets, flatten_ids=flatten_ids)
     if textured:
@@ -305,7 +305,8 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
               _p(masks.view(torch.uint8)) if masks is not None else None, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
-              _p(csum), _p(ccnt), _p(accd), _live_arg(live_entries, flatten_ids, tile_size), zd, zm, zb, _p(ws),
+              _p(csum), _p(ccnt), _p(accd), _live_arg(live_entries, flatten_ids, tile_size),
+              _counts_arg(tex_counts, textures.shape[0] if textured else 0), zd, zm, zb, _p(ws),
               ws.numel(), _stream(means2d)),
               "rasterize_to_pixels_fwd")
     return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd
@@ -337,14 +338,46 @@ def _live_arg(live_entries, flatten_ids, tile_size):
     return _p(live_entries)
 
 
-def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None):
+def tex_counts_buffer(textures: Tensor) -> Tensor:
+    """int32 [N] zeros: the forward counts per texture row the pairs that will add
+    texture gradient (no reference counterpart)."""
+    return torch.zeros(textures.shape[0], dtype=torch.int32, device=textures.device)
+
+
+def _counts_arg(tex_counts, N):
+    if tex_counts is None:
+        return None
+    _check_input(tex_counts, "tex_counts")
+    if tex_counts.dtype != torch.int32 or tex_counts.numel() != N:
+        raise RuntimeError(f"tex_counts must be int32 [{N}]")
+    return _p(tex_counts)
+
+
+def _tex_record_args(tex_counts, dt, N, dev):
+    """(counts, inclusive offsets, cursor, records) for the backward's record path."""
+    if tex_counts is None:
+        return None, None, None, None, None
+    _counts_arg(tex_counts, N)
+    incl = torch.empty(N, dtype=torch.int64, device=dev)
+    temp = torch.empty(max(1, lib().tgs_isect_count_temp_bytes(N)), dtype=torch.uint8, device=dev)
+    check(lib().tgs_prefix_sum_i32(N, _p(tex_counts), _p(incl), _p(temp), temp.numel(), _stream(tex_counts)),
+          "prefix_sum")
+    total = int(incl[-1].item()) if N > 0 else 0  # one device->host read, like isect_tiles
+    recs = torch.empty(max(1, total) * lib().tgs_tex_record_bytes(dt), dtype=torch.uint8, device=dev)
+    cursor = torch.zeros(N, dtype=torch.int32, device=dev)
+    return tex_counts, incl, cursor, recs, temp
+
+
+def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None, tex_counts=None):
     """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
     per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W].
     ``zero_rows`` = (buffer [N, ...], uint8 mask [N]): rows with mask 1 are
     zero-filled by the forward kernel (see ``mark_rows``).  ``live_entries``
     (from ``live_entries_buffer``) receives the entries each wavefront composited,
-    for the backward to visit only those."""
-    return _raster_fwd(True, *args, zero_rows=zero_rows, live_entries=live_entries)
+    for the backward to visit only those.  ``tex_counts`` (from ``tex_counts_buffer``)
+    receives per texture row the pairs that will add texture gradient, for the
+    backward's record-and-gather path."""
+    return _raster_fwd(True, *args, zero_rows=zero_rows, live_entries=live_entries, tex_counts=tex_counts)
 
 
 def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
@@ -367,7 +400,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
                 densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
-                v_textures_out=None, zero_rows=None, live_entries=None):
+                v_textures_out=None, zero_rows=None, live_entries=None, tex_counts=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -410,6 +443,8 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_densify = torch.zeros_like(densify)
         ws = _raster_workspace(dt, C, N, dev)
         zd, zm, zb = _zero_rows_args(zero_rows, N)
+        rc_, ri_, rcur_, rrec_, _tmp = _tex_record_args(tex_counts if textured else None, dt, textures.shape[0]
+                                                        if textured else 0, dev)
         with _timed("raster_bwd", means2d):
             check(lib().tgs_rasterize_bwd(
               dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
@@ -417,7 +452,8 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
               _p(masks.view(torch.uint8)) if masks is not None else None, image_width, image_height, tile_size,
               tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(g_weight),
               _p(render_colors), _p(render_alphas), _p(last_ids), _p(median_ids), _p(render_accum_depth),
-              _live_arg(live_entries, flatten_ids, tile_size), _p(v_render_colors),
+              _live_arg(live_entries, flatten_ids, tile_size), _p(rc_), _p(ri_), _p(rcur_), _p(rrec_),
+              _p(v_render_colors),
               _p(v_render_alphas), _p(v_render_normals), _p(v_render_distort), _p(v_render_median), uv_mode,
               _p(v_means2d_abs), _p(v_means2d), _p(v_rt), _p(v_colors), _p(v_opacities), _p(v_textures),
               _p(v_normals), _p(v_densify) if densify.shape == means2d.shape else None,
@@ -428,26 +464,29 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
 
 
 def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                               live_entries=None):
+                                               live_entries=None, tex_counts=None):
     """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
     return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
+                       tex_counts=tex_counts)
 
 
 def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                                live_entries=None):
+                                                live_entries=None, tex_counts=None):
     """Reference ``bilinear_bwd2`` rule (du/ds = (res-1)/range)."""
     return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
+                       tex_counts=tex_counts)
 
 
 def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                                     live_entries=None):
+                                                     live_entries=None, tex_counts=None):
     """True derivative: du/ds = (res-1)/(2 range), and the forward's own
     distortion accumulator when ``render_accum_depth`` is given (no reference
     counterpart)."""
     return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries)
+                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
+                       tex_counts=tex_counts)
 
 
 def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,

@@ -236,22 +236,36 @@ def rasterize_to_pixels_textured_gaussians(
     return (render_colors,) + tuple(out[1:])
 
 
+# Texture gradients through per-pair records and a per-row LDS gather (True), or
+# float atomics into a v_textures whose listed rows the forward zero-fills (False).
+# Measured on the 1M-Gaussian scene: atomics 21.9 ms/step, records 24.1 ms/step (the
+# gather is bound by per-row latency); at 16x16 textures 43.0 vs 45.7 ms.
+TEXTURE_GRAD_RECORDS = False
+
+
 class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
     @staticmethod
     def forward(ctx, fwd_fn, bwd_fn, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                 texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
                 flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight):
-        # The texture gradient is as large as the textures (40 GB at 1M x 50x50).  Its zero-fill
-        # is fused into the rasterizer kernels, which are compute-bound: the forward zeroes the
-        # rows of Gaussians present in the tile lists (the rows the backward accumulates
-        # into), the backward zeroes the rest.
+        # The texture gradient is as large as the textures (40 GB at 1M x 50x50) and is never
+        # zero-filled on its own.  The backward zeroes the rows of Gaussians absent from the
+        # tile lists along its walk.  The rows of listed Gaussians are zeroed by the forward
+        # along its walk and accumulated into by the backward, or, with records, written in
+        # full by the texture-gradient gather: the forward counts per row the pairs that will
+        # contribute, the backward writes one record per pair, the gather sums each row's
+        # records in LDS and writes the row once.
         ctx.v_tex = None
+        ctx.tex_counts = None
         zero_rows = None
         if ctx.needs_input_grad[6] and textures.is_cuda:
             v_tex = torch.empty_like(textures)
             rows = _C.mark_rows(flatten_ids, textures.shape[0])
             ctx.v_tex = (v_tex, rows)
-            zero_rows = ctx.v_tex
+            if TEXTURE_GRAD_RECORDS:
+                ctx.tex_counts = _C.tex_counts_buffer(textures)
+            else:  # the forward zero-fills the listed rows and the backward adds in place
+                zero_rows = ctx.v_tex
         # the forward records which list entries each wavefront composited; the backward
         # visits only those
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
@@ -259,7 +273,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
          gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func(fwd_fn + "_full")(
             means2d, ray_transforms, colors, opacities, textures, texture_range_x, texture_range_y, normals,
             backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
-            g_weight, zero_rows=zero_rows, live_entries=ctx.live)
+            g_weight, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds,
                               masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
                               median_ids, accum_depth)
@@ -289,8 +303,9 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             ctx.g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors.contiguous(),
             v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
             v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None,
-            v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live)
+            v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts)
         ctx.live = None
+        ctx.tex_counts = None
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
         v_backgrounds = None
