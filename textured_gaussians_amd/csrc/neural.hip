@@ -39,6 +39,51 @@ enum { EPI_NONE = 0, EPI_RELU = 1, EPI_SIGMOID = 2, EPI_RELU_MASK = 3, EPI_ATOMI
 constexpr int kBM = 64, kBN = 64, kBK = 32;
 constexpr int kLdsK = kBK + 8;  // row pitch in bf16 (80 bytes): spreads the 16 rows a wave reads over banks
 
+// Stages a 64 (rows r) x 32 (k) operand tile into S[r][k] as bfloat16, zero outside
+// the matrix.  KCONTIG: element (r, k) is P[r*ld + k]; else P[k*ld + r].  Each thread
+// moves 8 elements with two 16-byte loads (when vec and in range), along k or along r.
+template <bool KCONTIG>
+__device__ __forceinline__ void stage_tile(uint16_t (*S)[kLdsK], const float *__restrict__ P, int64_t ld, int r0, int R,
+                                           int kb, int k_end, bool vec, int t) {
+    float v[8];
+    if (KCONTIG) {
+        const int rr = t >> 2, kk = (t & 3) * 8;
+        const int gr = r0 + rr, gk = kb + kk;
+        const float *src = P + (int64_t)gr * ld + gk;
+        if (vec && gr < R && gk + 8 <= k_end) {
+            const f32x4 a = *(const f32x4 *)src, b = *(const f32x4 *)(src + 4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = a[i];
+                v[4 + i] = b[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = (gr < R && gk + i < k_end) ? src[i] : 0.f;
+        }
+        const bf16x8 w = {(__bf16)v[0], (__bf16)v[1], (__bf16)v[2], (__bf16)v[3],
+                          (__bf16)v[4], (__bf16)v[5], (__bf16)v[6], (__bf16)v[7]};
+        *(bf16x8 *)&S[rr][kk] = w;
+    } else {
+        const int kk = t >> 3, rr = (t & 7) * 8;
+        const int gk = kb + kk, gr = r0 + rr;
+        const float *src = P + (int64_t)gk * ld + gr;
+        if (vec && gk < k_end && gr + 8 <= R) {
+            const f32x4 a = *(const f32x4 *)src, b = *(const f32x4 *)(src + 4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = a[i];
+                v[4 + i] = b[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = (gk < k_end && gr + i < R) ? src[i] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) S[rr + i][kk] = bf16_bits(v[i]);
+    }
+}
+
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const float *__restrict__ A, int64_t lda,
                                                         const float *__restrict__ B, int64_t ldb,
@@ -53,6 +98,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
     const int k_end = min(K, k_begin + k_chunk);
     const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;  // wave quadrant
 
+    // 16-byte loads when the operand allows them
+    const bool a_vec = (lda % 4) == 0 && ((uintptr_t)A % 16) == 0;
+    const bool b_vec = (ldb % 4) == 0 && ((uintptr_t)B % 16) == 0;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -61,28 +109,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
 
     for (int kb = k_begin; kb < k_end; kb += kBK) {
         // ---- stage A tile (64 x 32) and B tile (32 x 64) as bf16, zero outside the matrices
-#pragma unroll
-        for (int it = 0; it < (kBM * kBK) / 256; ++it) {
-            const int e = it * 256 + t;
-            int mm, kk;
-            if (!TA) { mm = e / kBK; kk = e % kBK; }  // k fastest: coalesced along k
-            else { kk = e / kBM; mm = e % kBM; }      // m fastest: coalesced along m
-            const int gm = m0 + mm, gk = kb + kk;
-            float v = 0.f;
-            if (gm < M && gk < k_end) v = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
-            As[mm][kk] = bf16_bits(v);
-        }
-#pragma unroll
-        for (int it = 0; it < (kBN * kBK) / 256; ++it) {
-            const int e = it * 256 + t;
-            int nn, kk;
-            if (!TB) { kk = e / kBN; nn = e % kBN; }  // n fastest
-            else { nn = e / kBK; kk = e % kBK; }      // k fastest
-            const int gn = n0 + nn, gk = kb + kk;
-            float v = 0.f;
-            if (gn < N && gk < k_end) v = TB ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
-            Bs[nn][kk] = bf16_bits(v);
-        }
+        stage_tile<!TA>(As, A, lda, m0, M, kb, k_end, a_vec, t);
+        stage_tile<TB>(Bs, B, ldb, n0, N, kb, k_end, b_vec, t);
         __syncthreads();
         // ---- 2 x 2 MFMA tiles per wave
         bf16x8 af[2], bfr[2];
