@@ -15,5 +15,6 @@ from .cuda import (  # noqa: F401
     spherical_harmonics,
 )
 from .optimizers import SelectiveAdam  # noqa: F401
+from . import checkpoint  # noqa: F401
 
 __version__ = "0.1.0"
