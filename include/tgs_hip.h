@@ -96,6 +96,42 @@ int tgs_projection_2dgs_bwd(int dtype, int64_t C, int64_t N, const void *means,
                             void *v_quats, void *v_scales, void *v_viewmats,
                             tgs_stream_t stream);
 
+/* Packed layout (replaces `fully_fused_projection_packed_fwd_2dgs` /
+ * `..._packed_bwd_2dgs` / `..._packed_bwd2_2dgs`, ext.cpp:63-71, bindings.h:430-486): only the visible
+ * (camera, Gaussian) entries, in (camera, Gaussian) order.
+ * tgs_projection_2dgs_packed_count: flags int32[C*N] (1 = visible) and their
+ * inclusive prefix sum int64[C*N] (temp: tgs_isect_count_temp_bytes(C*N)); the
+ * caller reads nnz = inclusive[C*N-1].
+ * tgs_projection_2dgs_packed_fill: indptr int64[C+1] (entries of camera c are
+ * [indptr[c], indptr[c+1])), camera_ids, gaussian_ids int64[nnz], radii
+ * int32[nnz], means2d[nnz,2], depths[nnz], ray_transforms[nnz,3,3],
+ * normals[nnz,3]; the values are those of the grid layout's visible entries.
+ * tgs_projection_2dgs_packed_bwd: the backward over the nnz entries; sparse
+ * = 0 accumulates v_means[N,3], v_quats[N,4], v_scales[N,3] (caller
+ * zero-fills), sparse = 1 writes them per entry ([nnz,...], the values of the
+ * reference's sparse gradients).  v_viewmats as in tgs_projection_2dgs_bwd. */
+int tgs_projection_2dgs_packed_count(int dtype, int64_t C, int64_t N, const void *means,
+                                     const void *quats, const void *scales, const void *viewmats,
+                                     const void *Ks, int32_t width, int32_t height, double near_plane,
+                                     double far_plane, double radius_clip, int32_t *flags,
+                                     int64_t *inclusive, void *temp, size_t temp_bytes,
+                                     tgs_stream_t stream);
+int tgs_projection_2dgs_packed_fill(int dtype, int64_t C, int64_t N, const void *means,
+                                    const void *quats, const void *scales, const void *viewmats,
+                                    const void *Ks, int32_t width, int32_t height, double near_plane,
+                                    double far_plane, double radius_clip, const int64_t *inclusive,
+                                    int64_t *indptr, int64_t *camera_ids, int64_t *gaussian_ids,
+                                    int32_t *radii, void *means2d, void *depths, void *ray_transforms,
+                                    void *normals, tgs_stream_t stream);
+int tgs_projection_2dgs_packed_bwd(int dtype, int64_t C, int64_t N, int64_t nnz,
+                                   const int64_t *camera_ids, const int64_t *gaussian_ids, int sparse,
+                                   const void *means, const void *quats, const void *scales,
+                                   const void *viewmats, const void *Ks, const int32_t *radii,
+                                   const void *ray_transforms, const void *v_means2d,
+                                   const void *v_depths, const void *v_normals,
+                                   const void *v_ray_transforms, int vjp_mode, void *v_means,
+                                   void *v_quats, void *v_scales, void *v_viewmats, tgs_stream_t stream);
+
 /* ------------------------------------------------------- tile intersection
  * Replaces `isect_tiles` (ext.cpp:32, bindings.h:110-121) in three phases:
  * count -> (caller reads cum_tiles_per_gauss[C*N-1] = n_isects) -> fill -> sort.
@@ -110,6 +146,14 @@ int tgs_isect_tiles_fill(int64_t C, int64_t N, const float *means2d, const int32
                          const float *depths, const int64_t *cum_tiles_per_gauss,
                          int32_t tile_size, int32_t tile_width, int32_t tile_height,
                          int64_t *isect_ids, int32_t *flatten_ids, tgs_stream_t stream);
+/* Packed entries (the reference's isect_tiles with packed=True, [IS]:16-105):
+ * entry i of camera camera_ids[i]; tiles_per_gauss / cum from
+ * tgs_isect_tiles_count(1, nnz, ...); values are entry indices. */
+int tgs_isect_tiles_fill_packed(int64_t C, int64_t nnz, const int64_t *camera_ids, const float *means2d,
+                                const int32_t *radii, const float *depths,
+                                const int64_t *cum_tiles_per_gauss, int32_t tile_size, int32_t tile_width,
+                                int32_t tile_height, int64_t *isect_ids, int32_t *flatten_ids,
+                                tgs_stream_t stream);
 /* Stable LSD radix sort of (key, value) pairs on key bits [0, end_bit)
  * (CUB DeviceRadixSort::SortPairs with DoubleBuffer, isect_tiles.cu:252-287).
  * The result is in (keys, values) if *selector_host == 0 after the call,
@@ -143,6 +187,13 @@ int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs
  * means2d[C,N,2] ray_transforms[C,N,3,3] colors[C,N,K] opacities[C,N]
  * normals[C,N,3] textures[N,tex_h,tex_w,4] backgrounds[C,K] (may be NULL)
  * masks uint8[C,tile_h,tile_w] (may be NULL).
+ * Packed layout (gaussian_ids != NULL; the reference's `packed=True`): the
+ * per-Gaussian inputs are nnz entries ([nnz,2], [nnz,3,3], [nnz,K], [nnz],
+ * [nnz,3]) from tgs_projection_2dgs_packed_fill, flatten_ids index them, and
+ * entry i samples texture row gaussian_ids[i] (the reference indexes the
+ * textures with the entry index there, Q3).  N is then the number of
+ * Gaussians (texture rows, contribution counters), and the workspace is
+ * tgs_rasterize_workspace_bytes(dtype, 1, nnz).
  * Outputs (written): render_colors[C,H,W,K] render_alphas[C,H,W]
  * render_normals[C,H,W,3] render_distort[C,H,W] render_median[C,H,W]
  * last_ids i32[C,H,W] median_ids i32[C,H,W].
@@ -183,7 +234,8 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const void *ray_transforms, const void *colors, const void *opacities,
                       const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
                       double texture_range_x, double texture_range_y,
-                      const void *backgrounds, const uint8_t *masks, int32_t width,
+                      const void *backgrounds, const uint8_t *masks,
+                      const int64_t *gaussian_ids, int64_t nnz, int32_t width,
                       int32_t height, int32_t tile_size, int32_t tile_width,
                       int32_t tile_height, const int32_t *tile_offsets,
                       const int32_t *flatten_ids, int64_t n_isects,
@@ -226,7 +278,8 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const void *ray_transforms, const void *colors, const void *opacities,
                       const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
                       double texture_range_x, double texture_range_y,
-                      const void *backgrounds, const uint8_t *masks, int32_t width,
+                      const void *backgrounds, const uint8_t *masks,
+                      const int64_t *gaussian_ids, int64_t nnz, int32_t width,
                       int32_t height, int32_t tile_size, int32_t tile_width,
                       int32_t tile_height, const int32_t *tile_offsets,
                       const int32_t *flatten_ids, int64_t n_isects, double g_weight,

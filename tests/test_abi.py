@@ -40,21 +40,57 @@ def test_ctypes_prototypes_match_header_arity():
         assert len(_PROTOS[name][1]) == n, (name, len(_PROTOS[name][1]), n)
 
 
+def _ctype_kind(t):
+    if t is ctypes.c_void_p or (isinstance(t, type) and issubclass(t, ctypes._Pointer)):
+        return "ptr"
+    if t is ctypes.c_double:
+        return "f64"
+    return {ctypes.sizeof(ctypes.c_int32): "i32", 8: "i64"}[ctypes.sizeof(t)]
+
+
+def _c_kind(param):
+    p = param.strip()
+    if "*" in p or p.startswith("tgs_stream_t"):
+        return "ptr"
+    if p.startswith("double"):
+        return "f64"
+    if p.startswith(("int64_t", "size_t", "uint64_t")):
+        return "i64"
+    if p.startswith(("int32_t", "int ", "uint32_t", "uint8_t")):
+        return "i32"
+    raise AssertionError(f"unclassified parameter {p!r}")
+
+
+def test_ctypes_prototypes_match_header_types():
+    """Argument by argument: a pointer where the header has a pointer, a 64-bit integer
+    where it has int64_t/size_t, and so on (a shifted argument list passes the arity
+    check but not this one)."""
+    from textured_gaussians_amd.cuda._backend import _PROTOS
+    src = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "tgs_hip.h")).read(), flags=re.S)
+    for name in _declared():
+        params = [a for a in re.search(r"\b" + name + r"\s*\(([^)]*)\)", src).group(1).split(",")
+                  if a.strip() and a.strip() != "void"]
+        got = [_ctype_kind(t) for t in _PROTOS[name][1]]
+        want = [_c_kind(p) for p in params]
+        assert got == want, (name, [(i, p.strip(), g) for i, (p, g, w) in enumerate(zip(params, got, want))
+                                    if g != w])
+
+
 def test_host_validation_errors_without_gpu():
     L = lib()
     null = None
     # 5 channels are rejected before any device work (the reference dispatches 1..5 but
     # its python layer pads 5 -> 8, which it rejects)
-    rc = L.tgs_rasterize_fwd(0, 1, 10, 5, *([null] * 6), 0, 0, 3.0, 3.0, null, null, 16, 16, 16, 1, 1, null, null,
+    rc = L.tgs_rasterize_fwd(0, 1, 10, 5, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 16, 1, 1, null, null,
                              0, 0.0, 1.0, *([null] * 14), 0, null, 0, null)
     assert rc != 0 and b"channels" in L.tgs_last_error()
-    rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, 16, 16, 12, 2, 2, null, null,
+    rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 12, 2, 2, null, null,
                              0, 0.0, 1.0, *([null] * 14), 0, null, 0, null)
     assert rc != 0 and b"tile_size" in L.tgs_last_error()
     # the packed-record workspace is required and sized by tgs_rasterize_workspace_bytes
     assert L.tgs_rasterize_workspace_bytes(0, 2, 1000) >= 2 * 1000 * 20 * 4
     assert L.tgs_rasterize_workspace_bytes(1, 2, 1000) >= 2 * 1000 * 20 * 8
-    rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, 16, 16, 16, 1, 1, null, null,
+    rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 16, 1, 1, null, null,
                              0, 0.0, 1.0, *([null] * 14), 0, null, 0, null)
     assert rc != 0 and b"workspace" in L.tgs_last_error()
     # camera and tile ids must fit in 32 bits

@@ -1,0 +1,114 @@
+"""Packed layout (the reference's ``packed=True``) against the grid layout, whose every
+stage is pinned against the oracle elsewhere: the packed projection is the grid's visible
+entries in (camera, Gaussian) order, the packed tile lists are the grid's under the entry
+-> (camera, Gaussian) map (bit-exact), and renders and gradients through the public API
+agree (float64: to rounding of the atomics' order).  Unlike the reference, each packed
+entry samples its own Gaussian's texture (Q3)."""
+import pytest
+import torch
+
+import textured_gaussians_amd as tg
+from textured_gaussians_amd import scenes
+from textured_gaussians_amd.cuda import _ops
+
+from gpu_util import DEV, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CAMS = [(3.5, 0.3, 0.4), (-3.0, 1.5, 0.8), (0.5, -3.6, -0.6)]
+
+
+def _scene(dtype, n=700, tex=10, sh_degree=1, seed=3):
+    return scenes.make_scene(n, 64, 48, 50.0, CAMS, tex, logscale_mu=-3.0, logscale_sigma=0.4,
+                             sh_degree=sh_degree, seed=seed).to(DEV, dtype)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_packed_projection_is_the_visible_grid_entries(dtype):
+    s = _scene(dtype)
+    args = (s.means, s.quats, s.scales, s.viewmats, s.Ks, s.width, s.height)
+    radii, m2, dep, rt, nrm = _ops.fully_fused_projection_fwd_2dgs(*args, 0.3, 0.01, 1e10, 0.0)
+    indptr, cid, gid, pr, pm2, pdep, prt, pnrm = _ops.fully_fused_projection_packed_fwd_2dgs(*args, 0.01, 1e10, 0.0)
+    vis = (radii > 0).nonzero()
+    assert vis.shape[0] > 0 and torch.equal(cid, vis[:, 0]) and torch.equal(gid, vis[:, 1])
+    C = s.viewmats.shape[0]
+    counts = (radii > 0).sum(1)
+    assert torch.equal(indptr, torch.cat([torch.zeros(1, device=DEV, dtype=torch.int64), counts.cumsum(0)]))
+    for a, b in ((pr, radii), (pm2, m2), (pdep, dep), (prt, rt), (pnrm, nrm)):
+        assert torch.equal(a, b[cid, gid])
+    assert indptr.numel() == C + 1
+
+
+def test_packed_tile_lists_are_the_grid_lists():
+    s = _scene(torch.float32)
+    N = s.means.shape[0]
+    C = s.viewmats.shape[0]
+    args = (s.means, s.quats, s.scales, s.viewmats, s.Ks, s.width, s.height)
+    radii, m2, dep, rt, nrm = _ops.fully_fused_projection_fwd_2dgs(*args, 0.3, 0.01, 1e10, 0.0)
+    _, cid, gid, pr, pm2, pdep, _, _ = _ops.fully_fused_projection_packed_fwd_2dgs(*args, 0.01, 1e10, 0.0)
+    tw, th = 4, 3
+    tpg, ids, flat = _ops.isect_tiles(m2, radii, dep, None, None, C, 16, tw, th, True, True)
+    ptpg, pids, pflat = _ops.isect_tiles(pm2, pr, pdep, cid, gid, C, 16, tw, th, True, True)
+    assert ids.numel() > 100
+    assert torch.equal(pids, ids)
+    assert torch.equal((cid * N + gid)[pflat.long()].int(), flat)
+    assert torch.equal(ptpg, tpg[cid, gid])
+
+
+def _render(s, L, packed, textured=True, **kw):
+    bg = s.backgrounds
+    if textured:
+        return tg.rasterization_textured_gaussians(L["means"], L["quats"], L["scales"], L["opacities"], L["colors"],
+                                                   L["textures"], s.viewmats, s.Ks, s.width, s.height,
+                                                   sh_degree=s.sh_degree, backgrounds=bg, packed=packed, **kw)
+    return tg.rasterization_2dgs(L["means"], L["quats"], L["scales"], L["opacities"], L["colors"], s.viewmats, s.Ks,
+                                 s.width, s.height, sh_degree=s.sh_degree, backgrounds=bg, packed=packed, **kw)
+
+
+@pytest.mark.parametrize("textured", [True, False])
+@pytest.mark.parametrize("grad_mode,sparse", [("reference", False), ("exact", False), ("reference", True)])
+def test_packed_render_and_gradients_match_grid(textured, grad_mode, sparse):
+    s = _scene(torch.float64)
+    names = ["means", "quats", "scales", "opacities", "colors"] + (["textures"] if textured else [])
+    res = {}
+    for packed in (False, True):
+        L = {k: getattr(s, k).detach().clone().requires_grad_(True) for k in names}
+        out = _render(s, L, packed, textured, grad_mode=grad_mode, sparse_grad=sparse and packed)
+        g = torch.Generator(device=DEV).manual_seed(1)
+        loss = (out[0] * torch.rand(out[0].shape, device=DEV, dtype=out[0].dtype, generator=g)).sum() + \
+            out[1].sum() + (out[2] * 0.3).sum()
+        loss.backward()
+        grads = {k: (v.grad.to_dense() if v.grad.is_sparse else v.grad) for k, v in L.items()}
+        res[packed] = (out, grads)
+    (o0, g0), (o1, g1) = res[False], res[True]
+    for i in (0, 1, 2, 4, 5, 6, 7):
+        assert rel_err(o1[i], o0[i]) < 1e-12, i
+    for k in g0:
+        assert rel_err(g1[k], g0[k]) < 1e-10, k
+
+
+def test_packed_textures_sample_each_entrys_gaussian():
+    """A Gaussian visible only in the second camera: its packed entry index differs from its
+    Gaussian id, and its texture (not the texture at its entry index) must colour it."""
+    s = _scene(torch.float64, n=300)
+    L = {k: getattr(s, k).detach().clone() for k in ["means", "quats", "scales", "opacities", "colors", "textures"]}
+    a = _render(s, L, False)[0]
+    b = _render(s, L, True)[0]
+    assert rel_err(b, a) < 1e-12
+    L["textures"] = L["textures"].flip(0).contiguous()  # any mix-up of rows now shows
+    a2 = _render(s, L, False)[0]
+    b2 = _render(s, L, True)[0]
+    assert rel_err(b2, a2) < 1e-12 and rel_err(a2, a) > 1e-3
+
+
+def test_packed_viewmat_gradient_matches_grid():
+    s = _scene(torch.float64)
+    res = []
+    for packed in (False, True):
+        vm = s.viewmats.clone().requires_grad_(True)
+        out = tg.rasterization_textured_gaussians(s.means, s.quats, s.scales, s.opacities, s.colors, s.textures, vm,
+                                                  s.Ks, s.width, s.height, sh_degree=s.sh_degree,
+                                                  backgrounds=s.backgrounds, packed=packed, grad_mode="exact")
+        out[0].sum().backward()
+        res.append(vm.grad)
+    assert float(res[0].abs().max()) > 0 and rel_err(res[1], res[0]) < 1e-10

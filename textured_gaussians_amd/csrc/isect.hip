@@ -153,7 +153,8 @@ __global__ __launch_bounds__(256) void isect_count_kernel(int64_t n, const float
     tpg[idx] = (R.y1 - R.y0) * (R.x1 - R.x0);
 }
 
-__global__ __launch_bounds__(256) void isect_fill_kernel(int64_t n, int64_t N, const float *__restrict__ means2d,
+__global__ __launch_bounds__(256) void isect_fill_kernel(int64_t n, int64_t N, const int64_t *__restrict__ camera_ids,
+                                                         const float *__restrict__ means2d,
                                                          const int32_t *__restrict__ radii,
                                                          const float *__restrict__ depths,
                                                          const int64_t *__restrict__ cum, int32_t ts, int32_t tw,
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256) void isect_fill_kernel(int64_t n, int64_t N, c
     const int32_t r = radii[idx];
     if (r <= 0) return;
     const Rect R = tile_rect(means2d, r, idx, ts, tw, th);
-    const int64_t cam = idx / N;
+    const int64_t cam = camera_ids ? camera_ids[idx] : idx / N;  // packed entries carry their camera
     const int64_t hi = cam << (32 + tile_bits);
     const int64_t dbits = (int64_t)__float_as_int(depths[idx]);  // sign-extended
     int64_t o = idx == 0 ? 0 : cum[idx - 1];
@@ -338,8 +339,26 @@ extern "C" int tgs_isect_tiles_fill(int64_t C, int64_t N, const float *means2d, 
     if (n == 0) return TGS_OK;
     const int tb = bit_length((int64_t)tile_width * tile_height);
     isect_fill_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, (hipStream_t)stream>>>(
-        n, N, means2d, radii, depths, cum_tiles_per_gauss, tile_size, tile_width, tile_height, tb, isect_ids,
-        flatten_ids);
+        n, N, nullptr, means2d, radii, depths, cum_tiles_per_gauss, tile_size, tile_width, tile_height, tb,
+        isect_ids, flatten_ids);
+    TGS_CHECK_LAUNCH("isect_fill");
+    return TGS_OK;
+}
+
+extern "C" int tgs_isect_tiles_fill_packed(int64_t C, int64_t nnz, const int64_t *camera_ids, const float *means2d,
+                                           const int32_t *radii, const float *depths,
+                                           const int64_t *cum_tiles_per_gauss, int32_t tile_size, int32_t tile_width,
+                                           int32_t tile_height, int64_t *isect_ids, int32_t *flatten_ids,
+                                           tgs_stream_t stream) {
+    TGS_CHECK_ARG(C >= 0 && nnz >= 0 && nnz < (int64_t(1) << 31), "isect_tiles: bad sizes");
+    TGS_CHECK_ARG(bit_length((int64_t)tile_width * tile_height) + bit_length(C) <= 32,
+                  "isect_tiles: camera and tile ids need more than 32 bits");
+    TGS_CHECK_ARG(nnz == 0 || camera_ids, "isect_tiles: packed entries need camera_ids");
+    if (nnz == 0) return TGS_OK;
+    const int tb = bit_length((int64_t)tile_width * tile_height);
+    isect_fill_kernel<<<(unsigned)ceil_div(nnz, 256), 256, 0, (hipStream_t)stream>>>(
+        nnz, 1, camera_ids, means2d, radii, depths, cum_tiles_per_gauss, tile_size, tile_width, tile_height, tb,
+        isect_ids, flatten_ids);
     TGS_CHECK_LAUNCH("isect_fill");
     return TGS_OK;
 }

@@ -92,6 +92,8 @@ constexpr int kAbc = 9;  // backward scratch: centred sums of dL/dcr per (camera
 
 template <typename S> struct RasterArgs {
     int64_t C, N, n_isects;
+    int64_t E;            // per-entry inputs: C*N (grid layout) or nnz (packed layout)
+    const int64_t *gids;  // packed layout: texture row (Gaussian) of each entry; NULL: entry % N
     const S *means2d, *rt, *colors, *opac, *normals, *textures, *bg;
     const S *rec;  // packed records [C*N][kRec]
     const uint8_t *masks;
@@ -138,7 +140,8 @@ __device__ __forceinline__ int32_t rec_int_get(float v) { return __float_as_int(
 __device__ __forceinline__ int32_t rec_int_get(double v) { return (int32_t)v; }
 
 template <typename S, int K>
-__global__ __launch_bounds__(256) void pack_kernel(int64_t n, int64_t N, const S *__restrict__ means2d,
+__global__ __launch_bounds__(256) void pack_kernel(int64_t n, int64_t N, const int64_t *__restrict__ gids,
+                                                   const S *__restrict__ means2d,
                                                    const S *__restrict__ rt, const S *__restrict__ opac,
                                                    const S *__restrict__ colors, const S *__restrict__ normals,
                                                    S *__restrict__ rec) {
@@ -161,7 +164,7 @@ __global__ __launch_bounds__(256) void pack_kernel(int64_t n, int64_t N, const S
         r[kRecA + 3 * k + 1] = (S)A[k].y;
         r[kRecA + 3 * k + 2] = (S)A[k].z;
     }
-    r[kRecRow] = rec_int<S>((int32_t)(i % N));
+    r[kRecRow] = rec_int<S>((int32_t)(gids ? gids[i] : i % N));
     r[kRecOp] = opac[i];
 #pragma unroll
     for (int k = 0; k < 4; ++k) r[kRecCol + k] = k < K ? colors[i * K + k] : S(0);
@@ -1004,8 +1007,8 @@ template <typename S> static size_t abc_bytes(int64_t C, int64_t N) {
 
 template <typename S, int K>
 static void launch_pack(const RasterArgs<S> &a, hipStream_t st) {
-    const int64_t n = a.C * a.N;
-    pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.N, a.means2d, a.rt, a.opac, a.colors,
+    const int64_t n = a.E;
+    pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.N, a.gids, a.means2d, a.rt, a.opac, a.colors,
                                                                     a.normals, (S *)a.rec);
 }
 template <typename S, int K, bool TEX> static void launch_fwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
@@ -1055,8 +1058,11 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
                         int64_t tex_h, int64_t tex_w, double trx, double try_, const void *bg, const uint8_t *masks,
                         int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th, const int32_t *offsets,
                         const int32_t *flatten_ids, int64_t n_isects, double g_weight, void *workspace,
-                        void *zdst, const uint8_t *zmask, size_t zrow_bytes, int zsel) {
+                        void *zdst, const uint8_t *zmask, size_t zrow_bytes, int zsel, const int64_t *gids,
+                        int64_t nnz) {
     memset(&a, 0, sizeof(a));
+    a.gids = gids;
+    a.E = gids ? nnz : C * N;
     a.zdst = zdst;
     a.zmask = zmask;
     a.zrow16 = (int64_t)(zrow_bytes / 16);
@@ -1066,7 +1072,7 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
     a.opac = (const S *)opac; a.normals = (const S *)normals; a.textures = (const S *)textures;
     a.bg = (const S *)bg; a.masks = masks;
     a.rec = (const S *)workspace;
-    a.g_abc = (S *)((char *)workspace + rec_bytes<S>(C, N));
+    a.g_abc = (S *)((char *)workspace + rec_bytes<S>(1, a.E));
     if (textures) {
         a.tex_h = (int32_t)tex_h; a.tex_w = (int32_t)tex_w;
         a.tex_stride = tex_h * tex_w * 4;
@@ -1155,7 +1161,8 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const void *ray_transforms, const void *colors, const void *opacities,
                                  const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
                                  double texture_range_x, double texture_range_y, const void *backgrounds,
-                                 const uint8_t *masks, int32_t width, int32_t height, int32_t tile_size,
+                                 const uint8_t *masks, const int64_t *gaussian_ids, int64_t nnz, int32_t width,
+                                 int32_t height, int32_t tile_size,
                                  int32_t tile_width, int32_t tile_height, const int32_t *tile_offsets,
                                  const int32_t *flatten_ids, int64_t n_isects, double gs_contrib_threshold,
                                  double g_weight, void *render_colors, void *render_alphas, void *render_normals,
@@ -1167,8 +1174,10 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
     int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
                           tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
-    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, C, N),
-                  "rasterize_fwd: workspace of %zu bytes required", workspace_bytes(dtype, C, N));
+    const int64_t E = gaussian_ids ? nnz : C * N;  // per-entry inputs
+    TGS_CHECK_ARG(!gaussian_ids || (nnz >= 0 && nnz < (int64_t(1) << 31)), "rasterize_fwd: bad nnz");
+    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, 1, E),
+                  "rasterize_fwd: workspace of %zu bytes required", workspace_bytes(dtype, 1, E));
     rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -1178,7 +1187,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
                        tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
-                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 1);                                       \
+                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 1, gaussian_ids, nnz);                    \
         a.thr = (S)gs_contrib_threshold;                                                                        \
         a.out_c = (S *)render_colors; a.out_a = (S *)render_alphas; a.out_n = (S *)render_normals;              \
         a.out_d = (S *)render_distort; a.out_m = (S *)render_median; a.last_ids = last_ids;                     \
@@ -1199,7 +1208,8 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const void *ray_transforms, const void *colors, const void *opacities,
                                  const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
                                  double texture_range_x, double texture_range_y, const void *backgrounds,
-                                 const uint8_t *masks, int32_t width, int32_t height, int32_t tile_size,
+                                 const uint8_t *masks, const int64_t *gaussian_ids, int64_t nnz, int32_t width,
+                                 int32_t height, int32_t tile_size,
                                  int32_t tile_width, int32_t tile_height, const int32_t *tile_offsets,
                                  const int32_t *flatten_ids, int64_t n_isects, double g_weight,
                                  const void *render_colors, const void *render_alphas, const int32_t *last_ids,
@@ -1218,8 +1228,10 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
     if (rc) return rc;
     TGS_CHECK_ARG(uv_grad_mode >= 0 && uv_grad_mode <= 2, "rasterize_bwd: bad uv_grad_mode %d", uv_grad_mode);
     TGS_CHECK_ARG(!textures || v_textures, "rasterize_bwd: v_textures required with textures");
-    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, C, N),
-                  "rasterize_bwd: workspace of %zu bytes required", workspace_bytes(dtype, C, N));
+    const int64_t E = gaussian_ids ? nnz : C * N;  // per-entry inputs
+    TGS_CHECK_ARG(!gaussian_ids || (nnz >= 0 && nnz < (int64_t(1) << 31)), "rasterize_bwd: bad nnz");
+    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, 1, E),
+                  "rasterize_bwd: workspace of %zu bytes required", workspace_bytes(dtype, 1, E));
     rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -1229,7 +1241,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
                        tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
-                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 0);                                       \
+                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 0, gaussian_ids, nnz);                    \
         a.r_c = (const S *)render_colors; a.r_a = (const S *)render_alphas; a.r_last = last_ids;                \
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \
@@ -1250,7 +1262,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.v_m = (const S *)v_render_median; a.uv_mode = uv_grad_mode;                                           \
         a.g_xy_abs = (S *)v_means2d_abs; a.g_xy = (S *)v_means2d; a.g_rt = (S *)v_ray_transforms;              \
         a.g_col = (S *)v_colors; a.g_op = (S *)v_opacities; a.g_tex = (S *)v_textures; a.g_nrm = (S *)v_normals; \
-        const int64_t n = C * N;                                                                                \
+        const int64_t n = E;                                                                                    \
         const unsigned nb = (unsigned)ceil_div(n, 256);                                                         \
         zero_kernel<S><<<(unsigned)ceil_div(n * kAbc, 256), 256, 0, st>>>(n * kAbc, a.g_abc);                   \
         TGS_CHECK_LAUNCH("rasterize_bwd zero");                                                                 \

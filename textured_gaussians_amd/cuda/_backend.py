@@ -42,16 +42,23 @@ _PROTOS = {
     "tgs_sort_pairs_temp_bytes": (c_size, [c_i64]),
     "tgs_sort_pairs": (c_int, [c_i64, c_i32, vp, vp, vp, vp, vp, c_size, ctypes.POINTER(c_i32), vp]),
     "tgs_isect_offset_encode": (c_int, [c_i64, vp, c_i64, c_i32, c_i32, vp, vp]),
+    "tgs_isect_tiles_fill_packed": (c_int, [c_i64, c_i64, vp, vp, vp, vp, vp, c_i32, c_i32, c_i32, vp, vp, vp]),
+    "tgs_projection_2dgs_packed_count": (c_int, [c_int, c_i64, c_i64, vp, vp, vp, vp, vp, c_i32, c_i32, c_dbl,
+                                                 c_dbl, c_dbl, vp, vp, vp, c_size, vp]),
+    "tgs_projection_2dgs_packed_fill": (c_int, [c_int, c_i64, c_i64, vp, vp, vp, vp, vp, c_i32, c_i32, c_dbl, c_dbl,
+                                                c_dbl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "tgs_projection_2dgs_packed_bwd": (c_int, [c_int, c_i64, c_i64, c_i64, vp, vp, c_int, vp, vp, vp, vp, vp, vp,
+                                               vp, vp, vp, vp, vp, c_int, vp, vp, vp, vp, vp]),
     "tgs_sh_fwd": (c_int, [c_int, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp]),
     "tgs_sh_bwd": (c_int, [c_int, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
     "tgs_rasterize_fwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
-                                  vp, vp, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl, c_dbl,
+                                  vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl, c_dbl,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
     "tgs_tex_record_bytes": (c_size, [c_int]),
     "tgs_prefix_sum_i32": (c_int, [c_i64, vp, vp, vp, c_size, vp]),
     "tgs_rasterize_bwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
-                                  vp, vp, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl,
+                                  vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),
     "tgs_gemm_bf16": (c_int, [c_int, c_int, c_int, c_i64, c_i64, c_i64, vp, c_i64, vp, c_i64, vp, vp, c_i64, vp,

@@ -135,6 +135,86 @@ def fully_fused_projection_bwd_exact_2dgs(*args):
     return _projection_bwd(VJP_EXACT, *args)
 
 
+def fully_fused_projection_packed_fwd_2dgs(means, quats, scales, viewmats, Ks, image_width, image_height,
+                                           near_plane, far_plane, radius_clip):
+    """Packed layout: (indptr [C+1], camera_ids [nnz], gaussian_ids [nnz], radii, means2d, depths,
+    ray_transforms, normals), the visible (camera, Gaussian) entries in (camera, Gaussian) order."""
+    for n, t in dict(means=means, quats=quats, scales=scales, viewmats=viewmats, Ks=Ks).items():
+        _check_input(t, n)
+    dt = _dtype_code(means, "means")
+    _same(means.dtype, quats=quats, scales=scales, viewmats=viewmats, Ks=Ks)
+    N, C = means.size(0), viewmats.size(0)
+    dev, fdt = means.device, means.dtype
+    L = lib()
+    args = (_p(means), _p(quats), _p(scales), _p(viewmats), _p(Ks), int(image_width), int(image_height),
+            float(near_plane), float(far_plane), float(radius_clip))
+    with torch.cuda.device(dev):
+        st = _stream(means)
+        flags = torch.empty(C * N, dtype=torch.int32, device=dev)
+        incl = torch.empty(C * N, dtype=torch.int64, device=dev)
+        temp = torch.empty(max(1, L.tgs_isect_count_temp_bytes(C * N)), dtype=torch.uint8, device=dev)
+        check(L.tgs_projection_2dgs_packed_count(dt, C, N, *args, _p(flags), _p(incl), _p(temp), temp.numel(), st),
+              "fully_fused_projection_packed_fwd_2dgs")
+        nnz = int(incl[-1].item()) if C * N > 0 else 0  # device -> host, as the reference's two-pass kernel
+        indptr = torch.empty(C + 1, dtype=torch.int64, device=dev)
+        camera_ids = torch.empty(nnz, dtype=torch.int64, device=dev)
+        gaussian_ids = torch.empty(nnz, dtype=torch.int64, device=dev)
+        radii = torch.empty(nnz, dtype=torch.int32, device=dev)
+        means2d = torch.empty((nnz, 2), dtype=fdt, device=dev)
+        depths = torch.empty(nnz, dtype=fdt, device=dev)
+        rt = torch.empty((nnz, 3, 3), dtype=fdt, device=dev)
+        normals = torch.empty((nnz, 3), dtype=fdt, device=dev)
+        check(L.tgs_projection_2dgs_packed_fill(dt, C, N, *args, _p(incl), _p(indptr), _p(camera_ids),
+                                                _p(gaussian_ids), _p(radii), _p(means2d), _p(depths), _p(rt),
+                                                _p(normals), st), "fully_fused_projection_packed_fwd_2dgs")
+    return indptr, camera_ids, gaussian_ids, radii, means2d, depths, rt, normals
+
+
+def _projection_packed_bwd(mode, means, quats, scales, viewmats, Ks, image_width, image_height, camera_ids,
+                           gaussian_ids, ray_transforms, v_means2d, v_depths, v_normals, v_ray_transforms,
+                           viewmats_requires_grad, sparse_grad):
+    ts = dict(means=means, quats=quats, scales=scales, viewmats=viewmats, Ks=Ks, camera_ids=camera_ids,
+              gaussian_ids=gaussian_ids, ray_transforms=ray_transforms, v_means2d=v_means2d, v_depths=v_depths,
+              v_normals=v_normals, v_ray_transforms=v_ray_transforms)
+    for n, t in ts.items():
+        _check_input(t, n)
+    dt = _dtype_code(means, "means")
+    N, C, nnz = means.size(0), viewmats.size(0), camera_ids.numel()
+    with torch.cuda.device(means.device):
+        if sparse_grad:  # per-entry values, as the reference's sparse gradients
+            v_means = torch.zeros((nnz, 3), dtype=means.dtype, device=means.device)
+            v_quats = torch.zeros((nnz, 4), dtype=means.dtype, device=means.device)
+            v_scales = torch.zeros((nnz, 3), dtype=means.dtype, device=means.device)
+        else:
+            v_means = torch.zeros_like(means)
+            v_quats = torch.zeros_like(quats)
+            v_scales = torch.zeros_like(scales)
+        v_viewmats = torch.zeros_like(viewmats) if viewmats_requires_grad else None
+        vv = v_viewmats if (viewmats_requires_grad and mode == VJP_EXACT) else None
+        check(lib().tgs_projection_2dgs_packed_bwd(dt, C, N, nnz, _p(camera_ids), _p(gaussian_ids), int(sparse_grad),
+                                                   _p(means), _p(quats), _p(scales), _p(viewmats), _p(Ks), None,
+                                                   _p(ray_transforms), _p(v_means2d), _p(v_depths), _p(v_normals),
+                                                   _p(v_ray_transforms), mode, _p(v_means), _p(v_quats),
+                                                   _p(v_scales), _p(vv), _stream(means)),
+              "fully_fused_projection_packed_bwd_2dgs")
+    return v_means, v_quats, v_scales, v_viewmats
+
+
+def fully_fused_projection_packed_bwd_2dgs(*args):
+    """Reference VJP, packed layout."""
+    return _projection_packed_bwd(VJP_REFERENCE, *args)
+
+
+def fully_fused_projection_packed_bwd2_2dgs(*args):
+    """Reference ``norm_rot_grad`` VJP, packed layout."""
+    return _projection_packed_bwd(VJP_REFERENCE_NORM, *args)
+
+
+def fully_fused_projection_packed_bwd_exact_2dgs(*args):
+    """True derivative, packed layout (no reference counterpart)."""
+    return _projection_packed_bwd(VJP_EXACT, *args)
+
+
 # ------------------------------------------------------------------- isect
 def _bit_length(x: int) -> int:
     return int(x).bit_length()
@@ -145,15 +225,17 @@ def isect_tiles(means2d, radii, depths, camera_ids, gaussian_ids, C, tile_size, 
     _check_input(means2d, "means2d")
     _check_input(radii, "radii")
     _check_input(depths, "depths")
-    if means2d.dim() == 2:
+    packed = means2d.dim() == 2
+    if packed:
         if camera_ids is None or gaussian_ids is None:
             raise RuntimeError("When packed is set, camera_ids and gaussian_ids must be provided.")
-        raise NotImplementedError("isect_tiles: the packed layout is not implemented")
+        _check_input(camera_ids, "camera_ids")
     if means2d.dtype != torch.float32 or depths.dtype != torch.float32:
         # the key layout stores float32 depth bits (isect_tiles.cu:92)
         means2d = means2d.float().contiguous()
         depths = depths.float().contiguous()
-    C_, N = means2d.size(0), means2d.size(1)
+    # packed entries are counted as one row of nnz; the camera comes from camera_ids
+    C_, N = (1, means2d.size(0)) if packed else (means2d.size(0), means2d.size(1))
     n = C_ * N
     dev = means2d.device
     L = lib()
@@ -163,7 +245,7 @@ def isect_tiles(means2d, radii, depths, camera_ids, gaussian_ids, C, tile_size, 
     assert tile_bits + cam_bits <= 32
     with torch.cuda.device(dev):
         st = _stream(means2d)
-        tpg = torch.empty((C_, N), dtype=torch.int32, device=dev)
+        tpg = torch.empty((N,) if packed else (C_, N), dtype=torch.int32, device=dev)
         if n == 0:
             return tpg, torch.empty(0, dtype=torch.int64, device=dev), torch.empty(0, dtype=torch.int32, device=dev)
         cum = torch.empty(n, dtype=torch.int64, device=dev)
@@ -176,8 +258,14 @@ def isect_tiles(means2d, radii, depths, camera_ids, gaussian_ids, C, tile_size, 
         flatten_ids = torch.empty(n_isects, dtype=torch.int32, device=dev)
         if n_isects == 0:
             return tpg, isect_ids, flatten_ids
-        check(L.tgs_isect_tiles_fill(C_, N, _p(means2d), _p(radii), _p(depths), _p(cum), tile_size, tile_width,
-                                     tile_height, _p(isect_ids), _p(flatten_ids), st), "isect_tiles")
+        if packed:
+            check(L.tgs_isect_tiles_fill_packed(C, N, _p(camera_ids), _p(means2d), _p(radii), _p(depths), _p(cum),
+                                                tile_size, tile_width, tile_height, _p(isect_ids),
+                                                _p(flatten_ids), st), "isect_tiles")
+        else:
+            check(L.tgs_isect_tiles_fill(C_, N, _p(means2d), _p(radii), _p(depths), _p(cum), tile_size,
+                                         tile_width, tile_height, _p(isect_ids), _p(flatten_ids), st),
+                  "isect_tiles")
         if not sort:
             return tpg, isect_ids, flatten_ids
         keys_alt = torch.empty_like(isect_ids)
@@ -257,9 +345,29 @@ def _zero_rows_args(zero_rows, N):
     return _p(dst), _p(mask), dst[0].numel() * dst.element_size()
 
 
+def _packed_args(means2d, gaussian_ids, textures, n_gaussians):
+    """(gaussian_ids or None, nnz, N) for the grid ([C, N, ...]) or packed ([nnz, ...]) layout."""
+    if means2d.dim() == 3:
+        return None, 0, means2d.size(1)
+    if gaussian_ids is None:
+        # the reference indexes the textures with the packed entry index here (Q3); this build
+        # needs the Gaussian of each entry
+        raise NotImplementedError("rasterize_to_pixels: the packed layout needs gaussian_ids")
+    _check_input(gaussian_ids, "gaussian_ids")
+    if gaussian_ids.dtype != torch.int64 or gaussian_ids.numel() != means2d.size(0):
+        raise RuntimeError("gaussian_ids must be int64 [nnz]")
+    if textures is not None:
+        N = textures.size(0)
+    elif n_gaussians is not None:
+        N = int(n_gaussians)
+    else:
+        N = int(gaussian_ids.max().item()) + 1 if gaussian_ids.numel() else 1
+    return gaussian_ids, means2d.size(0), N
+
+
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
-                g_weight, zero_rows=None, live_entries=None, tex_counts=None):
+                g_weight, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None, n_gaussians=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, tile_offsets=tile_offsets, flatten_ids=flatten_ids)
     if textured:
@@ -270,13 +378,11 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         _check_input(backgrounds, "backgrounds")
     if masks is not None:
         _check_input(masks, "masks")
-    if means2d.dim() != 3:
-        raise NotImplementedError("rasterize_to_pixels: the packed layout is not implemented")
     dt = _dtype_code(means2d, "means2d")
     _same(means2d.dtype, ray_transforms=ray_transforms, colors=colors, opacities=opacities, normals=normals,
           textures=textures if textured else None, backgrounds=backgrounds)
     C = tile_offsets.size(0)
-    N = means2d.size(1)
+    gids, nnz, N = _packed_args(means2d, gaussian_ids, textures if textured else None, n_gaussians)
     K = colors.size(-1)
     th_t, tw_t = tile_offsets.size(1), tile_offsets.size(2)
     H, W = image_height, image_width
@@ -296,13 +402,13 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         csum = torch.zeros((N,), dtype=fdt, device=dev)
         ccnt = torch.zeros((N,), dtype=fdt, device=dev)
         accd = torch.empty((C, H, W), dtype=fdt, device=dev)
-        ws = _raster_workspace(dt, C, N, dev)
+        ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
         zd, zm, zb = _zero_rows_args(zero_rows, N)
         with _timed("raster_fwd", means2d):
             check(lib().tgs_rasterize_fwd(
               dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
               _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
-              _p(masks.view(torch.uint8)) if masks is not None else None, W, H, tile_size, tw_t, th_t,
+              _p(masks.view(torch.uint8)) if masks is not None else None, _p(gids), nnz, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
               _p(csum), _p(ccnt), _p(accd), _live_arg(live_entries, flatten_ids, tile_size),
@@ -368,7 +474,8 @@ def _tex_record_args(tex_counts, dt, N, dev):
     return tex_counts, incl, cursor, recs, temp
 
 
-def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None, tex_counts=None):
+def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None, tex_counts=None,
+                                                    gaussian_ids=None):
     """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
     per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W].
     ``zero_rows`` = (buffer [N, ...], uint8 mask [N]): rows with mask 1 are
@@ -377,7 +484,8 @@ def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_
     for the backward to visit only those.  ``tex_counts`` (from ``tex_counts_buffer``)
     receives per texture row the pairs that will add texture gradient, for the
     backward's record-and-gather path."""
-    return _raster_fwd(True, *args, zero_rows=zero_rows, live_entries=live_entries, tex_counts=tex_counts)
+    return _raster_fwd(True, *args, zero_rows=zero_rows, live_entries=live_entries, tex_counts=tex_counts,
+                       gaussian_ids=gaussian_ids)
 
 
 def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
@@ -390,17 +498,19 @@ def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, nor
 
 def rasterize_to_pixels_fwd_2dgs_full(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
                                       image_width, image_height, tile_size, tile_offsets, flatten_ids,
-                                      gs_contrib_threshold, live_entries=None):
+                                      gs_contrib_threshold, live_entries=None, gaussian_ids=None, n_gaussians=None):
     return _raster_fwd(False, means2d, ray_transforms, colors, opacities, None, 1.0, 1.0, normals, backgrounds,
                        masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
-                       gs_contrib_threshold, 1.0, live_entries=live_entries)
+                       gs_contrib_threshold, 1.0, live_entries=live_entries, gaussian_ids=gaussian_ids,
+                       n_gaussians=n_gaussians)
 
 
 def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals,
                 densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
-                v_textures_out=None, zero_rows=None, live_entries=None, tex_counts=None):
+                v_textures_out=None, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None,
+                n_gaussians=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -415,11 +525,9 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         _check_input(backgrounds, "backgrounds")
     if masks is not None:
         _check_input(masks, "masks")
-    if means2d.dim() != 3:
-        raise NotImplementedError("rasterize_to_pixels: the packed layout is not implemented")
     dt = _dtype_code(means2d, "means2d")
     C = tile_offsets.size(0)
-    N = means2d.size(1)
+    gids, nnz, N = _packed_args(means2d, gaussian_ids, textures if textured else None, n_gaussians)
     K = colors.size(-1)
     th_t, tw_t = tile_offsets.size(1), tile_offsets.size(2)
     dev = means2d.device
@@ -441,7 +549,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_opacities = torch.zeros_like(opacities)
         v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
         v_densify = torch.zeros_like(densify)
-        ws = _raster_workspace(dt, C, N, dev)
+        ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
         zd, zm, zb = _zero_rows_args(zero_rows, N)
         rc_, ri_, rcur_, rrec_, _tmp = _tex_record_args(tex_counts if textured else None, dt, textures.shape[0]
                                                         if textured else 0, dev)
@@ -449,8 +557,9 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
             check(lib().tgs_rasterize_bwd(
               dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
               _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
-              _p(masks.view(torch.uint8)) if masks is not None else None, image_width, image_height, tile_size,
-              tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(g_weight),
+              _p(masks.view(torch.uint8)) if masks is not None else None, _p(gids), nnz, image_width,
+              image_height, tile_size, tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(),
+              float(g_weight),
               _p(render_colors), _p(render_alphas), _p(last_ids), _p(median_ids), _p(render_accum_depth),
               _live_arg(live_entries, flatten_ids, tile_size), _p(rc_), _p(ri_), _p(rcur_), _p(rrec_),
               _p(v_render_colors),
@@ -464,41 +573,42 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
 
 
 def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                               live_entries=None, tex_counts=None):
+                                               live_entries=None, tex_counts=None, gaussian_ids=None):
     """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
     return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth,
                        v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
-                       tex_counts=tex_counts)
+                       tex_counts=tex_counts, gaussian_ids=gaussian_ids)
 
 
 def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                                live_entries=None, tex_counts=None):
+                                                live_entries=None, tex_counts=None, gaussian_ids=None):
     """Reference ``bilinear_bwd2`` rule (du/ds = (res-1)/range)."""
     return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth,
                        v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
-                       tex_counts=tex_counts)
+                       tex_counts=tex_counts, gaussian_ids=gaussian_ids)
 
 
 def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                                     live_entries=None, tex_counts=None):
+                                                     live_entries=None, tex_counts=None, gaussian_ids=None):
     """True derivative: du/ds = (res-1)/(2 range), and the forward's own
     distortion accumulator when ``render_accum_depth`` is given (no reference
     counterpart)."""
     return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth,
                        v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
-                       tex_counts=tex_counts)
+                       tex_counts=tex_counts, gaussian_ids=gaussian_ids)
 
 
 def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
                                  image_width, image_height, tile_size, tile_offsets, flatten_ids, render_colors,
                                  render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                                  v_render_normals, v_render_distort, v_render_median, absgrad,
-                                 render_accum_depth=None, live_entries=None):
+                                 render_accum_depth=None, live_entries=None, gaussian_ids=None, n_gaussians=None):
     return _raster_bwd(False, UV_NONE, means2d, ray_transforms, colors, opacities, None, 1.0, 1.0, normals,
                        densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets,
                        flatten_ids, 1.0, render_colors, render_alphas, last_ids, median_ids, v_render_colors,
                        v_render_alphas, v_render_normals, v_render_distort, v_render_median, absgrad,
-                       render_accum_depth=render_accum_depth, live_entries=live_entries)
+                       render_accum_depth=render_accum_depth, live_entries=live_entries, gaussian_ids=gaussian_ids,
+                       n_gaussians=n_gaussians)
 
 
 # ------------------------------------------------------------------ Adam

@@ -96,7 +96,10 @@ def fully_fused_projection_2dgs(means: Tensor, quats: Tensor, scales: Tensor, vi
     if sparse_grad:
         assert packed, "sparse_grad is only supported when packed is True"
     if packed:
-        raise NotImplementedError("fully_fused_projection_2dgs: packed=True is not implemented")
+        return _FullyFusedProjectionPacked2DGS.apply(means.contiguous(), quats.contiguous(), scales.contiguous(),
+                                                     viewmats.contiguous(), Ks.contiguous(), width, height,
+                                                     near_plane, far_plane, radius_clip, sparse_grad,
+                                                     norm_rot_grad, proj_grad)
     return _FullyFusedProjection2DGS.apply(means.contiguous(), quats.contiguous(), scales.contiguous(),
                                            viewmats.contiguous(), Ks.contiguous(), width, height, eps2d,
                                            near_plane, far_plane, radius_clip, norm_rot_grad, proj_grad)
@@ -128,6 +131,48 @@ class _FullyFusedProjection2DGS(torch.autograd.Function):
             means, quats, scales, viewmats, Ks, ctx.width, ctx.height, radii, ray_transforms,
             v_means2d.contiguous(), v_depths.contiguous(), v_normals.contiguous(), v_ray_transforms.contiguous(),
             ctx.needs_input_grad[3])
+        g = ctx.needs_input_grad
+        return (v_means if g[0] else None, v_quats if g[1] else None, v_scales if g[2] else None,
+                v_viewmats if g[3] else None, None, None, None, None, None, None, None, None, None)
+
+
+class _FullyFusedProjectionPacked2DGS(torch.autograd.Function):
+    """Packed layout ([W]:1487-1644): returns (camera_ids, gaussian_ids, radii, means2d,
+    depths, ray_transforms, normals) of the visible entries."""
+
+    @staticmethod
+    def forward(ctx, means, quats, scales, viewmats, Ks, width, height, near_plane, far_plane, radius_clip,
+                sparse_grad, norm_rot_grad, proj_grad):
+        (indptr, camera_ids, gaussian_ids, radii, means2d, depths, ray_transforms,
+         normals) = _make_lazy_cuda_func("fully_fused_projection_packed_fwd_2dgs")(
+            means, quats, scales, viewmats, Ks, width, height, near_plane, far_plane, radius_clip)
+        ctx.save_for_backward(camera_ids, gaussian_ids, means, quats, scales, viewmats, Ks, ray_transforms)
+        ctx.width, ctx.height = width, height
+        ctx.sparse_grad, ctx.norm_rot_grad, ctx.proj_grad = sparse_grad, norm_rot_grad, proj_grad
+        ctx.mark_non_differentiable(camera_ids, gaussian_ids, radii)
+        return camera_ids, gaussian_ids, radii, means2d, depths, ray_transforms, normals
+
+    @staticmethod
+    def backward(ctx, v_camera_ids, v_gaussian_ids, v_radii, v_means2d, v_depths, v_ray_transforms, v_normals):
+        camera_ids, gaussian_ids, means, quats, scales, viewmats, Ks, ray_transforms = ctx.saved_tensors
+        if ctx.proj_grad == "exact":
+            name = "fully_fused_projection_packed_bwd_exact_2dgs"
+        elif ctx.norm_rot_grad:
+            name = "fully_fused_projection_packed_bwd2_2dgs"
+        else:
+            name = "fully_fused_projection_packed_bwd_2dgs"
+        z = lambda t, shape: t.contiguous() if t is not None else torch.zeros(shape, device=means.device,  # noqa: E731
+                                                                              dtype=means.dtype)
+        nnz = camera_ids.numel()
+        v_means, v_quats, v_scales, v_viewmats = _make_lazy_cuda_func(name)(
+            means, quats, scales, viewmats, Ks, ctx.width, ctx.height, camera_ids, gaussian_ids, ray_transforms,
+            z(v_means2d, (nnz, 2)), z(v_depths, (nnz,)), z(v_normals, (nnz, 3)), z(v_ray_transforms, (nnz, 3, 3)),
+            ctx.needs_input_grad[3], ctx.sparse_grad)
+        if ctx.sparse_grad:
+            # as the reference: sparse COO tensors over the entries' Gaussians
+            def sp(v, like):
+                return torch.sparse_coo_tensor(gaussian_ids[None], v, like.size(), is_coalesced=len(viewmats) == 1)
+            v_means, v_quats, v_scales = sp(v_means, means), sp(v_quats, quats), sp(v_scales, scales)
         g = ctx.needs_input_grad
         return (v_means if g[0] else None, v_quats if g[1] else None, v_scales if g[2] else None,
                 v_viewmats if g[3] else None, None, None, None, None, None, None, None, None, None)
@@ -199,23 +244,37 @@ def rasterize_to_pixels_textured_gaussians(
         image_height: int, tile_size: int, isect_offsets: Tensor, flatten_ids: Tensor,
         backgrounds: Optional[Tensor] = None, masks: Optional[Tensor] = None, packed: bool = False,
         absgrad: bool = False, distloss: bool = False, gs_contrib_threshold: float = 0.0,
-        filtering: str = "bilinear", g_weight: float = 1.0):
+        filtering: str = "bilinear", g_weight: float = 1.0, gaussian_ids: Optional[Tensor] = None):
     """Rasterize textured 2D Gaussians to pixels.
 
     textures [N, th, tw, 4] (RGBA, bilinear, sampled over splat coordinates
     [-texture_range, texture_range]).  Returns (render_colors [C,H,W,D],
     render_alphas [C,H,W,1], render_normals [C,H,W,3], render_distort
     [C,H,W,1], render_median [C,H,W,1], gs_contrib_sum [N], gs_contrib_count [N]).
+    ``packed``: the per-Gaussian inputs are the [nnz, ...] entries of the packed
+    projection and ``gaussian_ids`` [nnz] (no reference counterpart) names each
+    entry's Gaussian, whose texture it samples (the reference samples the
+    texture of the entry index there).
     """
     C = isect_offsets.size(0)
+    N = textures.shape[0]
     if packed:
-        raise NotImplementedError("rasterize_to_pixels_textured_gaussians: packed=True is not implemented")
-    N = means2d.size(1)
-    assert means2d.shape == (C, N, 2), means2d.shape
-    assert ray_transforms.shape == (C, N, 3, 3), ray_transforms.shape
-    assert colors.shape[:2] == (C, N), colors.shape
-    assert opacities.shape == (C, N), opacities.shape
-    assert textures.dim() == 4 and textures.shape[0] == N and textures.shape[-1] == 4, textures.shape
+        if gaussian_ids is None:
+            raise NotImplementedError("rasterize_to_pixels_textured_gaussians: packed=True needs gaussian_ids")
+        nnz = means2d.size(0)
+        assert means2d.shape == (nnz, 2), means2d.shape
+        assert ray_transforms.shape == (nnz, 3, 3), ray_transforms.shape
+        assert colors.shape[0] == nnz and colors.dim() == 2, colors.shape
+        assert opacities.shape == (nnz,), opacities.shape
+        assert gaussian_ids.shape == (nnz,), gaussian_ids.shape
+        gaussian_ids = gaussian_ids.contiguous()
+    else:
+        gaussian_ids = None
+        assert means2d.shape == (C, N, 2), means2d.shape
+        assert ray_transforms.shape == (C, N, 3, 3), ray_transforms.shape
+        assert colors.shape[:2] == (C, N), colors.shape
+        assert opacities.shape == (C, N), opacities.shape
+    assert textures.dim() == 4 and textures.shape[-1] == 4, textures.shape
     if backgrounds is not None:
         assert backgrounds.shape == (C, colors.shape[-1]), backgrounds.shape
         backgrounds = backgrounds.contiguous()
@@ -229,7 +288,8 @@ def rasterize_to_pixels_textured_gaussians(
         tgs_fns[filtering][0], tgs_fns[filtering][1], means2d.contiguous(), ray_transforms.contiguous(),
         colors.contiguous(), opacities.contiguous(), textures.contiguous(), texture_range_x, texture_range_y,
         normals.contiguous(), densify.contiguous(), backgrounds, masks, image_width, image_height, tile_size,
-        isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad, distloss, gs_contrib_threshold, g_weight)
+        isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad, distloss, gs_contrib_threshold, g_weight,
+        gaussian_ids)
     render_colors = out[0]
     if padded > 0:
         render_colors = render_colors[..., :-padded]
@@ -247,7 +307,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
     @staticmethod
     def forward(ctx, fwd_fn, bwd_fn, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                 texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
-                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight):
+                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight, gaussian_ids):
         # The texture gradient is as large as the textures (40 GB at 1M x 50x50) and is never
         # zero-filled on its own.  The backward zeroes the rows of Gaussians absent from the
         # tile lists along its walk.  The rows of listed Gaussians are zeroed by the forward
@@ -260,7 +320,9 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
         zero_rows = None
         if ctx.needs_input_grad[6] and textures.is_cuda:
             v_tex = torch.empty_like(textures)
-            rows = _C.mark_rows(flatten_ids, textures.shape[0])
+            # texture rows present in the tile lists (packed: the entries' Gaussians)
+            listed = flatten_ids if gaussian_ids is None else gaussian_ids[flatten_ids.long()].int()
+            rows = _C.mark_rows(listed, textures.shape[0])
             ctx.v_tex = (v_tex, rows)
             if TEXTURE_GRAD_RECORDS:
                 ctx.tex_counts = _C.tex_counts_buffer(textures)
@@ -273,10 +335,10 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
          gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func(fwd_fn + "_full")(
             means2d, ray_transforms, colors, opacities, textures, texture_range_x, texture_range_y, normals,
             backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
-            g_weight, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts)
+            g_weight, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds,
                               masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
-                              median_ids, accum_depth)
+                              median_ids, accum_depth, gaussian_ids)
         ctx.bwd_fn = bwd_fn
         ctx.texture_range = (texture_range_x, texture_range_y)
         ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
@@ -289,7 +351,8 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
     def backward(ctx, v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
                  v_gs_contrib_sum, v_gs_contrib_count):
         (means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds, masks, isect_offsets,
-         flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth) = ctx.saved_tensors
+         flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth,
+         gaussian_ids) = ctx.saved_tensors
         exact = ctx.bwd_fn == tgs_fns["bilinear_exact"][1]
         v_tex_buf, zero_rows = None, None
         if ctx.v_tex is not None:
@@ -303,7 +366,8 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             ctx.g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors.contiguous(),
             v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
             v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None,
-            v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts)
+            v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts,
+            gaussian_ids=gaussian_ids)
         ctx.live = None
         ctx.tex_counts = None
         if ctx.absgrad:
@@ -312,7 +376,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
         if ctx.needs_input_grad[11]:
             v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
         return (None, None, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, None, None, v_normals,
-                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None)
+                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None, None)
 
 
 def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Tensor, opacities: Tensor,
@@ -320,18 +384,30 @@ def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Te
                              tile_size: int, isect_offsets: Tensor, flatten_ids: Tensor,
                              backgrounds: Optional[Tensor] = None, masks: Optional[Tensor] = None,
                              packed: bool = False, absgrad: bool = False, distloss: bool = False,
-                             gs_contrib_threshold: float = 0.0, grad_mode: str = "reference"):
+                             gs_contrib_threshold: float = 0.0, grad_mode: str = "reference",
+                             gaussian_ids: Optional[Tensor] = None, n_gaussians: Optional[int] = None):
     """Rasterize plain 2D Gaussians to pixels (``alpha = min(.999, o exp(-sigma))``).
     ``grad_mode="exact"`` uses the forward's own distortion accumulator in the
-    backward (the reference seeds it from ``render_colors``)."""
+    backward (the reference seeds it from ``render_colors``).  ``packed``: [nnz, ...]
+    entries with ``gaussian_ids`` [nnz] naming their Gaussians (the contribution
+    counters are per Gaussian, ``n_gaussians`` of them)."""
     C = isect_offsets.size(0)
     if packed:
-        raise NotImplementedError("rasterize_to_pixels_2dgs: packed=True is not implemented")
-    N = means2d.size(1)
-    assert means2d.shape == (C, N, 2), means2d.shape
-    assert ray_transforms.shape == (C, N, 3, 3), ray_transforms.shape
-    assert colors.shape[:2] == (C, N), colors.shape
-    assert opacities.shape == (C, N), opacities.shape
+        if gaussian_ids is None:
+            raise NotImplementedError("rasterize_to_pixels_2dgs: packed=True needs gaussian_ids")
+        nnz = means2d.size(0)
+        assert means2d.shape == (nnz, 2), means2d.shape
+        assert ray_transforms.shape == (nnz, 3, 3), ray_transforms.shape
+        assert colors.shape[0] == nnz and colors.dim() == 2, colors.shape
+        assert opacities.shape == (nnz,), opacities.shape
+        gaussian_ids = gaussian_ids.contiguous()
+    else:
+        gaussian_ids, n_gaussians = None, None
+        N = means2d.size(1)
+        assert means2d.shape == (C, N, 2), means2d.shape
+        assert ray_transforms.shape == (C, N, 3, 3), ray_transforms.shape
+        assert colors.shape[:2] == (C, N), colors.shape
+        assert opacities.shape == (C, N), opacities.shape
     if backgrounds is not None:
         assert backgrounds.shape == (C, colors.shape[-1]), backgrounds.shape
         backgrounds = backgrounds.contiguous()
@@ -343,7 +419,7 @@ def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Te
         means2d.contiguous(), ray_transforms.contiguous(), colors.contiguous(), opacities.contiguous(),
         normals.contiguous(), densify.contiguous(), backgrounds, masks, image_width, image_height, tile_size,
         isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad, distloss, gs_contrib_threshold,
-        grad_mode == "exact")
+        grad_mode == "exact", gaussian_ids, n_gaussians)
     render_colors = out[0]
     if padded > 0:
         render_colors = render_colors[..., :-padded]
@@ -353,12 +429,14 @@ def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Te
 class _RasterizeToPixels2DGS(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks, width,
-                height, tile_size, isect_offsets, flatten_ids, absgrad, distloss, gs_contrib_threshold, exact):
+                height, tile_size, isect_offsets, flatten_ids, absgrad, distloss, gs_contrib_threshold, exact,
+                gaussian_ids, n_gaussians):
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
+        ctx.packed = dict(gaussian_ids=gaussian_ids, n_gaussians=n_gaussians)
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func("rasterize_to_pixels_fwd_2dgs_full")(
             means2d, ray_transforms, colors, opacities, normals, backgrounds, masks, width, height, tile_size,
-            isect_offsets, flatten_ids, gs_contrib_threshold, live_entries=ctx.live)
+            isect_offsets, flatten_ids, gs_contrib_threshold, live_entries=ctx.live, **ctx.packed)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
                               isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids,
                               accum_depth)
@@ -379,7 +457,7 @@ class _RasterizeToPixels2DGS(torch.autograd.Function):
             ctx.height, ctx.tile_size, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
             median_ids, v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
             v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad,
-            render_accum_depth=accum_depth if ctx.exact else None, live_entries=ctx.live)
+            render_accum_depth=accum_depth if ctx.exact else None, live_entries=ctx.live, **ctx.packed)
         ctx.live = None
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
@@ -387,4 +465,4 @@ class _RasterizeToPixels2DGS(torch.autograd.Function):
         if ctx.needs_input_grad[6]:
             v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
         return (v_means2d, v_ray_transforms, v_colors, v_opacities, v_normals, v_densify, v_backgrounds, None,
-                None, None, None, None, None, None, None, None, None)
+                None, None, None, None, None, None, None, None, None, None, None)

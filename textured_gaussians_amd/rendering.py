@@ -7,7 +7,9 @@ are the MI355X ops of ``textured_gaussians_amd.cuda``.  Additions:
 ``proj_grad="exact"`` (true projection VJP) and ``filtering="bilinear_exact"``
 (true texture-coordinate gradient); ``grad_mode="exact"`` sets both.
 Differences: the ``densify`` dummy lives on the inputs' device (the reference
-hard-codes ``device="cuda"``, ``rendering.py:1541-1543``).
+hard-codes ``device="cuda"``, ``rendering.py:1541-1543``); with ``packed=True``
+each entry samples the texture of its own Gaussian (the reference uses the
+entry index, Q3).
 """
 import math
 from typing import Dict, Literal, Optional, Tuple
@@ -53,11 +55,16 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
              proj_grad):
     """Projection, tile intersection, per-view colours ([RD]:1507-1594)."""
     N, C = means.shape[0], viewmats.shape[0]
-    radii, means2d, depths, ray_transforms, normals = fully_fused_projection_2dgs(
+    proj = fully_fused_projection_2dgs(
         means, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane, radius_clip, packed,
         sparse_grad, norm_rot_grad, proj_grad=proj_grad)
-    opacities = opacities.repeat(C, 1)
-    camera_ids, gaussian_ids = None, None
+    if packed:
+        camera_ids, gaussian_ids, radii, means2d, depths, ray_transforms, normals = proj
+        opacities = opacities[gaussian_ids]
+    else:
+        radii, means2d, depths, ray_transforms, normals = proj
+        opacities = opacities.repeat(C, 1)
+        camera_ids, gaussian_ids = None, None
     densify = torch.zeros_like(means2d, dtype=means.dtype, requires_grad=True, device=means.device)
     tile_width = math.ceil(width / float(tile_size))
     tile_height = math.ceil(height / float(tile_size))
@@ -66,10 +73,15 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
         camera_ids=camera_ids, gaussian_ids=gaussian_ids)
     isect_offsets = isect_offset_encode(isect_ids, C, tile_width, tile_height)
     if not (colors.dim() == 3 and sh_degree is None):
-        colors = colors.expand(C, *([-1] * colors.dim()))
+        colors = colors[gaussian_ids] if packed else colors.expand(C, *([-1] * colors.dim()))
+    elif packed:
+        colors = colors[camera_ids, gaussian_ids, :]
     if sh_degree is not None:
         camtoworlds = torch.inverse(viewmats)
-        dirs = means[None, :, :] - camtoworlds[:, None, :3, 3]
+        if packed:
+            dirs = means[gaussian_ids, :] - camtoworlds[camera_ids, :3, 3]
+        else:
+            dirs = means[None, :, :] - camtoworlds[:, None, :3, 3]
         colors = spherical_harmonics(sh_degree, dirs, colors, masks=radii > 0)
         colors = torch.clamp_min(colors + 0.5, 0.0)
     if render_mode in ["RGB+D", "RGB+ED"]:
@@ -140,7 +152,8 @@ def rasterization_textured_gaussians(
         meta["means2d"], meta["ray_transforms"], colors, opac_c, textures, texture_range_x, texture_range_y,
         meta["normals"], densify, width, height, tile_size, meta["isect_offsets"], meta["flatten_ids"],
         backgrounds=backgrounds, packed=packed, absgrad=absgrad, distloss=distloss,
-        gs_contrib_threshold=gs_contrib_threshold, filtering=filtering, g_weight=g_weight)
+        gs_contrib_threshold=gs_contrib_threshold, filtering=filtering, g_weight=g_weight,
+        gaussian_ids=meta["gaussian_ids"])
     render_colors, render_normals, render_normals_from_depth = _finish(
         render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, viewmats, Ks)
     meta["render_distort"] = render_distort
@@ -171,7 +184,8 @@ def rasterization_2dgs(
      gs_contrib_count) = rasterize_to_pixels_2dgs(
         meta["means2d"], meta["ray_transforms"], colors, opac_c, meta["normals"], densify, width, height,
         tile_size, meta["isect_offsets"], meta["flatten_ids"], backgrounds=backgrounds, packed=packed,
-        absgrad=absgrad, distloss=distloss, gs_contrib_threshold=gs_contrib_threshold, grad_mode=raster_grad)
+        absgrad=absgrad, distloss=distloss, gs_contrib_threshold=gs_contrib_threshold, grad_mode=raster_grad,
+        gaussian_ids=meta["gaussian_ids"], n_gaussians=N)
     render_colors, render_normals, render_normals_from_depth = _finish(
         render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, viewmats, Ks)
     meta["render_distort"] = render_distort
