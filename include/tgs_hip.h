@@ -33,6 +33,9 @@ extern "C" {
 typedef struct ihipStream_t *tgs_stream_t;
 
 enum { TGS_F32 = 0, TGS_F64 = 1 };
+/* OR-ed into the dtype of the rasterizers: textures stored as bfloat16 RGBA (float32
+ * compute only; the texture gradient stays float32). */
+enum { TGS_TEX_BF16 = 0x100 };
 enum { TGS_OK = 0, TGS_ERR_ARG = 1, TGS_ERR_LAUNCH = 2, TGS_ERR_UNSUPPORTED = 3 };
 
 /* Gradient rule for the bilinear texture lookup w.r.t. the splat coordinate.
@@ -266,7 +269,10 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
  * row once: rows with zero_rows_mask == 1 are written in full (so they need no
  * zero-fill beforehand; the rows with mask 0 are zero-filled by this call), or,
  * with zero_rows_dst NULL, the sums are added to v_textures.  Rows larger than
- * 64 KB fall back to direct accumulation.
+ * 64 KB fall back to direct accumulation.  With v_textures NULL (and
+ * zero_rows_dst NULL) only the records are written, for tgs_texture_adam_rows.
+ * dtype | TGS_TEX_BF16: textures are bfloat16 (float32 compute); the gradient
+ * outputs stay float32.
  * Accumulated outputs (caller zero-fills): v_means2d_abs[C,N,2] (may be
  * NULL), v_means2d[C,N,2], v_ray_transforms[C,N,3,3], v_colors[C,N,K],
  * v_opacities[C,N], v_textures[N,tex_h,tex_w,4] (NULL if textures NULL),
@@ -295,6 +301,26 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *v_opacities, void *v_textures, void *v_normals, void *v_densify,
                       void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
                       void *workspace, size_t workspace_size, tgs_stream_t stream);
+
+/* Selective Adam step of bfloat16 textures from texture-gradient records (no
+ * reference counterpart; the arithmetic is the reference's `selective_adam_update`,
+ * adam.cu:16-47: no bias correction, rows with visibility[n] == 0 untouched).
+ * It replaces building a dense float32 texture gradient and calling
+ * selective_adam_update on it, which at 2M x 64x64 textures does not fit in HBM.
+ * The records come from tgs_rasterize_bwd with v_textures NULL (tex_counts,
+ * tex_rec_incl, tex_rec_cursor, tex_records as there, float32).  State, each
+ * [N, tex_h, tex_w, 4] 16-bit, 16-byte aligned:
+ *   tex_hi  bfloat16 render copy = the float32 master rounded to nearest (ties away)
+ *   tex_lo  int16, master bits - (hi << 16): hi and lo hold the master exactly
+ *   exp_avg, exp_avg_sq  bfloat16 moments, stochastically rounded (hash of the
+ *           element index and seed; pass a new seed each step).
+ * visibility uint8[N], 4-byte aligned.  Needs tex_h * tex_w even and two float
+ * rows in LDS (tex_h * tex_w <= 5120).                                    */
+int tgs_texture_adam_rows(int64_t N, int32_t tex_h, int32_t tex_w, const int32_t *tex_counts,
+                          const int64_t *tex_rec_incl, const int32_t *tex_rec_cursor,
+                          const void *tex_records, const uint8_t *visibility, uint16_t *tex_hi,
+                          uint16_t *tex_lo, uint16_t *exp_avg, uint16_t *exp_avg_sq, double lr,
+                          double b1, double b2, double eps, uint32_t seed, tgs_stream_t stream);
 
 /* ------------------------------------------------ neural-texture decoder
  * No reference counterpart (its closest relative is the coordinate-MLP

@@ -287,6 +287,33 @@ template <typename S> __device__ __forceinline__ Texels<S> load_texels(const S *
     return t;
 }
 
+// Texture storage: none (plain 2DGS), the compute type S, or bfloat16 RGBA texels
+// (8 bytes, float compute only).  Texel offsets are kept in units of S texels
+// (16 bytes) whatever the storage, so the gradient side (records, v_textures in
+// float) is the same for both; a bf16 texel sits at half that byte offset.
+constexpr int kTexNone = 0, kTexNative = 1, kTexBF16 = 2;
+
+__device__ __forceinline__ Texel<float> load_texel_bf16(const uint16_t *tb, uint32_t off_f32) {
+    const uint2 w = *(const uint2 *)((const char *)tb + (off_f32 >> 1));
+    return {{__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u), __uint_as_float(w.y << 16),
+             __uint_as_float(w.y & 0xffff0000u)}};
+}
+
+template <typename S, int TEX>
+__device__ __forceinline__ Texels<S> fetch_texels(const S *textures, int64_t row_off, const Bilerp<S> &b) {
+    if constexpr (TEX == kTexBF16) {
+        const uint16_t *tb = (const uint16_t *)textures + row_off;
+        Texels<S> t;
+        t.t00 = load_texel_bf16(tb, b.o00);
+        t.t10 = load_texel_bf16(tb, b.o00 + b.du);
+        t.t01 = load_texel_bf16(tb, b.o00 + b.dv);
+        t.t11 = load_texel_bf16(tb, b.o00 + b.du + b.dv);
+        return t;
+    } else {
+        return load_texels(textures + row_off, b);
+    }
+}
+
 template <typename S> struct BWeights {
     S w00, w10, w01, w11;
 };
@@ -386,7 +413,7 @@ template <typename S> __device__ __forceinline__ void zeroer_init(RowZeroer &z, 
 }
 
 // ------------------------------------------------------------------ forward
-template <typename S, int K, bool TEX>
+template <typename S, int K, int TEX>
 __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) {
     const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
     const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
@@ -451,7 +478,8 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
             inr = !done && in_tex_range(p, a);
             if (inr) {
                 const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
-                a_s = bilerp_blend<S, K>(bweights(bl), load_texels(a.textures + (int64_t)G.row * a.tex_stride, bl), tex);
+                a_s = bilerp_blend<S, K>(bweights(bl), fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl),
+                                         tex);
             }
         }
         const S alpha = pair_alpha(H.op, p.vis, a_s);
@@ -527,7 +555,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 }
 
 // ----------------------------------------------------------------- backward
-template <typename S, int K, bool TEX>
+template <typename S, int K, int TEX>
 __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
     extern __shared__ __align__(16) char smem[];
     // per-wave staging of texture-gradient updates (2 texel rows x 64 pixels)
@@ -646,14 +674,12 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         BWeights<S> bw;
         Texels<S> tt;
         bool inr = false;
-        const S *tb = nullptr;
         if constexpr (TEX) {
-            tb = a.textures + (int64_t)G.row * a.tex_stride;
             inr = valid && in_tex_range(p, a);
             if (inr) {
                 bl = bilerp_setup<S>(p.sx, p.sy, a);
                 bw = bweights(bl);
-                tt = load_texels(tb, bl);
+                tt = fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl);
                 a_s = bilerp_blend<S, K>(bw, tt, tex);
             }
         }
@@ -744,8 +770,10 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
 #pragma unroll
                             for (int k = 0; k < 4; ++k) rec.q[k] = q[k];
                             ((TexRecord<S> *)a.recs)[slot] = rec;
-                        } else {
-                            // more pairs than the forward counted (not expected): add directly
+                        } else if (a.g_tex) {
+                            // more pairs than the forward counted (not expected): add directly.
+                            // Without v_textures (the records feed the row-wise Adam step) the
+                            // pair is dropped; the cursor exceeding the count shows it.
 #pragma unroll
                             for (int k = 0; k < 4; ++k) {
                                 if (q[k] == S(0)) continue;
@@ -998,6 +1026,152 @@ __global__ __launch_bounds__(512) void tex_gather_kernel(int64_t N, int64_t rows
     }
 }
 
+// ------------------------------------------------------- row-wise texture Adam
+// Selective Adam (adam.cu:16-47 semantics: no bias correction, rows whose
+// visibility flag is clear are left alone) applied to bfloat16 textures straight
+// from the backward's texture-gradient records, so no dense gradient exists.
+// State per texel channel, 8 bytes:
+//   hi  bf16   the render copy, = the float32 master rounded to nearest (ties away)
+//   lo  int16  master bits - (hi << 16): hi and lo together are the float32 master
+//   m,v bf16   Adam moments, stochastically rounded (unbiased; round-to-nearest
+//              would freeze v, whose per-step change (1 - b2) is below half a bf16 ulp)
+// Workgroup layout as the gather: waves 0-3 sum row k's records into an LDS row
+// while waves 4-7 step row k-1 out of the other LDS buffer.
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {  // murmur3 finaliser
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+
+// float32 -> bf16 bits, stochastic: add 16 random bits below the kept ones, truncate
+__device__ __forceinline__ uint32_t bf16_stochastic(float x, uint32_t r16) {
+    return (__float_as_uint(x) + (r16 & 0xffffu)) >> 16;
+}
+
+__device__ __forceinline__ void adam_texel(uint32_t &hi, uint32_t &lo, uint32_t &m, uint32_t &v, float g, float lr,
+                                           float b1, float b2, float eps, uint32_t rnd) {
+    float p = __uint_as_float((hi << 16) + (uint32_t)(int32_t)(int16_t)lo);
+    float mm = __uint_as_float(m << 16), vv = __uint_as_float(v << 16);
+    mm = b1 * mm + (1.0f - b1) * g;
+    vv = b2 * vv + (1.0f - b2) * g * g;
+    p += -lr * mm / (sqrtf(vv) + eps);
+    const uint32_t pb = __float_as_uint(p);
+    hi = (pb + 0x8000u) >> 16;
+    lo = (pb - (hi << 16)) & 0xffffu;
+    m = bf16_stochastic(mm, rnd);
+    v = bf16_stochastic(vv, rnd >> 16);
+}
+
+__global__ __launch_bounds__(512) void tex_adam_kernel(int64_t N, int64_t rows_per_block, int32_t tex_w,
+                                                       int64_t row_elems, const int64_t *__restrict__ rec_incl,
+                                                       const int32_t *__restrict__ rec_cnt,
+                                                       const int32_t *__restrict__ rec_cur,
+                                                       const TexRecord<float> *__restrict__ recs,
+                                                       const uint8_t *__restrict__ visible, uint16_t *__restrict__ hi,
+                                                       uint16_t *__restrict__ lo, uint16_t *__restrict__ m,
+                                                       uint16_t *__restrict__ v, float lr, float b1, float b2,
+                                                       float eps, uint32_t seed) {
+    typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+    extern __shared__ __align__(16) char smem[];
+    float *const acc = (float *)smem;  // two rows, addressed from the LDS base (ds_* accesses)
+    u32x4_t *const acc4 = (u32x4_t *)smem;
+    const int64_t row_vec = row_elems / 4;  // 16-byte units of one float row
+    const int64_t row_oct = row_elems / 8;  // 8-channel groups (16 bytes of each bf16 plane)
+    const int t = threadIdx.x, tt = t & 255;
+    const bool stepper = t >= 256;  // uniform per wave
+    const uint32_t tb = 16;         // float texel stride of the record offsets
+    cptr<int32_t> cnt = as_const(rec_cnt), cur = as_const(rec_cur);
+    cptr<int64_t> incl = as_const(rec_incl);
+
+    // stepper waves: Adam over half h of row `row`, gradient from LDS buffer b
+    auto step_half = [&](int b, int64_t row, int h) {
+        const int64_t o0 = h ? row_oct / 2 : 0, o1 = h ? row_oct : row_oct / 2;
+        u32x4_t *H = (u32x4_t *)(hi + row * row_elems), *L = (u32x4_t *)(lo + row * row_elems);
+        u32x4_t *Mv = (u32x4_t *)(m + row * row_elems), *Vv = (u32x4_t *)(v + row * row_elems);
+        for (int64_t i = o0 + tt; i < o1; i += 256) {
+            u32x4_t hv = H[i], lv = L[i], mv = Mv[i], vv = Vv[i];
+            const u32x4_t g0 = acc4[b * row_vec + 2 * i], g1 = acc4[b * row_vec + 2 * i + 1];
+            const float g[8] = {__uint_as_float(g0.x), __uint_as_float(g0.y), __uint_as_float(g0.z),
+                                __uint_as_float(g0.w), __uint_as_float(g1.x), __uint_as_float(g1.y),
+                                __uint_as_float(g1.z), __uint_as_float(g1.w)};
+            const uint32_t base = mix32((uint32_t)(row * row_oct + i) ^ seed);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                uint32_t h0 = hv[w] & 0xffffu, h1 = hv[w] >> 16, l0 = lv[w] & 0xffffu, l1 = lv[w] >> 16;
+                uint32_t m0 = mv[w] & 0xffffu, m1 = mv[w] >> 16, v0 = vv[w] & 0xffffu, v1 = vv[w] >> 16;
+                adam_texel(h0, l0, m0, v0, g[2 * w], lr, b1, b2, eps, mix32(base + 2 * w));
+                adam_texel(h1, l1, m1, v1, g[2 * w + 1], lr, b1, b2, eps, mix32(base + 2 * w + 1));
+                hv[w] = h0 | (h1 << 16);
+                lv[w] = l0 | (l1 << 16);
+                mv[w] = m0 | (m1 << 16);
+                vv[w] = v0 | (v1 << 16);
+            }
+            H[i] = hv;
+            L[i] = lv;
+            Mv[i] = mv;
+            Vv[i] = vv;
+        }
+    };
+
+    const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r_end = min(N, r_begin + rows_per_block);
+    int64_t prev = -1;
+    int k = 0;
+    for (int64_t row = r_begin; row < r_end; ++row) {
+        // the visibility byte through a scalar load of its (4-byte aligned) word, but the
+        // last partial word, which may end the buffer
+        const uint32_t word = (row | 3) < N ? as_const((const uint32_t *)visible)[row >> 2]
+                                            : (uint32_t)visible[row] << (8 * (row & 3));
+        if (((word >> (8 * (row & 3))) & 0xffu) == 0) continue;  // uniform over the workgroup
+        const int32_t nrec = min(cnt[row], cur[row]);  // records actually written
+        const int bc = k & 1, bp = bc ^ 1;
+        if (stepper) {
+            if (prev >= 0) step_half(bp, prev, 0);
+        } else {
+            for (int64_t i = tt; i < row_vec; i += 256) acc4[bc * row_vec + i] = u32x4_t{0, 0, 0, 0};
+        }
+        __syncthreads();
+        if (stepper) {
+            if (prev >= 0) step_half(bp, prev, 1);
+        } else {
+            const TexRecord<float> *rr = recs + (row ? incl[row - 1] : 0);
+            for (int32_t r = tt; r < nrec; r += 256) {
+                const TexRecord<float> rec = rr[r];
+                Bilerp<float> bl;
+                bl.wu = rec.wu;
+                bl.wv = rec.wv;
+                bl.o00 = rec.o00;
+                bl.du = (rec.flags & 1u) ? tb : 0u;
+                bl.dv = (rec.flags & 2u) ? (uint32_t)tex_w * tb : 0u;
+                const BWeights<float> bw = bweights(bl);
+                const uint32_t e00 = bl.o00 / 4, e10 = (bl.o00 + bl.du) / 4;
+                const uint32_t e01 = (bl.o00 + bl.dv) / 4, e11 = (bl.o00 + bl.du + bl.dv) / 4;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float qc = rec.q[c];
+                    if (qc == 0.0f) continue;
+                    const int64_t o = bc * row_elems + c;
+                    atomicAdd(acc + o + e00, bw.w00 * qc);
+                    atomicAdd(acc + o + e10, bw.w10 * qc);
+                    atomicAdd(acc + o + e01, bw.w01 * qc);
+                    atomicAdd(acc + o + e11, bw.w11 * qc);
+                }
+            }
+        }
+        __syncthreads();
+        prev = row;
+        ++k;
+    }
+    if (stepper && prev >= 0) {
+        step_half((k + 1) & 1, prev, 0);
+        step_half((k + 1) & 1, prev, 1);
+    }
+}
+
 template <typename S> static size_t rec_bytes(int64_t C, int64_t N) {
     return (((size_t)(C * N) * kRec * sizeof(S)) + 255) / 256 * 256;
 }
@@ -1011,32 +1185,39 @@ static void launch_pack(const RasterArgs<S> &a, hipStream_t st) {
     pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.N, a.gids, a.means2d, a.rt, a.opac, a.colors,
                                                                     a.normals, (S *)a.rec);
 }
-template <typename S, int K, bool TEX> static void launch_fwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
+template <typename S, int K, int TEX> static void launch_fwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
     launch_pack<S, K>(a, st);
     raster_fwd_kernel<S, K, TEX><<<grid, bs, 0, st>>>(a);
 }
-template <typename S, int K, bool TEX> static void launch_bwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
+template <typename S, int K, int TEX> static void launch_bwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
     launch_pack<S, K>(a, st);
     const size_t stage = TEX ? (size_t)(bs / kWave) * 2 * kWave * sizeof(TexRec<S>) : 0;
     raster_bwd_kernel<S, K, TEX><<<grid, bs, stage, st>>>(a);
 }
 
 template <typename S, bool BWD>
-static int dispatch(const RasterArgs<S> &a, int K, bool tex, hipStream_t st) {
+static int dispatch(const RasterArgs<S> &a, int K, int tex, hipStream_t st) {
     const unsigned grid = (unsigned)(a.C * a.tw * a.th);
     const int bs = a.ts * a.ts;
     if (grid == 0) {  // no kernel to fuse the row fill into
         return a.zdst ? tgs_zero_fill_rows(a.zdst, (size_t)a.zrow16 * 16, a.N, a.zmask, a.zsel, st) : TGS_OK;
     }
-#define TGS_CASE(KK)                                                   \
-    case KK:                                                           \
-        if (BWD) {                                                     \
-            if (tex) launch_bwd<S, KK, true>(a, grid, bs, st);         \
-            else launch_bwd<S, KK, false>(a, grid, bs, st);            \
-        } else {                                                       \
-            if (tex) launch_fwd<S, KK, true>(a, grid, bs, st);         \
-            else launch_fwd<S, KK, false>(a, grid, bs, st);            \
-        }                                                              \
+// bf16 texture storage is instantiated for float compute only
+#define TGS_CASE(KK)                                                                      \
+    case KK:                                                                              \
+        if (BWD) {                                                                        \
+            if (tex == kTexNative) launch_bwd<S, KK, kTexNative>(a, grid, bs, st);        \
+            else if constexpr (sizeof(S) == 4) {                                          \
+                if (tex == kTexBF16) launch_bwd<S, KK, kTexBF16>(a, grid, bs, st);        \
+                else launch_bwd<S, KK, kTexNone>(a, grid, bs, st);                        \
+            } else launch_bwd<S, KK, kTexNone>(a, grid, bs, st);                          \
+        } else {                                                                          \
+            if (tex == kTexNative) launch_fwd<S, KK, kTexNative>(a, grid, bs, st);        \
+            else if constexpr (sizeof(S) == 4) {                                          \
+                if (tex == kTexBF16) launch_fwd<S, KK, kTexBF16>(a, grid, bs, st);        \
+                else launch_fwd<S, KK, kTexNone>(a, grid, bs, st);                        \
+            } else launch_fwd<S, KK, kTexNone>(a, grid, bs, st);                          \
+        }                                                                                 \
         break;
     switch (K) {
         TGS_CASE(1)
@@ -1157,6 +1338,42 @@ extern "C" size_t tgs_tex_record_bytes(int dtype) {
     return dtype == TGS_F64 ? sizeof(TexRecord<double>) : sizeof(TexRecord<float>);
 }
 
+extern "C" int tgs_texture_adam_rows(int64_t N, int32_t tex_h, int32_t tex_w, const int32_t *tex_counts,
+                                     const int64_t *tex_rec_incl, const int32_t *tex_rec_cursor,
+                                     const void *tex_records, const uint8_t *visibility, uint16_t *tex_hi,
+                                     uint16_t *tex_lo, uint16_t *exp_avg, uint16_t *exp_avg_sq, double lr, double b1,
+                                     double b2, double eps, uint32_t seed, tgs_stream_t stream) {
+    TGS_CHECK_ARG(N >= 0 && tex_h >= 1 && tex_w >= 1, "texture_adam_rows: bad sizes");
+    const int64_t row_elems = (int64_t)tex_h * tex_w * 4;
+    TGS_CHECK_ARG(gather_fits<float>(tex_h, tex_w) && row_elems % 8 == 0,
+                  "texture_adam_rows: a texture must hold an even number of texels and two rows must fit in LDS");
+    TGS_CHECK_ARG(((uintptr_t)visibility % 4) == 0, "texture_adam_rows: visibility must be 4-byte aligned");
+    TGS_CHECK_ARG(N == 0 || (tex_counts && tex_rec_incl && tex_rec_cursor && tex_records && visibility && tex_hi &&
+                             tex_lo && exp_avg && exp_avg_sq),
+                  "texture_adam_rows: null buffer");
+    TGS_CHECK_ARG(((uintptr_t)tex_hi | (uintptr_t)tex_lo | (uintptr_t)exp_avg | (uintptr_t)exp_avg_sq) % 16 == 0,
+                  "texture_adam_rows: state planes must be 16-byte aligned");
+    if (N == 0) return TGS_OK;
+    const size_t lds = 2 * (size_t)row_elems * sizeof(float);
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)tex_adam_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess) {
+        set_error("texture_adam_rows: cannot reserve %zu bytes of LDS", lds);
+        return TGS_ERR_LAUNCH;
+    }
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, kGatherMaxLds / std::max<size_t>(lds, 1)));
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
+    const int64_t rows_per_block = (N + blocks - 1) / blocks;
+    tex_adam_kernel<<<(unsigned)blocks, 512, lds, (hipStream_t)stream>>>(
+        N, rows_per_block, tex_w, row_elems, tex_rec_incl, tex_counts, tex_rec_cursor,
+        (const TexRecord<float> *)tex_records, visibility, tex_hi, tex_lo, exp_avg, exp_avg_sq, (float)lr, (float)b1,
+        (float)b2, (float)eps, seed);
+    TGS_CHECK_LAUNCH("texture_adam_rows");
+    return TGS_OK;
+}
+
 extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
                                  const void *ray_transforms, const void *colors, const void *opacities,
                                  const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
@@ -1171,6 +1388,9 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  uint8_t *live_entries, int32_t *tex_counts, void *zero_rows_dst,
                                  const uint8_t *zero_rows_mask,
                                  size_t zero_row_bytes, void *workspace, size_t workspace_size, tgs_stream_t stream) {
+    const int tex_mode = !textures ? kTexNone : (dtype & TGS_TEX_BF16) ? kTexBF16 : kTexNative;
+    dtype &= ~TGS_TEX_BF16;
+    TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_fwd: bfloat16 textures need float32 compute");
     int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
                           tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
@@ -1195,7 +1415,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.out_accd = (S *)render_accum_depth;                                                                   \
         a.live = live_entries;                                                                                  \
         a.tex_cnt = textures ? tex_counts : nullptr;                                                            \
-        return dispatch<S, false>(a, K, textures != nullptr, st);                                               \
+        return dispatch<S, false>(a, K, tex_mode, st);                                                          \
     }
     if (dtype == TGS_F32) TGS_FWD(float)
     if (dtype == TGS_F64) TGS_FWD(double)
@@ -1223,11 +1443,20 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  void *v_normals, void *v_densify, void *zero_rows_dst,
                                  const uint8_t *zero_rows_mask, size_t zero_row_bytes, void *workspace,
                                  size_t workspace_size, tgs_stream_t stream) {
+    const int tex_mode = !textures ? kTexNone : (dtype & TGS_TEX_BF16) ? kTexBF16 : kTexNative;
+    dtype &= ~TGS_TEX_BF16;
+    TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_bwd: bfloat16 textures need float32 compute");
     int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
                           tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
     TGS_CHECK_ARG(uv_grad_mode >= 0 && uv_grad_mode <= 2, "rasterize_bwd: bad uv_grad_mode %d", uv_grad_mode);
-    TGS_CHECK_ARG(!textures || v_textures, "rasterize_bwd: v_textures required with textures");
+    // without v_textures the texture gradient leaves only as records (for tgs_texture_adam_rows)
+    const bool records_only = textures && !v_textures;
+    TGS_CHECK_ARG(!records_only || (tex_records && tex_counts && tex_rec_incl && tex_rec_cursor && !zero_rows_dst &&
+                                    (dtype == TGS_F64 ? gather_fits<double>(tex_h, tex_w)
+                                                      : gather_fits<float>(tex_h, tex_w))),
+                  "rasterize_bwd: v_textures required with textures (or the record buffers, for textures whose "
+                  "two rows fit in LDS)");
     const int64_t E = gaussian_ids ? nnz : C * N;  // per-entry inputs
     TGS_CHECK_ARG(!gaussian_ids || (nnz >= 0 && nnz < (int64_t(1) << 31)), "rasterize_bwd: bad nnz");
     TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, 1, E),
@@ -1267,12 +1496,12 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         zero_kernel<S><<<(unsigned)ceil_div(n * kAbc, 256), 256, 0, st>>>(n * kAbc, a.g_abc);                   \
         TGS_CHECK_LAUNCH("rasterize_bwd zero");                                                                 \
         if (n_isects > 0) {                                                                                     \
-            rc = dispatch<S, true>(a, K, textures != nullptr, st);                                              \
+            rc = dispatch<S, true>(a, K, tex_mode, st);                                                         \
         } else if (zero_rows_dst) {                                                                             \
             rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, 0, st);                   \
         }                                                                                                       \
         if (rc) return rc;                                                                                      \
-        if (recs) {                                                                                             \
+        if (recs && !records_only) {                                                                            \
             rc = launch_gather<S>(N, tex_h, tex_w, a, zero_rows_dst ? zero_rows_mask : nullptr, st);             \
             if (rc) return rc;                                                                                  \
         }                                                                                                       \

@@ -75,6 +75,18 @@ def _same(dtype: torch.dtype, **ts: Tensor) -> None:
             raise RuntimeError(f"{n} must have dtype {dtype}, got {t.dtype}")
 
 
+TEX_BF16 = 0x100  # OR-ed into the rasterizers' dtype: bfloat16 textures
+
+
+def _tex_dtype_flag(textured, textures, means2d):
+    """TEX_BF16 for bfloat16 textures (float32 compute only), else 0; checks the dtype."""
+    if not textured or textures.dtype == means2d.dtype:
+        return 0
+    if textures.dtype == torch.bfloat16 and means2d.dtype == torch.float32:
+        return TEX_BF16
+    raise RuntimeError(f"textures must have dtype {means2d.dtype} (or bfloat16 with float32), got {textures.dtype}")
+
+
 # ------------------------------------------------------------------ projection
 def fully_fused_projection_fwd_2dgs(means, quats, scales, viewmats, Ks, image_width, image_height,
                                     eps2d, near_plane, far_plane, radius_clip):
@@ -380,7 +392,8 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         _check_input(masks, "masks")
     dt = _dtype_code(means2d, "means2d")
     _same(means2d.dtype, ray_transforms=ray_transforms, colors=colors, opacities=opacities, normals=normals,
-          textures=textures if textured else None, backgrounds=backgrounds)
+          backgrounds=backgrounds)
+    tex_flag = _tex_dtype_flag(textured, textures, means2d)
     C = tile_offsets.size(0)
     gids, nnz, N = _packed_args(means2d, gaussian_ids, textures if textured else None, n_gaussians)
     K = colors.size(-1)
@@ -406,7 +419,7 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         zd, zm, zb = _zero_rows_args(zero_rows, N)
         with _timed("raster_fwd", means2d):
             check(lib().tgs_rasterize_fwd(
-              dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
+              dt | tex_flag, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
               _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
               _p(masks.view(torch.uint8)) if masks is not None else None, _p(gids), nnz, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
@@ -474,6 +487,28 @@ def _tex_record_args(tex_counts, dt, N, dev):
     return tex_counts, incl, cursor, recs, temp
 
 
+def texture_adam_rows(records, visibility, tex_hi, tex_lo, exp_avg, exp_avg_sq, lr, b1, b2, eps, seed):
+    """Selective Adam step of bfloat16 textures [N, h, w, 4] from the backward's
+    texture-gradient records ``(counts, inclusive offsets, cursor, records)``: the
+    float32 master is (tex_hi, tex_lo), tex_hi the render copy; moments bfloat16.
+    ``visibility`` bool/uint8 [N].  All in place."""
+    counts, incl, cursor, recs = records
+    for n, t in dict(counts=counts, incl=incl, cursor=cursor, recs=recs, visibility=visibility, tex_hi=tex_hi,
+                     tex_lo=tex_lo, exp_avg=exp_avg, exp_avg_sq=exp_avg_sq).items():
+        _check_input(t, n)
+    N, th, tw = tex_hi.size(0), tex_hi.size(1), tex_hi.size(2)
+    for n, t in dict(tex_lo=tex_lo, exp_avg=exp_avg, exp_avg_sq=exp_avg_sq).items():
+        if t.shape != tex_hi.shape or t.element_size() != 2:
+            raise RuntimeError(f"{n} must be 16-bit with the shape of tex_hi")
+    if visibility.numel() != N or visibility.element_size() != 1:
+        raise RuntimeError("visibility must be bool/uint8 [N]")
+    with torch.cuda.device(tex_hi.device):
+        check(lib().tgs_texture_adam_rows(N, th, tw, _p(counts), _p(incl), _p(cursor), _p(recs),
+                                          _p(visibility.view(torch.uint8)), _p(tex_hi), _p(tex_lo), _p(exp_avg),
+                                          _p(exp_avg_sq), float(lr), float(b1), float(b2), float(eps),
+                                          int(seed) & 0xFFFFFFFF, _stream(tex_hi)), "texture_adam_rows")
+
+
 def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None, tex_counts=None,
                                                     gaussian_ids=None):
     """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
@@ -526,11 +561,13 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
     if masks is not None:
         _check_input(masks, "masks")
     dt = _dtype_code(means2d, "means2d")
+    tex_flag = _tex_dtype_flag(textured, textures, means2d)
     C = tile_offsets.size(0)
     gids, nnz, N = _packed_args(means2d, gaussian_ids, textures if textured else None, n_gaussians)
     K = colors.size(-1)
     th_t, tw_t = tile_offsets.size(1), tile_offsets.size(2)
     dev = means2d.device
+    records_only = textured and v_textures_out is False
     if textured:
         tex_h, tex_w = textures.size(1), textures.size(2)
     else:
@@ -539,12 +576,18 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_means2d = torch.zeros_like(means2d)
         v_rt = torch.zeros_like(ray_transforms)
         v_colors = torch.zeros_like(colors)
-        if textured and v_textures_out is not None:
+        if records_only:
+            # the texture gradient leaves as records only (for texture_adam_rows)
+            if tex_counts is None:
+                raise RuntimeError("rasterize_to_pixels_bwd: v_textures_out=False needs the forward's tex_counts")
+            v_textures = None
+        elif textured and v_textures_out is not None:
             # caller-provided, already zero-filled (see the textured autograd Function)
-            assert v_textures_out.shape == textures.shape and v_textures_out.dtype == textures.dtype
+            assert v_textures_out.shape == textures.shape and v_textures_out.dtype == means2d.dtype
             v_textures = v_textures_out
         else:
-            v_textures = torch.zeros_like(textures) if textured else None
+            # float gradient also for bfloat16 textures
+            v_textures = torch.zeros(textures.shape, dtype=means2d.dtype, device=dev) if textured else None
         v_normals = torch.zeros_like(normals)
         v_opacities = torch.zeros_like(opacities)
         v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
@@ -555,7 +598,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
                                                         if textured else 0, dev)
         with _timed("raster_bwd", means2d):
             check(lib().tgs_rasterize_bwd(
-              dt, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
+              dt | tex_flag, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
               _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
               _p(masks.view(torch.uint8)) if masks is not None else None, _p(gids), nnz, image_width,
               image_height, tile_size, tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(),
@@ -567,6 +610,9 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
               _p(v_means2d_abs), _p(v_means2d), _p(v_rt), _p(v_colors), _p(v_opacities), _p(v_textures),
               _p(v_normals), _p(v_densify) if densify.shape == means2d.shape else None,
               zd, zm, zb, _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_bwd")
+    if records_only:
+        # (counts, inclusive offsets, cursor, records) in place of v_textures
+        return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, (rc_, ri_, rcur_, rrec_), v_normals, v_densify
     if textured:
         return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_textures, v_normals, v_densify
     return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_normals, v_densify

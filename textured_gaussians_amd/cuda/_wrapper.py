@@ -244,7 +244,8 @@ def rasterize_to_pixels_textured_gaussians(
         image_height: int, tile_size: int, isect_offsets: Tensor, flatten_ids: Tensor,
         backgrounds: Optional[Tensor] = None, masks: Optional[Tensor] = None, packed: bool = False,
         absgrad: bool = False, distloss: bool = False, gs_contrib_threshold: float = 0.0,
-        filtering: str = "bilinear", g_weight: float = 1.0, gaussian_ids: Optional[Tensor] = None):
+        filtering: str = "bilinear", g_weight: float = 1.0, gaussian_ids: Optional[Tensor] = None,
+        texture_adam=None):
     """Rasterize textured 2D Gaussians to pixels.
 
     textures [N, th, tw, 4] (RGBA, bilinear, sampled over splat coordinates
@@ -255,6 +256,9 @@ def rasterize_to_pixels_textured_gaussians(
     projection and ``gaussian_ids`` [nnz] (no reference counterpart) names each
     entry's Gaussian, whose texture it samples (the reference samples the
     texture of the entry index there).
+    ``texture_adam`` (a ``BF16TextureAdam`` whose ``textures`` are passed as
+    ``textures``; no reference counterpart): the backward applies one selective Adam
+    step to the textures instead of returning their gradient.
     """
     C = isect_offsets.size(0)
     N = textures.shape[0]
@@ -289,7 +293,7 @@ def rasterize_to_pixels_textured_gaussians(
         colors.contiguous(), opacities.contiguous(), textures.contiguous(), texture_range_x, texture_range_y,
         normals.contiguous(), densify.contiguous(), backgrounds, masks, image_width, image_height, tile_size,
         isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad, distloss, gs_contrib_threshold, g_weight,
-        gaussian_ids)
+        gaussian_ids, texture_adam)
     render_colors = out[0]
     if padded > 0:
         render_colors = render_colors[..., :-padded]
@@ -307,7 +311,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
     @staticmethod
     def forward(ctx, fwd_fn, bwd_fn, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                 texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
-                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight, gaussian_ids):
+                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight, gaussian_ids, texture_adam):
         # The texture gradient is as large as the textures (40 GB at 1M x 50x50) and is never
         # zero-filled on its own.  The backward zeroes the rows of Gaussians absent from the
         # tile lists along its walk.  The rows of listed Gaussians are zeroed by the forward
@@ -317,9 +321,19 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
         # records in LDS and writes the row once.
         ctx.v_tex = None
         ctx.tex_counts = None
+        ctx.adam = None
         zero_rows = None
-        if ctx.needs_input_grad[6] and textures.is_cuda:
-            v_tex = torch.empty_like(textures)
+        if texture_adam is not None:
+            # the backward steps the textures from per-pair records; the rows in this
+            # step's tile lists are the visible ones
+            if textures.data_ptr() != texture_adam.textures.data_ptr():
+                raise ValueError("texture_adam: pass texture_adam.textures as the textures")
+            listed = flatten_ids if gaussian_ids is None else gaussian_ids[flatten_ids.long()].int()
+            ctx.adam = (texture_adam, _C.mark_rows(listed, textures.shape[0]))
+            ctx.tex_counts = _C.tex_counts_buffer(textures)
+        elif ctx.needs_input_grad[6] and textures.is_cuda:
+            # float gradient (also for bfloat16 textures; autograd casts it to theirs)
+            v_tex = torch.empty(textures.shape, dtype=means2d.dtype, device=textures.device)
             # texture rows present in the tile lists (packed: the entries' Gaussians)
             listed = flatten_ids if gaussian_ids is None else gaussian_ids[flatten_ids.long()].int()
             rows = _C.mark_rows(listed, textures.shape[0])
@@ -355,7 +369,9 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
          gaussian_ids) = ctx.saved_tensors
         exact = ctx.bwd_fn == tgs_fns["bilinear_exact"][1]
         v_tex_buf, zero_rows = None, None
-        if ctx.v_tex is not None:
+        if ctx.adam is not None:
+            v_tex_buf = False  # records only
+        elif ctx.v_tex is not None:
             v_tex_buf = ctx.v_tex[0]
             zero_rows = ctx.v_tex
             ctx.v_tex = None
@@ -370,13 +386,18 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             gaussian_ids=gaussian_ids)
         ctx.live = None
         ctx.tex_counts = None
+        if ctx.adam is not None:
+            opt, rows = ctx.adam
+            opt.step_from_records(v_textures, rows)
+            v_textures = None
+            ctx.adam = None
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
         v_backgrounds = None
         if ctx.needs_input_grad[11]:
             v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
         return (None, None, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, None, None, v_normals,
-                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None, None)
+                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None, None, None)
 
 
 def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Tensor, opacities: Tensor,

@@ -123,7 +123,8 @@ def rasterization_textured_gaussians(
         absgrad: bool = False, distloss: bool = False, norm_rot_grad: bool = False,
         depth_mode: Literal["expected", "median"] = "expected", gs_contrib_threshold: float = 0.0,
         filtering: str = "bilinear", g_weight: float = 1.0, grad_mode: Optional[str] = None,
-        proj_grad: str = "reference") -> Tuple[Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, Dict]:
+        proj_grad: str = "reference", texture_adam=None
+) -> Tuple[Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, Dict]:
     """Rasterize N textured 2D Gaussians to C image planes.
 
     means [N,3], quats [N,4] (wxyz, unnormalised), scales [N,3] (two used),
@@ -135,7 +136,9 @@ def rasterization_textured_gaussians(
     [C,H,W,3] (world frame), render_normals_from_depth or None, render_distort
     [C,H,W,1], render_median [C,H,W,1], gs_contrib_sum [N], gs_contrib_count
     [N], meta).  ``grad_mode="exact"`` selects true derivatives for the
-    projection and the texture lookup.
+    projection and the texture lookup.  ``textures`` may be bfloat16 (float32
+    inputs otherwise).  ``texture_adam``: a ``BF16TextureAdam`` holding these
+    textures; the backward steps it instead of producing a texture gradient.
     """
     N, C = _common_checks(means, quats, scales, opacities, colors, viewmats, Ks, render_mode, distloss, sh_degree)
     assert textures.shape[0] == N and textures.shape[-1] == 4 and textures.dim() == 4, textures.shape
@@ -153,7 +156,7 @@ def rasterization_textured_gaussians(
         meta["normals"], densify, width, height, tile_size, meta["isect_offsets"], meta["flatten_ids"],
         backgrounds=backgrounds, packed=packed, absgrad=absgrad, distloss=distloss,
         gs_contrib_threshold=gs_contrib_threshold, filtering=filtering, g_weight=g_weight,
-        gaussian_ids=meta["gaussian_ids"])
+        gaussian_ids=meta["gaussian_ids"], texture_adam=texture_adam)
     render_colors, render_normals, render_normals_from_depth = _finish(
         render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, viewmats, Ks)
     meta["render_distort"] = render_distort
