@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=60, help="tiles in the CPU oracle sample")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-texture leg")
+    ap.add_argument("--no-config5", action="store_true", help="skip the full-training-iteration leg")
+    ap.add_argument("--config5-only", action="store_true", help="run only the full-training-iteration leg")
     return ap.parse_args()
 
 
@@ -289,6 +291,57 @@ def neural_leg(tg, n, steps, warmup, dev):
     }
 
 
+def config5_leg(tg, steps, warmup, dev, n=2_000_000):
+    """Full training iteration at scale: N=2M Gaussians, 64x64 RGBA textures stored
+    bfloat16 with a float32 master (262 GB of texture state in HBM), 1920x1080, SH3;
+    projection + tile sort + textured raster forward/backward + L1 + 0.2 D-SSIM + fused
+    selective Adam over every parameter (the textures' step runs inside the backward from
+    per-pair gradient records).  Times whole iterations and, with HIP events on their
+    stream, the rasterizer and texture-Adam kernels."""
+    from textured_gaussians_amd import scenes, train
+    from textured_gaussians_amd.cuda import _ops
+    sc = scenes.config5(n=n).to(dev)
+    texopt = train.scene_texture_adam(n, 64, dev)
+    state = train.init_state(sc.means, sc.quats, sc.scales, sc.opacities, sc.colors, texopt)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    args = (sc.viewmats, sc.Ks, sc.width, sc.height, sc.target, sc.backgrounds)
+    for _ in range(warmup):
+        train.train_step(state, *args)
+    torch.cuda.synchronize()
+    timer = {}
+    _ops.set_kernel_timer(timer)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = train.train_step(state, *args)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    _ops.set_kernel_timer(None)
+    kt = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in timer.items()}
+    with torch.no_grad():
+        radii = tg.fully_fused_projection_2dgs(state.params["means"], state.params["quats"],
+                                               torch.exp(state.params["scales"]), sc.viewmats, sc.Ks, sc.width,
+                                               sc.height)[0]
+    vis = int((radii > 0).sum())
+    adam_bytes = vis * 64 * 64 * 4 * 16  # hi, lo, m, v: 2 bytes each, read and written
+    achieved = adam_bytes / (kt["texture_adam"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "texture_adam_traffic_config5.json")
+    if os.path.exists(tf) and n == 2_000_000:
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    return {
+        "workload": f"full training iteration, N={n}, 64x64 RGBA bf16 textures + f32 master, 1920x1080, SH3, "
+                    "L1+0.2 D-SSIM, selective Adam on all parameters",
+        "ms_per_step": round(ms, 3), "loss": round(float(loss), 6),
+        "kernels_ms": {k: round(v, 3) for k, v in kt.items()},
+        "visible": vis, "state_bytes": train.state_bytes(state),
+        "peak_step_GB": round(torch.cuda.max_memory_allocated() / 1e9, 1),
+        "texture_adam_roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                                  "kernel": "tex_adam_kernel", "algorithmic_bytes": adam_bytes},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -299,6 +352,10 @@ def main():
     lib()
     dev = torch.device("cuda:0")
     torch.manual_seed(0)
+    if args.config5_only:
+        print("[bench] full training iteration leg only", file=sys.stderr, flush=True)
+        print(json.dumps({"config5": config5_leg(tg, args.steps, args.warmup, dev)}), flush=True)
+        return
     sc, params = make_inputs(args.n, dev)
     torch.cuda.synchronize()
     print("[bench] inputs resident; timing textured steps", file=sys.stderr, flush=True)
@@ -351,11 +408,15 @@ def main():
             line["cpu_baseline"] = cpu_baseline(sc, meta, args.cpu_tiles)
         except MemoryError as e:  # pragma: no cover
             line["cpu_baseline"] = {"error": str(e)}
+    del sc, params  # the other legs build their own scenes
+    torch.cuda.empty_cache()
     if not args.no_neural:
         print("[bench] neural-texture leg", file=sys.stderr, flush=True)
-        del sc, params
-        torch.cuda.empty_cache()
         line["neural"] = neural_leg(tg, args.n, max(3, args.steps // 2), 1, dev)
+        torch.cuda.empty_cache()
+    if not args.no_config5:
+        print("[bench] full training iteration leg", file=sys.stderr, flush=True)
+        line["config5"] = config5_leg(tg, max(3, args.steps // 2), 2, dev)
     print(json.dumps(line), flush=True)
 
 

@@ -1035,8 +1035,9 @@ __global__ __launch_bounds__(512) void tex_gather_kernel(int64_t N, int64_t rows
 //   lo  int16  master bits - (hi << 16): hi and lo together are the float32 master
 //   m,v bf16   Adam moments, stochastically rounded (unbiased; round-to-nearest
 //              would freeze v, whose per-step change (1 - b2) is below half a bf16 ulp)
-// Workgroup layout as the gather: waves 0-3 sum row k's records into an LDS row
-// while waves 4-7 step row k-1 out of the other LDS buffer.
+// One workgroup per row at a time: zero an LDS float row, add the row's records,
+// then step the row with all waves.  The LDS row is single-buffered (64 KB at
+// 64x64) so two workgroups share a CU and cover each other's barriers.
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {  // murmur3 finaliser
     x ^= x >> 16;
@@ -1058,7 +1059,9 @@ __device__ __forceinline__ void adam_texel(uint32_t &hi, uint32_t &lo, uint32_t 
     float mm = __uint_as_float(m << 16), vv = __uint_as_float(v << 16);
     mm = b1 * mm + (1.0f - b1) * g;
     vv = b2 * vv + (1.0f - b2) * g * g;
-    p += -lr * mm / (sqrtf(vv) + eps);
+    // hardware square root and reciprocal (about 1 ulp each) in place of the
+    // correctly rounded forms: the step is a learning-rate-sized quantity
+    p -= lr * mm * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(vv) + eps);
     const uint32_t pb = __float_as_uint(p);
     hi = (pb + 0x8000u) >> 16;
     lo = (pb - (hi << 16)) & 0xffffu;
@@ -1077,50 +1080,19 @@ __global__ __launch_bounds__(512) void tex_adam_kernel(int64_t N, int64_t rows_p
                                                        float eps, uint32_t seed) {
     typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
     extern __shared__ __align__(16) char smem[];
-    float *const acc = (float *)smem;  // two rows, addressed from the LDS base (ds_* accesses)
+    float *const acc = (float *)smem;  // one float row, addressed from the LDS base (ds_* accesses)
     u32x4_t *const acc4 = (u32x4_t *)smem;
-    const int64_t row_vec = row_elems / 4;  // 16-byte units of one float row
+    const int64_t row_vec = row_elems / 4;  // 16-byte units of the float row
     const int64_t row_oct = row_elems / 8;  // 8-channel groups (16 bytes of each bf16 plane)
-    const int t = threadIdx.x, tt = t & 255;
-    const bool stepper = t >= 256;  // uniform per wave
-    const uint32_t tb = 16;         // float texel stride of the record offsets
+    const int t = threadIdx.x;
+    constexpr int kT = 512;
+    const uint32_t tb = 16;  // float texel stride of the record offsets
     cptr<int32_t> cnt = as_const(rec_cnt), cur = as_const(rec_cur);
     cptr<int64_t> incl = as_const(rec_incl);
-
-    // stepper waves: Adam over half h of row `row`, gradient from LDS buffer b
-    auto step_half = [&](int b, int64_t row, int h) {
-        const int64_t o0 = h ? row_oct / 2 : 0, o1 = h ? row_oct : row_oct / 2;
-        u32x4_t *H = (u32x4_t *)(hi + row * row_elems), *L = (u32x4_t *)(lo + row * row_elems);
-        u32x4_t *Mv = (u32x4_t *)(m + row * row_elems), *Vv = (u32x4_t *)(v + row * row_elems);
-        for (int64_t i = o0 + tt; i < o1; i += 256) {
-            u32x4_t hv = H[i], lv = L[i], mv = Mv[i], vv = Vv[i];
-            const u32x4_t g0 = acc4[b * row_vec + 2 * i], g1 = acc4[b * row_vec + 2 * i + 1];
-            const float g[8] = {__uint_as_float(g0.x), __uint_as_float(g0.y), __uint_as_float(g0.z),
-                                __uint_as_float(g0.w), __uint_as_float(g1.x), __uint_as_float(g1.y),
-                                __uint_as_float(g1.z), __uint_as_float(g1.w)};
-            const uint32_t base = mix32((uint32_t)(row * row_oct + i) ^ seed);
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                uint32_t h0 = hv[w] & 0xffffu, h1 = hv[w] >> 16, l0 = lv[w] & 0xffffu, l1 = lv[w] >> 16;
-                uint32_t m0 = mv[w] & 0xffffu, m1 = mv[w] >> 16, v0 = vv[w] & 0xffffu, v1 = vv[w] >> 16;
-                adam_texel(h0, l0, m0, v0, g[2 * w], lr, b1, b2, eps, mix32(base + 2 * w));
-                adam_texel(h1, l1, m1, v1, g[2 * w + 1], lr, b1, b2, eps, mix32(base + 2 * w + 1));
-                hv[w] = h0 | (h1 << 16);
-                lv[w] = l0 | (l1 << 16);
-                mv[w] = m0 | (m1 << 16);
-                vv[w] = v0 | (v1 << 16);
-            }
-            H[i] = hv;
-            L[i] = lv;
-            Mv[i] = mv;
-            Vv[i] = vv;
-        }
-    };
+    constexpr int kU = 4;  // groups per thread whose loads are issued together
 
     const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
     const int64_t r_end = min(N, r_begin + rows_per_block);
-    int64_t prev = -1;
-    int k = 0;
     for (int64_t row = r_begin; row < r_end; ++row) {
         // the visibility byte through a scalar load of its (4-byte aligned) word, but the
         // last partial word, which may end the buffer
@@ -1128,18 +1100,11 @@ __global__ __launch_bounds__(512) void tex_adam_kernel(int64_t N, int64_t rows_p
                                             : (uint32_t)visible[row] << (8 * (row & 3));
         if (((word >> (8 * (row & 3))) & 0xffu) == 0) continue;  // uniform over the workgroup
         const int32_t nrec = min(cnt[row], cur[row]);  // records actually written
-        const int bc = k & 1, bp = bc ^ 1;
-        if (stepper) {
-            if (prev >= 0) step_half(bp, prev, 0);
-        } else {
-            for (int64_t i = tt; i < row_vec; i += 256) acc4[bc * row_vec + i] = u32x4_t{0, 0, 0, 0};
-        }
-        __syncthreads();
-        if (stepper) {
-            if (prev >= 0) step_half(bp, prev, 1);
-        } else {
+        if (nrec > 0) {
+            for (int64_t i = t; i < row_vec; i += kT) acc4[i] = u32x4_t{0, 0, 0, 0};
+            __syncthreads();
             const TexRecord<float> *rr = recs + (row ? incl[row - 1] : 0);
-            for (int32_t r = tt; r < nrec; r += 256) {
+            for (int32_t r = t; r < nrec; r += kT) {
                 const TexRecord<float> rec = rr[r];
                 Bilerp<float> bl;
                 bl.wu = rec.wu;
@@ -1154,21 +1119,65 @@ __global__ __launch_bounds__(512) void tex_adam_kernel(int64_t N, int64_t rows_p
                 for (int c = 0; c < 4; ++c) {
                     const float qc = rec.q[c];
                     if (qc == 0.0f) continue;
-                    const int64_t o = bc * row_elems + c;
-                    atomicAdd(acc + o + e00, bw.w00 * qc);
-                    atomicAdd(acc + o + e10, bw.w10 * qc);
-                    atomicAdd(acc + o + e01, bw.w01 * qc);
-                    atomicAdd(acc + o + e11, bw.w11 * qc);
+                    atomicAdd(acc + c + e00, bw.w00 * qc);
+                    atomicAdd(acc + c + e10, bw.w10 * qc);
+                    atomicAdd(acc + c + e01, bw.w01 * qc);
+                    atomicAdd(acc + c + e11, bw.w11 * qc);
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-        prev = row;
-        ++k;
-    }
-    if (stepper && prev >= 0) {
-        step_half((k + 1) & 1, prev, 0);
-        step_half((k + 1) & 1, prev, 1);
+        u32x4_t *H = (u32x4_t *)(hi + row * row_elems), *L = (u32x4_t *)(lo + row * row_elems);
+        u32x4_t *Mv = (u32x4_t *)(m + row * row_elems), *Vv = (u32x4_t *)(v + row * row_elems);
+        for (int64_t i0 = t; i0 < row_oct; i0 += kU * kT) {
+            u32x4_t hv[kU], lv[kU], mv[kU], vv[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t i = i0 + u * kT;
+                if (i < row_oct) {
+                    hv[u] = __builtin_nontemporal_load(H + i);
+                    lv[u] = __builtin_nontemporal_load(L + i);
+                    mv[u] = __builtin_nontemporal_load(Mv + i);
+                    vv[u] = __builtin_nontemporal_load(Vv + i);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t i = i0 + u * kT;
+                if (i >= row_oct) continue;
+                float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (nrec > 0) {
+                    const u32x4_t g0 = acc4[2 * i], g1 = acc4[2 * i + 1];
+                    g[0] = __uint_as_float(g0.x); g[1] = __uint_as_float(g0.y);
+                    g[2] = __uint_as_float(g0.z); g[3] = __uint_as_float(g0.w);
+                    g[4] = __uint_as_float(g1.x); g[5] = __uint_as_float(g1.y);
+                    g[6] = __uint_as_float(g1.z); g[7] = __uint_as_float(g1.w);
+                }
+                // one hash per 8 channels, spread to each by an odd multiplier and a shift
+                const uint32_t base = mix32((uint32_t)(row * row_oct + i) ^ seed);
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    uint32_t r0 = base * (4u * w + 1u) + 0x9e3779b9u * w, r1 = base * (4u * w + 3u) + 0x7f4a7c15u * w;
+                    r0 ^= r0 >> 15;
+                    r1 ^= r1 >> 15;
+                    uint32_t h0 = hv[u][w] & 0xffffu, h1 = hv[u][w] >> 16, l0 = lv[u][w] & 0xffffu,
+                             l1 = lv[u][w] >> 16;
+                    uint32_t m0 = mv[u][w] & 0xffffu, m1 = mv[u][w] >> 16, v0 = vv[u][w] & 0xffffu,
+                             v1 = vv[u][w] >> 16;
+                    adam_texel(h0, l0, m0, v0, g[2 * w], lr, b1, b2, eps, r0);
+                    adam_texel(h1, l1, m1, v1, g[2 * w + 1], lr, b1, b2, eps, r1);
+                    hv[u][w] = h0 | (h1 << 16);
+                    lv[u][w] = l0 | (l1 << 16);
+                    mv[u][w] = m0 | (m1 << 16);
+                    vv[u][w] = v0 | (v1 << 16);
+                }
+                __builtin_nontemporal_store(hv[u], H + i);
+                __builtin_nontemporal_store(lv[u], L + i);
+                __builtin_nontemporal_store(mv[u], Mv + i);
+                __builtin_nontemporal_store(vv[u], Vv + i);
+            }
+        }
+        if (nrec > 0) __syncthreads();  // the LDS row is read before the next row zeroes it
     }
 }
 
@@ -1354,7 +1363,7 @@ extern "C" int tgs_texture_adam_rows(int64_t N, int32_t tex_h, int32_t tex_w, co
     TGS_CHECK_ARG(((uintptr_t)tex_hi | (uintptr_t)tex_lo | (uintptr_t)exp_avg | (uintptr_t)exp_avg_sq) % 16 == 0,
                   "texture_adam_rows: state planes must be 16-byte aligned");
     if (N == 0) return TGS_OK;
-    const size_t lds = 2 * (size_t)row_elems * sizeof(float);
+    const size_t lds = (size_t)row_elems * sizeof(float);  // one float row
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute((const void *)tex_adam_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
             hipSuccess) {

@@ -502,7 +502,7 @@ def texture_adam_rows(records, visibility, tex_hi, tex_lo, exp_avg, exp_avg_sq, 
             raise RuntimeError(f"{n} must be 16-bit with the shape of tex_hi")
     if visibility.numel() != N or visibility.element_size() != 1:
         raise RuntimeError("visibility must be bool/uint8 [N]")
-    with torch.cuda.device(tex_hi.device):
+    with torch.cuda.device(tex_hi.device), _timed("texture_adam", tex_hi):
         check(lib().tgs_texture_adam_rows(N, th, tw, _p(counts), _p(incl), _p(cursor), _p(recs),
                                           _p(visibility.view(torch.uint8)), _p(tex_hi), _p(tex_lo), _p(exp_avg),
                                           _p(exp_avg_sq), float(lr), float(b1), float(b2), float(eps),

@@ -129,6 +129,17 @@ def config3(**kw) -> Scene:
     return make_scene(**args)
 
 
+def config5(**kw) -> Scene:
+    """Full training iteration at scale: N=2,000,000, 1920x1080 (fov 60, camera at radius 5),
+    SH degree 3; the 64x64 bfloat16 textures are drawn straight into the optimizer state
+    (``train.scene_texture_adam``), so none here."""
+    args = dict(n=2_000_000, width=1920, height=1080, fov_deg=60.0, cam_positions=[(5.0, 0.0, 0.0)], tex_res=None,
+                mean_dist="normal", mean_scale=2.0, logscale_mu=-4.5, logscale_sigma=0.7, sh_degree=3,
+                sh_sigma=0.2)
+    args.update(kw)
+    return make_scene(**args)
+
+
 def smooth_textures(n: int, res: int = 16, seed: int = 7, max_cycles: float = 0.75) -> torch.Tensor:
     """Procedural ground-truth textures [n, res, res, 4] in (0, 1): per texture a base
     colour plus one low-frequency cosine wave per channel group (RGB shares a wave,

@@ -69,7 +69,7 @@ def train_step(state: TrainState, viewmats: Tensor, Ks: Tensor, width: int, heig
 
 
 def scene_texture_adam(n: int, res: int, device, lr: float = TEXTURE_LR, seed: int = 1000,
-                       chunk_rows: int = 131072) -> BF16TextureAdam:
+                       chunk_rows: int = 16384) -> BF16TextureAdam:
     """Textures of the synthetic scenes (RGB N(0, 0.1), alpha U(0, 1)) drawn on the
     device chunk by chunk straight into the bfloat16 state (no float32 copy of the
     whole set)."""
