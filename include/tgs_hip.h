@@ -36,6 +36,11 @@ enum { TGS_F32 = 0, TGS_F64 = 1 };
 /* OR-ed into the dtype of the rasterizers: textures stored as bfloat16 RGBA (float32
  * compute only; the texture gradient stays float32). */
 enum { TGS_TEX_BF16 = 0x100 };
+/* OR-ed into the dtype of the rasterizers: the `mipmapped2` filter.  textures is then
+ * the flat pyramid [N, (4^(L+1) - 1) / 3, 4] of tgs_generate_mipmap_fwd over
+ * tex_h = tex_w = 2^L (L <= 12) textures; the backward takes uv_grad_mode
+ * TGS_UV_NONE (the reference's rule) and v_textures of the pyramid's shape. */
+enum { TGS_TEX_MIP2 = 0x200 };
 enum { TGS_OK = 0, TGS_ERR_ARG = 1, TGS_ERR_LAUNCH = 2, TGS_ERR_UNSUPPORTED = 3 };
 
 /* Gradient rule for the bilinear texture lookup w.r.t. the splat coordinate.
@@ -301,6 +306,18 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *v_opacities, void *v_textures, void *v_normals, void *v_densify,
                       void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
                       void *workspace, size_t workspace_size, tgs_stream_t stream);
+
+/* Mipmap pyramid for the `mipmapped2` filter (generate_mipmap_fwd.cu,
+ * generate_mipmap_bwd.cu; ext.cpp:188-193): textures [N, R, R, C] (R = 2^L,
+ * L <= 15, C <= 5) -> mip_textures [N, tgs_mipmap_texels(L), C], levels coarsest
+ * first (level l at texel (4^(L-l) - 1) / 3), each the 2x2 box average of the
+ * finer one.  The backward is its adjoint: v_mip_textures -> v_textures
+ * [N, R, R, C] (written, not accumulated). */
+int64_t tgs_mipmap_texels(int32_t log_texture_res);
+int tgs_generate_mipmap_fwd(int dtype, int64_t N, int32_t C, int32_t log_texture_res,
+                            const void *textures, void *mip_textures, tgs_stream_t stream);
+int tgs_generate_mipmap_bwd(int dtype, int64_t N, int32_t C, int32_t log_texture_res,
+                            const void *v_mip_textures, void *v_textures, tgs_stream_t stream);
 
 /* Selective Adam step of bfloat16 textures from texture-gradient records (no
  * reference counterpart; the arithmetic is the reference's `selective_adam_update`,

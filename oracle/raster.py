@@ -72,6 +72,77 @@ def _bilinear(textures: Tensor, n: Tensor, s: Tensor, rx: float, ry: float, uv_g
     return samp, in_range
 
 
+def mip_base(log_res: int, level: int) -> int:
+    """First texel of mip level ``level`` (0 = finest, resolution 2^(log_res - level))
+    in the flat pyramid, stored coarsest first: (4^(log_res - level) - 1) / 3
+    (``filters/mip2.cuh`` t_base_index)."""
+    return ((1 << (2 * (log_res - level))) - 1) // 3
+
+
+def generate_mipmap(textures: Tensor) -> Tensor:
+    """[N, R, R, C] (R = 2^L) -> flat pyramid [N, (4^(L+1) - 1) / 3, C]: level l+1 is
+    the 2x2 box average of level l, summed as ((t00 + t01) + t10) + t11 then / 4
+    (``generate_mipmap_fwd.cu:40-58``).  Differentiable."""
+    N, R, R2, C = textures.shape
+    assert R == R2 and R & (R - 1) == 0, textures.shape
+    L = R.bit_length() - 1
+    levels = [textures]
+    cur = textures
+    for _ in range(L):
+        a, b = cur[:, 0::2, 0::2], cur[:, 0::2, 1::2]
+        c, d = cur[:, 1::2, 0::2], cur[:, 1::2, 1::2]
+        cur = (a + b + c + d) / 4
+        levels.append(cur)
+    return torch.cat([lv.reshape(N, -1, C) for lv in reversed(levels)], 1)
+
+
+def _mip2(mip: Tensor, n: Tensor, s: Tensor, dsdx: Tensor, dsdy: Tensor, log_res: int, rx: float, ry: float):
+    """Trilinear sample of the pyramid ``mip[n]`` at splat coords ``s`` [P,L,2] with
+    the level of detail from the splat-coordinate change over one pixel
+    (``filters/mip2.cuh`` precompute): duv/dx uses the x range for both components and
+    duv/dy the y range (as there); t = log2(max |duv/dx|, |duv/dy|) clamped to [0, 31];
+    u = (s.x + rx) / (2 rx) (res - 1) unclamped; per level l, u_l = (u + 0.5) / 2^l - 0.5
+    and texels floor / ceil; invalid when any of them leaves its level.  Sample
+    positions and weights carry no gradient (the reference's rule).
+    Returns (sample [P,L,4], valid [P,L])."""
+    res = 1 << log_res
+    s, dsdx, dsdy = s.detach(), dsdx.detach(), dsdy.detach()
+    kx, ky = (res - 1) / (2 * rx), (res - 1) / (2 * ry)
+    duv = torch.maximum(torch.sqrt((dsdx[..., 0] * kx) ** 2 + (dsdx[..., 1] * kx) ** 2),
+                        torch.sqrt((dsdy[..., 0] * ky) ** 2 + (dsdy[..., 1] * ky) ** 2))
+    t = torch.log2(duv)
+    t = torch.where(t < 0, torch.zeros_like(t), torch.where(t > 31, torch.full_like(t, 31.0), t))
+    u = (s[..., 0] + rx) / (2 * rx) * (res - 1)
+    v = (s[..., 1] + ry) / (2 * ry) * (res - 1)
+    tl = torch.floor(t)
+    th = torch.ceil(t)
+    valid = torch.ones_like(u, dtype=torch.bool)
+    corners = []
+    for lvl in (tl, th):
+        sc = torch.pow(2.0, lvl)
+        ul = (u + 0.5) / sc - 0.5
+        vl = (v + 0.5) / sc - 0.5
+        u0, u1 = torch.floor(ul), torch.ceil(ul)
+        v0, v1 = torch.floor(vl), torch.ceil(vl)
+        hi = torch.floor((res - 1) / sc)  # (res - 1) >> level
+        valid &= (u0 >= 0) & (v0 >= 0) & (u1 <= hi) & (v1 <= hi)
+        corners.append((lvl, ul - u0, vl - v0, u0, u1, v0, v1))
+    wt = t - tl
+    nn = n[None, :].expand_as(u)
+    samp = torch.zeros(u.shape + (4,), dtype=mip.dtype)
+    for j, (lvl, wu, wv, u0, u1, v0, v1) in enumerate(corners):
+        lv = torch.where(valid, lvl, torch.zeros_like(lvl)).long().clamp(max=log_res)
+        lres = torch.div(torch.full_like(lv, res), torch.pow(2, lv), rounding_mode="floor")
+        base = torch.div(torch.pow(4, log_res - lv) - 1, 3, rounding_mode="floor")
+        z = torch.zeros_like(u0)
+        idx = lambda vv, uu: base + torch.where(valid, vv, z).long() * lres + torch.where(valid, uu, z).long()  # noqa
+        wl = (1 - wt) if j == 0 else wt
+        for (vv, uu, w) in ((v0, u0, (1 - wu) * (1 - wv)), (v0, u1, wu * (1 - wv)), (v1, u0, (1 - wu) * wv),
+                            (v1, u1, wu * wv)):
+            samp = samp + (w * wl)[..., None] * mip[nn, idx(vv, uu)]
+    return samp, valid
+
+
 def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
                         opacities: Tensor, normals: Tensor, image_width: int,
                         image_height: int, tile_size: int, isect_offsets: Tensor,
@@ -80,7 +151,7 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
                         masks: Optional[Tensor] = None, g_weight: float = 1.0,
                         gs_contrib_threshold: float = 0.0, uv_grad: str = "exact",
                         distort_grad: str = "exact", pair_means: Optional[list] = None,
-                        tiles: Optional[list] = None) -> Dict[str, Tensor]:
+                        tiles: Optional[list] = None, mip_log_res: Optional[int] = None) -> Dict[str, Tensor]:
     """Render [C,H,W,*] images.  Shapes as the reference op
     ``rasterize_to_pixels_fwd_textured_gaussians`` (``bindings.h:509-538``):
     means2d [C,N,2], ray_transforms [C,N,3,3], colors [C,N,K], opacities
@@ -105,6 +176,12 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     coordinate over its kept pairs (large for splats seen edge-on);
     ``texel_edge`` [C,N] marks Gaussians with a kept pair whose texture
     coordinate is within 1e-4 texel of a texel edge.
+
+    ``mip_log_res``: ``textures`` is the flat pyramid [N, T, 4] of ``generate_mipmap``
+    over 2^mip_log_res textures and sampling is the reference's ``mipmapped2`` filter
+    (``rasterize_to_pixels_mip2_textured_gaussians_2dgs_fwd.cu:294-395``): the pair
+    also needs non-zero z for the crosses at px +- 1/2 and py +- 1/2, from which
+    ds/dx and ds/dy come; ``uv_grad`` must be "none".
 
     ``tiles`` (optional list of (camera, tile_y, tile_x) or (camera, tile_y,
     tile_x, start, end)) restricts the work to those tiles, optionally with
@@ -177,6 +254,15 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
         okz = cz != 0
         czs = torch.where(okz, cz, torch.ones_like(cz))
         s = cr[..., :2] / czs[..., None]
+        if mip_log_res is not None:
+            # splat coordinate at the pixel's edges: crosses with h_u -/+ w/2, h_v -/+ w/2
+            half = 0.5 * T2[None]
+            sh = []
+            for a_, b_ in ((hu - half, hv), (hu + half, hv), (hu, hv - half), (hu, hv + half)):
+                q = torch.cross(a_, b_, dim=-1).detach()
+                okz = okz & (q[..., 2] != 0)
+                sh.append(q[..., :2] / torch.where(q[..., 2] != 0, q[..., 2], torch.ones_like(q[..., 2]))[..., None])
+            dsdx, dsdy = sh[1] - sh[0], sh[3] - sh[2]
         w3 = s[..., 0] * s[..., 0] + s[..., 1] * s[..., 1]
         mp = mflat[g][None].expand(P, L, 2)
         if pair_means is not None:
@@ -192,7 +278,11 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
         op = oflat[g][None]
         if textured:
             vis = (0.998 - g_weight) + g_weight * ker
-            samp, inr = _bilinear(textures, n, s, rx, ry, uv_grad)
+            if mip_log_res is not None:
+                assert uv_grad == "none", "the mip2 filter has the reference's rule only"
+                samp, inr = _mip2(textures, n, s, dsdx, dsdy, mip_log_res, rx, ry)
+            else:
+                samp, inr = _bilinear(textures, n, s, rx, ry, uv_grad)
             a_s = torch.where(inr, samp[..., 3], torch.ones_like(samp[..., 3]))
             tex = torch.where(inr[..., None], samp[..., :K], torch.zeros_like(samp[..., :K]))
             araw = op * vis * a_s
@@ -228,14 +318,14 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
             fr = okz & ((alpha - 1.0 / 255.0).abs() < 1e-6)
             fr |= (Tincl / 1e-4 - 1).abs() < 1e-4
             fr |= (Texcl - 0.5).abs() < 1e-5
-            if textured:
+            if textured and mip_log_res is None:
                 fr |= ((s[..., 0].abs() - rx).abs() < 1e-4) | ((s[..., 1].abs() - ry).abs() < 1e-4)
             fragile[c, yy, xx] = fr.any(1)
             # conditioning of s = cross.xy / cross.z for kept pairs: |hu| |hv| / |cross.z|
             cnd = (hu.norm(dim=-1) * hv.norm(dim=-1) / cz.abs().clamp_min(1e-300)).to(torch.float64)
             cnd = torch.where(inc, cnd, torch.zeros_like(cnd)).amax(0)
             cond_max.scatter_reduce_(0, g, cnd, reduce="amax")
-            if textured:
+            if textured and mip_log_res is None:
                 # kept in-range pairs whose texture coordinate lies within rounding
                 # distance of a texel edge (the bilinear derivative jumps there)
                 uu = (s[..., 0] + rx) / (2 * rx) * (textures.shape[2] - 1)

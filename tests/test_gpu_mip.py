@@ -1,0 +1,109 @@
+"""The ``mipmapped2`` filter on the GPU against the oracle: the pyramid (bit-exact in
+the same arithmetic), its adjoint, and the trilinear rasterizer forward and backward
+(the reference's gradient rule) in float64; the filter through the public API."""
+import pytest
+import torch
+
+import textured_gaussians_amd as tg
+from oracle.raster import generate_mipmap as oracle_mipmap
+from oracle.raster import rasterize_to_pixels
+from textured_gaussians_amd import scenes
+from textured_gaussians_amd.cuda import _ops
+
+from gpu_util import DEV, cpu64, gpu_stages, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("C,L", [(4, 4), (3, 5), (5, 0), (1, 6)])
+def test_generate_mipmap_matches_oracle(dtype, C, L):
+    g = torch.Generator().manual_seed(C * 10 + L)
+    t = torch.rand(37, 1 << L, 1 << L, C, generator=g, dtype=dtype)
+    got = _ops.generate_mipmap_fwd(t.to(DEV), L)
+    want = oracle_mipmap(t)
+    assert got.shape == want.shape == (37, _ops.mipmap_texels(L), C)
+    assert torch.equal(got.cpu(), want)  # ((a + b) + c) + d then / 4 in both
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_generate_mipmap_backward_is_the_adjoint(dtype):
+    L, C = 5, 4
+    g = torch.Generator().manual_seed(1)
+    t = torch.rand(9, 1 << L, 1 << L, C, generator=g, dtype=torch.float64, requires_grad=True)
+    v = torch.randn(9, _ops.mipmap_texels(L), C, generator=g, dtype=torch.float64)
+    (oracle_mipmap(t) * v).sum().backward()
+    got = _ops.generate_mipmap_bwd(9, C, L, 4, 16, v.to(DEV, dtype))
+    assert rel_err(got, t.grad) < (1e-14 if dtype == torch.float64 else 1e-6)
+
+
+def test_mip2_raster_forward_and_backward_match_oracle():
+    dtype = torch.float64
+    sc = scenes.config1(n=2500, width=80, height=64, tex_res=16, seed=2, sh_degree=None)
+    st = gpu_stages(sc, dtype)
+    s = st["scene"]
+    C, N = st["radii"].shape
+    H, W = s.height, s.width
+    g = torch.Generator().manual_seed(9)
+    colors = torch.rand(C, N, 3, generator=g, dtype=torch.float64)
+    opac = sc.opacities.double()[None].repeat(C, 1)
+    bg = torch.rand(C, 3, generator=g, dtype=torch.float64)
+    mip = oracle_mipmap(sc.textures.double())
+    L = 4
+    dv = lambda t: t.to(DEV, dtype).contiguous()  # noqa: E731
+    out = _ops.rasterize_to_pixels_fwd_mip2_textured_gaussians(
+        st["means2d"], st["rt"], dv(colors), dv(opac), dv(mip), L, 3.0, 3.0, st["normals"], dv(bg), None, W, H, 16,
+        st["offsets"], st["flatten_ids"], 0.0, 1.0)
+    vc = torch.randn(C, H, W, 3, generator=g, dtype=torch.float64)
+    va = torch.randn(C, H, W, 1, generator=g, dtype=torch.float64)
+    vn = torch.randn(C, H, W, 3, generator=g, dtype=torch.float64)
+    vd = torch.randn(C, H, W, 1, generator=g, dtype=torch.float64)
+    vm = torch.randn(C, H, W, 1, generator=g, dtype=torch.float64)
+    res = _ops.rasterize_to_pixels_bwd_mip2_textured_gaussians(
+        st["means2d"], st["rt"], dv(colors), dv(opac), dv(mip), L, 3.0, 3.0, st["normals"],
+        torch.zeros_like(st["means2d"]), dv(bg), None, W, H, 16, st["offsets"], st["flatten_ids"], 1.0, out[0], out[1],
+        out[5], out[6], dv(vc), dv(va), dv(vn), dv(vd), dv(vm), False)
+    leaves = dict(means2d=cpu64(st["means2d"]).requires_grad_(True), rt=cpu64(st["rt"]).requires_grad_(True),
+                  colors=colors.clone().requires_grad_(True), opac=opac.clone().requires_grad_(True),
+                  normals=cpu64(st["normals"]).requires_grad_(True), tex=mip.clone().requires_grad_(True))
+    r = rasterize_to_pixels(leaves["means2d"], leaves["rt"], leaves["colors"], leaves["opac"], leaves["normals"], W,
+                            H, 16, st["offsets"].cpu(), st["flatten_ids"].cpu(), textures=leaves["tex"],
+                            backgrounds=bg, uv_grad="none", distort_grad="reference", mip_log_res=L)
+    names = ["render_colors", "render_alphas", "render_normals", "render_distort", "render_median"]
+    for nm, o in zip(names, out[:5]):
+        assert rel_err(o, r[nm]) < 1e-9, nm
+    assert torch.equal(out[5].cpu(), r["last_ids"]) and torch.equal(out[6].cpu(), r["median_ids"])
+    # the textures contribute (some pairs sample a coarser level than 0)
+    plain = rasterize_to_pixels(cpu64(st["means2d"]), cpu64(st["rt"]), colors, opac, cpu64(st["normals"]), W, H, 16,
+                                st["offsets"].cpu(), st["flatten_ids"].cpu(), textures=None, backgrounds=bg)
+    assert rel_err(r["render_colors"], plain["render_colors"]) > 1e-2
+    loss = sum((r[k] * v).sum() for k, v in (("render_colors", vc), ("render_alphas", va), ("render_normals", vn),
+                                             ("render_distort", vd), ("render_median", vm)))
+    loss.backward()
+    got = dict(means2d=res[1], rt=res[2], colors=res[3], opac=res[4], tex=res[5], normals=res[6])
+    for k, v in got.items():
+        assert rel_err(v, leaves[k].grad) < 1e-8, k
+    assert float(res[5].abs().sum()) > 0
+
+
+def test_mipmapped2_through_the_api():
+    """filtering="mipmapped2": the API builds the pyramid and differentiates through it;
+    same result as the ops composed by hand."""
+    sc = scenes.config1(n=1500, width=64, height=48, tex_res=16, seed=4, sh_degree=None).to(DEV, torch.float64)
+    tex = sc.textures.clone().requires_grad_(True)
+    out = tg.rasterization_textured_gaussians(sc.means, sc.quats, sc.scales, sc.opacities, sc.colors, tex, sc.viewmats,
+                                              sc.Ks, sc.width, sc.height, backgrounds=sc.backgrounds,
+                                              filtering="mipmapped2")
+    (out[0] * sc.target).sum().backward()
+    base = tg.rasterization_textured_gaussians(sc.means, sc.quats, sc.scales, sc.opacities, sc.colors, sc.textures,
+                                               sc.viewmats, sc.Ks, sc.width, sc.height, backgrounds=sc.backgrounds)
+    assert rel_err(out[0], base[0]) > 1e-4  # a different filter
+    assert tex.grad is not None and float(tex.grad.abs().sum()) > 0
+    # the texture gradient is the pyramid adjoint of the rasterizer's pyramid gradient
+    m = out[-1]
+    mip = _ops.generate_mipmap_fwd(sc.textures, 4)
+    fwd = _ops.rasterize_to_pixels_fwd_mip2_textured_gaussians(
+        m["means2d"], m["ray_transforms"], sc.colors[None].expand(1, -1, -1).contiguous(),
+        m["opacities"], mip, 4, 3.0, 3.0, m["normals"], sc.backgrounds, None, sc.width, sc.height, 16,
+        m["isect_offsets"], m["flatten_ids"], 0.0, 1.0)
+    assert rel_err(fwd[0], out[0]) < 1e-12

@@ -100,6 +100,7 @@ template <typename S> struct RasterArgs {
     int32_t tex_h, tex_w;
     int64_t tex_stride;                   // elements per texture (tex_h * tex_w * 4)
     S trx, try_, su, sv, uh, vh, umax, vmax;  // u = s.x su + uh, v = s.y sv + vh, clamped to [0, umax/vmax]
+    int32_t mip_log;                          // mip2 filter: log2 of the finest resolution
     S g_weight, vis_bias, thr;
     int32_t W, H, ts, tw, th;
     const int32_t *offsets, *flatten_ids;
@@ -291,7 +292,7 @@ template <typename S> __device__ __forceinline__ Texels<S> load_texels(const S *
 // (8 bytes, float compute only).  Texel offsets are kept in units of S texels
 // (16 bytes) whatever the storage, so the gradient side (records, v_textures in
 // float) is the same for both; a bf16 texel sits at half that byte offset.
-constexpr int kTexNone = 0, kTexNative = 1, kTexBF16 = 2;
+constexpr int kTexNone = 0, kTexNative = 1, kTexBF16 = 2, kTexMip2 = 3;
 
 __device__ __forceinline__ Texel<float> load_texel_bf16(const uint16_t *tb, uint32_t off_f32) {
     const uint2 w = *(const uint2 *)((const char *)tb + (off_f32 >> 1));
@@ -331,6 +332,85 @@ template <typename S> struct __align__(16) TexRecord {
     S wu, wv;
     S q[4];
 };
+
+// Trilinear (`mipmapped2`) lookup of one pair (filters/mip2.cuh precompute): the
+// level of detail comes from the splat-coordinate change across the pixel.  With
+// cr affine in the pixel position, the reference's crosses at px -/+ 1/2 and
+// py -/+ 1/2 are cr -/+ A0/2 and cr -/+ A1/2; a zero z in any of them drops the
+// pair (zok).  valid: every texel of both levels lies inside its level.  Texel
+// offsets are element offsets (texel * 4) in the Gaussian's pyramid.
+template <typename S> struct Mip2 {
+    S w[8];
+    uint32_t off[8];
+    bool zok, valid;
+};
+
+template <typename S>
+__device__ __forceinline__ Mip2<S> mip2_setup(const PairGeom<S> &p, const GaussGeom<S> &G, const RasterArgs<S> &a) {
+    Mip2<S> m;
+    const S crx = tfma(-p.dx, G.A[0], tfma(-p.dy, G.A[3], G.A[6]));
+    const S cry = tfma(-p.dx, G.A[1], tfma(-p.dy, G.A[4], G.A[7]));
+    const S h = S(0.5);
+    const S x0z = p.cz - h * G.A[2], x1z = p.cz + h * G.A[2];
+    const S y0z = p.cz - h * G.A[5], y1z = p.cz + h * G.A[5];
+    m.zok = x0z != S(0) && x1z != S(0) && y0z != S(0) && y1z != S(0);
+    m.valid = false;
+    if (!m.zok || !p.okz) return m;
+    const S dsdx_x = (crx + h * G.A[0]) / x1z - (crx - h * G.A[0]) / x0z;
+    const S dsdx_y = (cry + h * G.A[1]) / x1z - (cry - h * G.A[1]) / x0z;
+    const S dsdy_x = (crx + h * G.A[3]) / y1z - (crx - h * G.A[3]) / y0z;
+    const S dsdy_y = (cry + h * G.A[4]) / y1z - (cry - h * G.A[4]) / y0z;
+    const int32_t res = 1 << a.mip_log;
+    // duv/dx scales both components by the x range, duv/dy by the y range (as the reference)
+    const S ex = dsdx_x * a.su, ey = dsdx_y * a.su, fx = dsdy_x * a.sv, fy = dsdy_y * a.sv;
+    const S duv = max(sqrt(ex * ex + ey * ey), sqrt(fx * fx + fy * fy));
+    S t = log2(duv);
+    if (t != t) return m;  // NaN footprint: no texture (every comparison below would pass)
+    t = t < S(0) ? S(0) : (t > S(31) ? S(31) : t);
+    const S u = tfma(p.sx, a.su, a.uh), v = tfma(p.sy, a.sv, a.vh);  // unclamped
+    const int32_t tl = (int32_t)floor(t), th = (int32_t)ceil(t);
+    const S wt = t - (S)tl;
+    int32_t lvl[2] = {tl, th};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (lvl[j] > a.mip_log) return m;  // beyond the coarsest level: always invalid
+        const S inv = S(1) / (S)(1 << lvl[j]);
+        const S ul = (u + S(0.5)) * inv - S(0.5), vl = (v + S(0.5)) * inv - S(0.5);
+        const S fu = floor(ul), fv = floor(vl), cu = ceil(ul), cv = ceil(vl);
+        const int32_t hi = (res - 1) >> lvl[j];
+        if (!(fu >= S(0) && fv >= S(0) && cu <= (S)hi && cv <= (S)hi)) return m;  // NaN fails too
+        const int32_t lres = res >> lvl[j];
+        const uint32_t base = (uint32_t)(((1u << (2 * (a.mip_log - lvl[j]))) - 1) / 3);
+        const S wu = ul - fu, wv = vl - fv, wl = j == 0 ? S(1) - wt : wt;
+        const uint32_t r0 = base + (uint32_t)fv * lres, r1 = base + (uint32_t)cv * lres;
+        m.off[4 * j + 0] = 4 * (r0 + (uint32_t)fu);
+        m.off[4 * j + 1] = 4 * (r0 + (uint32_t)cu);
+        m.off[4 * j + 2] = 4 * (r1 + (uint32_t)fu);
+        m.off[4 * j + 3] = 4 * (r1 + (uint32_t)cu);
+        m.w[4 * j + 0] = (S(1) - wu) * (S(1) - wv) * wl;
+        m.w[4 * j + 1] = wu * (S(1) - wv) * wl;
+        m.w[4 * j + 2] = (S(1) - wu) * wv * wl;
+        m.w[4 * j + 3] = wu * wv * wl;
+    }
+    m.valid = true;
+    return m;
+}
+
+// a_s and the K colour channels of the trilinear sample, summed over the 8 texels in order
+template <typename S, int K>
+__device__ __forceinline__ S mip2_blend(const Mip2<S> &m, const S *tb, S tex[K]) {
+    S as = S(0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) tex[k] = S(0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const Texel<S> t = load_texel(tb, m.off[i] * (uint32_t)sizeof(S));
+#pragma unroll
+        for (int k = 0; k < K; ++k) tex[k] += m.w[i] * t.c[k];
+        as += m.w[i] * t.c[3];
+    }
+    return as;
+}
 
 // Bilinear blend of the four texels: a_s (alpha channel) and the K colour channels.
 template <typename S, int K>
@@ -469,12 +549,17 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
         const int32_t g = ids[idx];
         const GaussGeom<S> G = load_geom<S>(a, g);
         const GaussShade<S, K> H = load_shade<S, K>(a, g);
-        const PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
+        PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
         S a_s = 1, tex[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) tex[k] = 0;
         bool inr = false;
-        if constexpr (TEX) {
+        if constexpr (TEX == kTexMip2) {
+            const Mip2<S> mm = mip2_setup(p, G, a);
+            p.okz = p.okz && mm.zok;
+            inr = !done && p.okz && mm.valid;
+            if (inr) a_s = mip2_blend<S, K>(mm, a.textures + (int64_t)G.row * a.tex_stride, tex);
+        } else if constexpr (TEX) {
             inr = !done && in_tex_range(p, a);
             if (inr) {
                 const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
@@ -625,7 +710,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         ds_u = S(2) * a.su;
         ds_v = S(2) * a.sv;
     }
-    const bool uv_term = TEX && a.uv_mode != TGS_UV_NONE;
+    const bool uv_term = TEX && TEX != kTexMip2 && a.uv_mode != TGS_UV_NONE;  // mip2: the reference's rule only
     const int32_t first = min(end - 1, wave_last);
     const uint8_t *live = a.live ? a.live + (int64_t)(threadIdx.x / kWave) * a.n_isects : nullptr;
     // per-Gaussian gradient fields in reduce-scatter order: dL/dA (9), colour (K),
@@ -665,7 +750,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         const int32_t g = ids[idx];
         const GaussGeom<S> G = load_geom<S>(a, g);
         const GaussShade<S, K> H = load_shade<S, K>(a, g);
-        const PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
+        PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
         bool valid = inside && idx <= bin_final && p.okz;
         S a_s = 1, tex[K];
 #pragma unroll
@@ -674,7 +759,13 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         BWeights<S> bw;
         Texels<S> tt;
         bool inr = false;
-        if constexpr (TEX) {
+        Mip2<S> mm;
+        if constexpr (TEX == kTexMip2) {
+            mm = mip2_setup(p, G, a);
+            p.okz = p.okz && mm.zok;
+            inr = valid && p.okz && mm.valid;
+            if (inr) a_s = mip2_blend<S, K>(mm, a.textures + (int64_t)G.row * a.tex_stride, tex);
+        } else if constexpr (TEX) {
             inr = valid && in_tex_range(p, a);
             if (inr) {
                 bl = bilerp_setup<S>(p.sx, p.sy, a);
@@ -692,7 +783,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         int tpos = 0;
         int32_t rbase = 0;
         if constexpr (TEX) {
-            tl = __ballot(valid && inr);
+            tl = TEX == kTexMip2 ? 0ull : __ballot(valid && inr);  // mip2 adds its texels directly
             tpos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(tl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tl, 0u));
             if (a.recs && tl) {
                 // claim this entry's slots in the row's record segment now; used at the end
@@ -756,7 +847,16 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
 #pragma unroll
                     for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
                     q[3] += v_as;
-                    if (a.recs) {
+                    if constexpr (TEX == kTexMip2) {
+                        // the 8 texels of the two levels: direct atomics, no staging
+                        S *gtb = a.g_tex + (int64_t)G.row * a.tex_stride;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+                                if (q[k] != S(0) && mm.w[i] != S(0)) atomic_add(gtb + mm.off[i] + k, mm.w[i] * q[k]);
+                        }
+                    } else if (a.recs) {
                         // lane 0 claimed the slots; read its value whatever lanes are active here
                         const int32_t pos = __builtin_amdgcn_readlane(rbase, 0) + tpos;
                         S *gtb = a.g_tex + (int64_t)G.row * a.tex_stride;
@@ -1216,12 +1316,14 @@ static int dispatch(const RasterArgs<S> &a, int K, int tex, hipStream_t st) {
     case KK:                                                                              \
         if (BWD) {                                                                        \
             if (tex == kTexNative) launch_bwd<S, KK, kTexNative>(a, grid, bs, st);        \
+            else if (tex == kTexMip2) launch_bwd<S, KK, kTexMip2>(a, grid, bs, st);       \
             else if constexpr (sizeof(S) == 4) {                                          \
                 if (tex == kTexBF16) launch_bwd<S, KK, kTexBF16>(a, grid, bs, st);        \
                 else launch_bwd<S, KK, kTexNone>(a, grid, bs, st);                        \
             } else launch_bwd<S, KK, kTexNone>(a, grid, bs, st);                          \
         } else {                                                                          \
             if (tex == kTexNative) launch_fwd<S, KK, kTexNative>(a, grid, bs, st);        \
+            else if (tex == kTexMip2) launch_fwd<S, KK, kTexMip2>(a, grid, bs, st);       \
             else if constexpr (sizeof(S) == 4) {                                          \
                 if (tex == kTexBF16) launch_fwd<S, KK, kTexBF16>(a, grid, bs, st);        \
                 else launch_fwd<S, KK, kTexNone>(a, grid, bs, st);                        \
@@ -1297,6 +1399,27 @@ static int check_common(int64_t C, int64_t N, int32_t K, const void *textures, i
     return TGS_OK;
 }
 
+static int tex_mode_of(const void *textures, int dtype) {
+    if (!textures) return kTexNone;
+    if (dtype & TGS_TEX_BF16) return kTexBF16;
+    return (dtype & TGS_TEX_MIP2) ? kTexMip2 : kTexNative;
+}
+
+// log2 of the resolution of square power-of-two textures (<= 4096), else -1
+static int mip_log_of(int64_t tex_h, int64_t tex_w) {
+    if (tex_h != tex_w || tex_h < 1 || tex_h > 4096 || (tex_h & (tex_h - 1))) return -1;
+    int l = 0;
+    while ((int64_t(1) << l) < tex_h) ++l;
+    return l;
+}
+
+// mip2: the textures are the flat pyramid [N, (4^(L+1) - 1) / 3, 4]
+template <typename S> static void set_mip(RasterArgs<S> &a, int tex_mode, int64_t tex_h, int64_t tex_w) {
+    if (tex_mode != kTexMip2) return;
+    a.mip_log = mip_log_of(tex_h, tex_w);
+    a.tex_stride = ((int64_t(1) << (2 * (a.mip_log + 1))) - 1) / 3 * 4;
+}
+
 static int check_zero_rows(const void *dst, const uint8_t *mask, size_t row_bytes) {
     TGS_CHECK_ARG(!dst || (mask && ((uintptr_t)dst % 16) == 0 && row_bytes > 0 && row_bytes % 16 == 0),
                   "rasterize: zero_rows_dst needs a mask, 16-byte alignment and rows of a multiple of 16 bytes");
@@ -1321,7 +1444,7 @@ static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterAr
         return TGS_ERR_LAUNCH;
     }
     int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, kGatherMaxLds / std::max<size_t>(lds, 1)));
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
     const int64_t rows_per_block = (N + blocks - 1) / blocks;
@@ -1371,7 +1494,7 @@ extern "C" int tgs_texture_adam_rows(int64_t N, int32_t tex_h, int32_t tex_w, co
         return TGS_ERR_LAUNCH;
     }
     int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, kGatherMaxLds / std::max<size_t>(lds, 1)));
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
     const int64_t rows_per_block = (N + blocks - 1) / blocks;
@@ -1397,9 +1520,11 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  uint8_t *live_entries, int32_t *tex_counts, void *zero_rows_dst,
                                  const uint8_t *zero_rows_mask,
                                  size_t zero_row_bytes, void *workspace, size_t workspace_size, tgs_stream_t stream) {
-    const int tex_mode = !textures ? kTexNone : (dtype & TGS_TEX_BF16) ? kTexBF16 : kTexNative;
-    dtype &= ~TGS_TEX_BF16;
+    const int tex_mode = tex_mode_of(textures, dtype);
+    dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2);
     TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_fwd: bfloat16 textures need float32 compute");
+    TGS_CHECK_ARG(tex_mode != kTexMip2 || mip_log_of(tex_h, tex_w) >= 0,
+                  "rasterize_fwd: the mip2 filter needs square power-of-two textures up to 4096");
     int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
                           tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
@@ -1417,6 +1542,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
                        tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
                        zero_rows_dst, zero_rows_mask, zero_row_bytes, 1, gaussian_ids, nnz);                    \
+        set_mip<S>(a, tex_mode, tex_h, tex_w);                                                                 \
         a.thr = (S)gs_contrib_threshold;                                                                        \
         a.out_c = (S *)render_colors; a.out_a = (S *)render_alphas; a.out_n = (S *)render_normals;              \
         a.out_d = (S *)render_distort; a.out_m = (S *)render_median; a.last_ids = last_ids;                     \
@@ -1452,9 +1578,13 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  void *v_normals, void *v_densify, void *zero_rows_dst,
                                  const uint8_t *zero_rows_mask, size_t zero_row_bytes, void *workspace,
                                  size_t workspace_size, tgs_stream_t stream) {
-    const int tex_mode = !textures ? kTexNone : (dtype & TGS_TEX_BF16) ? kTexBF16 : kTexNative;
-    dtype &= ~TGS_TEX_BF16;
+    const int tex_mode = tex_mode_of(textures, dtype);
+    dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2);
     TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_bwd: bfloat16 textures need float32 compute");
+    TGS_CHECK_ARG(tex_mode != kTexMip2 || (mip_log_of(tex_h, tex_w) >= 0 && uv_grad_mode == TGS_UV_NONE &&
+                                          v_textures && !tex_records),
+                  "rasterize_bwd: the mip2 filter needs square power-of-two textures up to 4096, the reference "
+                  "gradient rule and v_textures");
     int rc = check_common(C, N, K, textures, tex_h, tex_w, texture_range_x, texture_range_y, width, height,
                           tile_size, tile_width, tile_height, n_isects);
     if (rc) return rc;
@@ -1480,6 +1610,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
                        tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
                        zero_rows_dst, zero_rows_mask, zero_row_bytes, 0, gaussian_ids, nnz);                    \
+        set_mip<S>(a, tex_mode, tex_h, tex_w);                                                                 \
         a.r_c = (const S *)render_colors; a.r_a = (const S *)render_alphas; a.r_last = last_ids;                \
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \

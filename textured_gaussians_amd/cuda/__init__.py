@@ -1,6 +1,7 @@
 """Native (gfx950) ops of the textured-2DGS render path and their autograd wrappers."""
 from ._wrapper import (  # noqa: F401
     fully_fused_projection_2dgs,
+    generate_mipmap,
     isect_offset_encode,
     isect_tiles,
     rasterize_to_pixels_2dgs,

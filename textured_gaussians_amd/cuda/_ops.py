@@ -76,6 +76,7 @@ def _same(dtype: torch.dtype, **ts: Tensor) -> None:
 
 
 TEX_BF16 = 0x100  # OR-ed into the rasterizers' dtype: bfloat16 textures
+TEX_MIP2 = 0x200  # OR-ed into the rasterizers' dtype: the mipmapped2 filter over a pyramid
 
 
 def _tex_dtype_flag(textured, textures, means2d):
@@ -357,6 +358,19 @@ def _zero_rows_args(zero_rows, N):
     return _p(dst), _p(mask), dst[0].numel() * dst.element_size()
 
 
+def _tex_dims(textured, textures, mip_log_res):
+    """(tex_h, tex_w, dtype flag): the texture grid, or for the mip2 filter the finest
+    level of the pyramid [N, (4^(L+1) - 1) / 3, 4] (L = mip_log_res)."""
+    if not textured:
+        return 0, 0, 0
+    if mip_log_res is None:
+        return textures.size(1), textures.size(2), 0
+    L = int(mip_log_res)
+    if textures.dim() != 3 or textures.size(1) != mipmap_texels(L) or textures.size(2) != 4:
+        raise RuntimeError(f"mip2 textures must be [N, {mipmap_texels(L)}, 4] (a pyramid over 2^{L} textures)")
+    return 1 << L, 1 << L, TEX_MIP2
+
+
 def _packed_args(means2d, gaussian_ids, textures, n_gaussians):
     """(gaussian_ids or None, nnz, N) for the grid ([C, N, ...]) or packed ([nnz, ...]) layout."""
     if means2d.dim() == 3:
@@ -379,7 +393,8 @@ def _packed_args(means2d, gaussian_ids, textures, n_gaussians):
 
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
-                g_weight, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None, n_gaussians=None):
+                g_weight, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None, n_gaussians=None,
+                mip_log_res=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, tile_offsets=tile_offsets, flatten_ids=flatten_ids)
     if textured:
@@ -400,10 +415,8 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
     th_t, tw_t = tile_offsets.size(1), tile_offsets.size(2)
     H, W = image_height, image_width
     dev, fdt = means2d.device, means2d.dtype
-    if textured:
-        tex_h, tex_w = textures.size(1), textures.size(2)
-    else:
-        tex_h = tex_w = 0
+    tex_h, tex_w, mip_flag = _tex_dims(textured, textures, mip_log_res)
+    tex_flag |= mip_flag
     with torch.cuda.device(dev):
         renders = torch.empty((C, H, W, K), dtype=fdt, device=dev)
         alphas = torch.empty((C, H, W, 1), dtype=fdt, device=dev)
@@ -545,7 +558,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
                 v_textures_out=None, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None,
-                n_gaussians=None):
+                n_gaussians=None, mip_log_res=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -568,10 +581,8 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
     th_t, tw_t = tile_offsets.size(1), tile_offsets.size(2)
     dev = means2d.device
     records_only = textured and v_textures_out is False
-    if textured:
-        tex_h, tex_w = textures.size(1), textures.size(2)
-    else:
-        tex_h = tex_w = 0
+    tex_h, tex_w, mip_flag = _tex_dims(textured, textures, mip_log_res)
+    tex_flag |= mip_flag
     with torch.cuda.device(dev):
         v_means2d = torch.zeros_like(means2d)
         v_rt = torch.zeros_like(ray_transforms)
@@ -655,6 +666,73 @@ def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, nor
                        v_render_alphas, v_render_normals, v_render_distort, v_render_median, absgrad,
                        render_accum_depth=render_accum_depth, live_entries=live_entries, gaussian_ids=gaussian_ids,
                        n_gaussians=n_gaussians)
+
+
+# ---------------------------------------------------------------- mipmaps
+def mipmap_texels(log_texture_res: int) -> int:
+    """Texels of the pyramid over 2^L x 2^L textures: (4^(L+1) - 1) / 3."""
+    return ((1 << (2 * (int(log_texture_res) + 1))) - 1) // 3
+
+
+def generate_mipmap_fwd(textures, log_texture_res, log_reduce=4, tile_size=16):
+    """[N, R, R, C] (R = 2^L) -> [N, (4^(L+1) - 1) / 3, C], coarsest level first
+    (``generate_mipmap_fwd``, ext.cpp:188; ``log_reduce`` and ``tile_size`` only
+    shape the reference's launches)."""
+    _check_input(textures, "textures")
+    dt = _dtype_code(textures, "textures")
+    N, R, R2, C = textures.shape
+    L = int(log_texture_res)
+    if R != R2 or R != 1 << L:
+        raise RuntimeError(f"generate_mipmap: textures must be [N, {1 << L}, {1 << L}, C]")
+    if C > 5:
+        raise RuntimeError(f"Unsupported number of channels: {C}")
+    with torch.cuda.device(textures.device):
+        mip = torch.empty((N, mipmap_texels(L), C), dtype=textures.dtype, device=textures.device)
+        check(lib().tgs_generate_mipmap_fwd(dt, N, C, L, _p(textures), _p(mip), _stream(textures)),
+              "generate_mipmap_fwd")
+    return mip
+
+
+def generate_mipmap_bwd(N, channels, log_texture_res, log_reduce, tile_size, v_mip_textures):
+    """Adjoint of ``generate_mipmap_fwd`` (ext.cpp:192): -> v_textures [N, R, R, C]."""
+    _check_input(v_mip_textures, "v_mip_textures")
+    dt = _dtype_code(v_mip_textures, "v_mip_textures")
+    L = int(log_texture_res)
+    R = 1 << L
+    with torch.cuda.device(v_mip_textures.device):
+        v = torch.empty((N, R, R, channels), dtype=v_mip_textures.dtype, device=v_mip_textures.device)
+        check(lib().tgs_generate_mipmap_bwd(dt, N, channels, L, _p(v_mip_textures), _p(v),
+                                            _stream(v_mip_textures)), "generate_mipmap_bwd")
+    return v
+
+
+def rasterize_to_pixels_fwd_mip2_textured_gaussians(means2d, ray_transforms, colors, opacities, textures,
+                                                    log_texture_res, texture_range_x, texture_range_y, normals,
+                                                    backgrounds, masks, image_width, image_height, tile_size,
+                                                    tile_offsets, flatten_ids, gs_contrib_threshold, g_weight,
+                                                    live_entries=None):
+    """The ``mipmapped2`` forward (ext.cpp:136-137) over the pyramid ``textures``
+    [N, (4^(L+1) - 1) / 3, 4]; returns the 9-tuple of the bilinear forward."""
+    return _raster_fwd(True, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
+                       texture_range_y, normals, backgrounds, masks, image_width, image_height, tile_size,
+                       tile_offsets, flatten_ids, gs_contrib_threshold, g_weight, live_entries=live_entries,
+                       mip_log_res=log_texture_res)[:9]
+
+
+def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, colors, opacities, textures,
+                                                    log_texture_res, texture_range_x, texture_range_y, normals,
+                                                    densify, backgrounds, masks, image_width, image_height,
+                                                    tile_size, tile_offsets, flatten_ids, g_weight, render_colors,
+                                                    render_alphas, last_ids, median_ids, v_render_colors,
+                                                    v_render_alphas, v_render_normals, v_render_distort,
+                                                    v_render_median, absgrad, live_entries=None):
+    """The ``mipmapped2`` backward (ext.cpp:139-140): the reference's rule (no
+    d(sample)/d(position) term); v_textures has the pyramid's shape."""
+    return _raster_bwd(True, UV_NONE, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
+                       texture_range_y, normals, densify, backgrounds, masks, image_width, image_height, tile_size,
+                       tile_offsets, flatten_ids, g_weight, render_colors, render_alphas, last_ids, median_ids,
+                       v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
+                       absgrad, live_entries=live_entries, mip_log_res=log_texture_res)
 
 
 # ------------------------------------------------------------------ Adam

@@ -286,6 +286,23 @@ def rasterize_to_pixels_textured_gaussians(
     tile_height, tile_width = isect_offsets.shape[1:3]
     assert tile_height * tile_size >= image_height, f"Assert Failed: {tile_height} * {tile_size} >= {image_height}"
     assert tile_width * tile_size >= image_width, f"Assert Failed: {tile_width} * {tile_size} >= {image_width}"
+    if filtering == "mipmapped2":
+        # [W]:1944-1989: square power-of-two textures, pyramid built (and differentiated) here
+        if packed or texture_adam is not None:
+            raise NotImplementedError("mipmapped2: packed layout / texture_adam not supported")
+        assert textures.size(1) == textures.size(2)
+        log_res = textures.size(1).bit_length() - 1
+        assert textures.size(1) == 1 << log_res
+        mip = generate_mipmap(textures.contiguous(), log_res, 4, tile_size)
+        out = _RasterizeToPixelsMip2TexturedGaussians.apply(
+            means2d.contiguous(), ray_transforms.contiguous(), colors.contiguous(), opacities.contiguous(), mip,
+            log_res, texture_range_x, texture_range_y, normals.contiguous(), densify.contiguous(), backgrounds, masks,
+            image_width, image_height, tile_size, isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad,
+            distloss, gs_contrib_threshold, g_weight)
+        render_colors = out[0]
+        if padded > 0:
+            render_colors = render_colors[..., :-padded]
+        return (render_colors,) + tuple(out[1:])
     if filtering not in tgs_fns:
         raise Exception(f"Unsupported filter type {filtering}")
     out = _RasterizeToPixelsTexturedGaussians.apply(
@@ -398,6 +415,72 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
         return (None, None, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, None, None, v_normals,
                 v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None, None, None)
+
+
+class _GenerateMipmap(torch.autograd.Function):
+    """Box-filter pyramid of textures [N, R, R, C] -> [N, (4^(L+1) - 1) / 3, C] ([W]:4013-4049)."""
+
+    @staticmethod
+    def forward(ctx, textures, log_texture_res, log_reduce, tile_size):
+        ctx.N, ctx.channels = textures.shape[0], textures.shape[-1]
+        ctx.log_texture_res, ctx.log_reduce, ctx.tile_size = log_texture_res, log_reduce, tile_size
+        return _make_lazy_cuda_func("generate_mipmap_fwd")(textures, log_texture_res, log_reduce, tile_size)
+
+    @staticmethod
+    def backward(ctx, v_mip_textures):
+        v = _make_lazy_cuda_func("generate_mipmap_bwd")(ctx.N, ctx.channels, ctx.log_texture_res, ctx.log_reduce,
+                                                        ctx.tile_size, v_mip_textures.contiguous())
+        return v, None, None, None
+
+
+def generate_mipmap(textures: Tensor, log_texture_res: int, log_reduce: int, tile_size: int) -> Tensor:
+    return _GenerateMipmap.apply(textures, log_texture_res, log_reduce, tile_size)
+
+
+class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
+    """The ``mipmapped2`` filter ([W]:3375-3560): trilinear lookups in the pyramid with
+    the level of detail from the pixel footprint; the reference's gradient rule."""
+
+    @staticmethod
+    def forward(ctx, means2d, ray_transforms, colors, opacities, mip_textures, log_texture_res, texture_range_x,
+                texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
+                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight):
+        ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
+        (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
+         gs_contrib_sum, gs_contrib_count) = _make_lazy_cuda_func("rasterize_to_pixels_fwd_mip2_textured_gaussians")(
+            means2d, ray_transforms, colors, opacities, mip_textures, log_texture_res, texture_range_x,
+            texture_range_y, normals, backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids,
+            gs_contrib_threshold, g_weight, live_entries=ctx.live)
+        ctx.save_for_backward(means2d, ray_transforms, colors, opacities, mip_textures, normals, densify, backgrounds,
+                              masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids)
+        ctx.log_texture_res = log_texture_res
+        ctx.texture_range = (texture_range_x, texture_range_y)
+        ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
+        ctx.absgrad, ctx.g_weight = absgrad, g_weight
+        ctx.mark_non_differentiable(gs_contrib_sum, gs_contrib_count)
+        return (render_colors, render_alphas, render_normals, render_distort, render_median, gs_contrib_sum,
+                gs_contrib_count)
+
+    @staticmethod
+    def backward(ctx, v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
+                 v_gs_contrib_sum, v_gs_contrib_count):
+        (means2d, ray_transforms, colors, opacities, mip_textures, normals, densify, backgrounds, masks,
+         isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids) = ctx.saved_tensors
+        (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, v_normals,
+         v_densify) = _make_lazy_cuda_func("rasterize_to_pixels_bwd_mip2_textured_gaussians")(
+            means2d, ray_transforms, colors, opacities, mip_textures, ctx.log_texture_res, ctx.texture_range[0],
+            ctx.texture_range[1], normals, densify, backgrounds, masks, ctx.width, ctx.height, ctx.tile_size,
+            isect_offsets, flatten_ids, ctx.g_weight, render_colors, render_alphas, last_ids, median_ids,
+            v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
+            v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad, live_entries=ctx.live)
+        ctx.live = None
+        if ctx.absgrad:
+            means2d.absgrad = v_means2d_abs
+        v_backgrounds = None
+        if ctx.needs_input_grad[10]:
+            v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
+        return (v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, None, None, None, v_normals,
+                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None)
 
 
 def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Tensor, opacities: Tensor,
