@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-texture leg")
     ap.add_argument("--no-config5", action="store_true", help="skip the full-training-iteration leg")
     ap.add_argument("--config5-only", action="store_true", help="run only the full-training-iteration leg")
+    ap.add_argument("--no-layouts", action="store_true", help="skip the mipmapped2 and packed-layout legs")
     return ap.parse_args()
 
 
@@ -56,14 +57,14 @@ def make_inputs(n, dev):
     return sc, params
 
 
-def step(tg, sc, params, textured=True):
+def step(tg, sc, params, textured=True, **kw):
     for p in params.values():
         p.grad = None
     op = torch.sigmoid(params["opacities_logit"])
     if textured:
         out = tg.rasterization_textured_gaussians(
             params["means"], params["quats"], params["scales"], op, params["sh"], params["textures"], sc.viewmats,
-            sc.Ks, sc.width, sc.height, sh_degree=3, backgrounds=sc.backgrounds)
+            sc.Ks, sc.width, sc.height, sh_degree=3, backgrounds=sc.backgrounds, **kw)
     else:
         out = tg.rasterization_2dgs(params["means"], params["quats"], params["scales"], op, params["sh"], sc.viewmats,
                                     sc.Ks, sc.width, sc.height, sh_degree=3, backgrounds=sc.backgrounds)
@@ -72,17 +73,17 @@ def step(tg, sc, params, textured=True):
     return out
 
 
-def timed_steps(tg, sc, params, steps, warmup, textured):
+def timed_steps(tg, sc, params, steps, warmup, textured, **kw):
     from textured_gaussians_amd.cuda import _ops
     for _ in range(warmup):
-        step(tg, sc, params, textured)
+        step(tg, sc, params, textured, **kw)
     torch.cuda.synchronize()
     timer = {}
     _ops.set_kernel_timer(timer)
     t0 = time.perf_counter()
     out = None
     for _ in range(steps):
-        out = step(tg, sc, params, textured)
+        out = step(tg, sc, params, textured, **kw)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     _ops.set_kernel_timer(None)
@@ -291,6 +292,45 @@ def neural_leg(tg, n, steps, warmup, dev):
     }
 
 
+def _scene_params(sc):
+    params = dict(means=sc.means, quats=sc.quats, scales=sc.scales,
+                  opacities_logit=torch.logit(sc.opacities.clamp(1e-6, 1 - 1e-6)), sh=sc.colors, textures=sc.textures)
+    for p in params.values():
+        p.requires_grad_(True)
+    return params
+
+
+def filters_leg(tg, steps, warmup, dev, n=1_000_000):
+    """The mipmapped2 filter beside bilinear on the headline scene with 32x32 textures
+    (mipmapped2 needs power-of-two textures): whole fwd+bwd iterations and the
+    rasterizer kernels (HIP events)."""
+    from textured_gaussians_amd import scenes
+    sc = scenes.config2(n=n, tex_res=32, texture_device=dev).to(dev)
+    params = _scene_params(sc)
+    res = {"workload": f"N={n}, 32x32 RGBA f32 textures, SH3, 1297x840, fwd+bwd through the API"}
+    for filt in ("bilinear", "mipmapped2"):
+        ms, kt, out = timed_steps(tg, sc, params, steps, warmup, True, filtering=filt)
+        res[filt] = {"ms_per_iter": round(ms, 3), "raster_ms": {k: round(v, 3) for k, v in kt.items()}}
+        del out
+    return res
+
+
+def packed_leg(tg, steps, warmup, dev, n=500_000):
+    """The packed layout beside the grid layout on the batched-camera scene (8 cameras
+    800x800, N=500k, 32x32 textures): whole fwd+bwd iterations."""
+    from textured_gaussians_amd import scenes
+    sc = scenes.config3(n=n, texture_device=dev).to(dev)
+    params = _scene_params(sc)
+    res = {"workload": f"N={n}, 8 cameras 800x800, 32x32 RGBA f32 textures, SH3, fwd+bwd through the API"}
+    for packed in (False, True):
+        ms, kt, out = timed_steps(tg, sc, params, steps, warmup, True, packed=packed)
+        res["packed" if packed else "grid"] = {"ms_per_iter": round(ms, 3),
+                                               "raster_ms": {k: round(v, 3) for k, v in kt.items()},
+                                               "entries": int(out[-1]["radii"].numel())}
+        del out
+    return res
+
+
 def config5_leg(tg, steps, warmup, dev, n=2_000_000):
     """Full training iteration at scale: N=2M Gaussians, 64x64 RGBA textures stored
     bfloat16 with a float32 master (262 GB of texture state in HBM), 1920x1080, SH3;
@@ -413,6 +453,12 @@ def main():
     if not args.no_neural:
         print("[bench] neural-texture leg", file=sys.stderr, flush=True)
         line["neural"] = neural_leg(tg, args.n, max(3, args.steps // 2), 1, dev)
+        torch.cuda.empty_cache()
+    if not args.no_layouts:
+        print("[bench] filter and layout legs", file=sys.stderr, flush=True)
+        line["filters"] = filters_leg(tg, max(3, args.steps // 2), 1, dev)
+        torch.cuda.empty_cache()
+        line["layouts"] = packed_leg(tg, max(3, args.steps // 2), 1, dev)
         torch.cuda.empty_cache()
     if not args.no_config5:
         print("[bench] full training iteration leg", file=sys.stderr, flush=True)

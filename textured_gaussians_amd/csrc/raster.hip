@@ -643,8 +643,10 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 template <typename S, int K, int TEX>
 __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
     extern __shared__ __align__(16) char smem[];
-    // per-wave staging of texture-gradient updates (2 texel rows x 64 pixels)
-    TexRec<S> *stage = (TexRec<S> *)smem + (threadIdx.x / kWave) * (2 * kWave);
+    // per-wave staging of texture-gradient updates (texel rows x 64 pixels: 2 rows for
+    // bilinear, 4 for mip2, two per level)
+    constexpr int kTexRows = TEX == kTexMip2 ? 4 : 2;
+    TexRec<S> *stage = (TexRec<S> *)smem + (threadIdx.x / kWave) * (kTexRows * kWave);
 
     const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
     const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
@@ -783,7 +785,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         int tpos = 0;
         int32_t rbase = 0;
         if constexpr (TEX) {
-            tl = TEX == kTexMip2 ? 0ull : __ballot(valid && inr);  // mip2 adds its texels directly
+            tl = __ballot(valid && inr);
             tpos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(tl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tl, 0u));
             if (a.recs && tl) {
                 // claim this entry's slots in the row's record segment now; used at the end
@@ -848,13 +850,17 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
                     for (int k = 0; k < K; ++k) q[k] = fac * vc[k];
                     q[3] += v_as;
                     if constexpr (TEX == kTexMip2) {
-                        // the 8 texels of the two levels: direct atomics, no staging
-                        S *gtb = a.g_tex + (int64_t)G.row * a.tex_stride;
+                        // four texel rows (two per level) of two adjacent texels each
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) {
+                        for (int j = 0; j < 4; ++j) {
+                            TexRec<S> &r = stage[j * kWave + tpos];
+                            r.off = mm.off[2 * j] * (uint32_t)sizeof(S);
+                            r.du = (mm.off[2 * j + 1] - mm.off[2 * j]) * (uint32_t)sizeof(S);
 #pragma unroll
-                            for (int k = 0; k < 4; ++k)
-                                if (q[k] != S(0) && mm.w[i] != S(0)) atomic_add(gtb + mm.off[i] + k, mm.w[i] * q[k]);
+                            for (int k = 0; k < 4; ++k) {
+                                r.v[k] = mm.w[2 * j] * q[k];
+                                r.v[4 + k] = mm.w[2 * j + 1] * q[k];
+                            }
                         }
                     } else if (a.recs) {
                         // lane 0 claimed the slots; read its value whatever lanes are active here
@@ -946,7 +952,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
                     const int src = grp * 8 + sub;
                     if (src < n_tl) {
 #pragma unroll
-                        for (int row = 0; row < 2; ++row) {
+                        for (int row = 0; row < kTexRows; ++row) {
                             const TexRec<S> &rr = stage[row * kWave + src];
                             const S val = rr.v[e];
                             const uint32_t off = rr.off + (e >> 2) * rr.du + (e & 3) * (uint32_t)sizeof(S);
@@ -1300,7 +1306,14 @@ template <typename S, int K, int TEX> static void launch_fwd(const RasterArgs<S>
 }
 template <typename S, int K, int TEX> static void launch_bwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
     launch_pack<S, K>(a, st);
-    const size_t stage = TEX ? (size_t)(bs / kWave) * 2 * kWave * sizeof(TexRec<S>) : 0;
+    const size_t rows = TEX == kTexMip2 ? 4 : 2;
+    const size_t stage = TEX ? (size_t)(bs / kWave) * rows * kWave * sizeof(TexRec<S>) : 0;
+    if (stage > 64 * 1024) {  // mip2 in float64: 72 KB of staging per 256-thread block
+        static bool raised = false;
+        if (!raised)
+            raised = hipFuncSetAttribute((const void *)raster_bwd_kernel<S, K, TEX>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage) == hipSuccess;
+    }
     raster_bwd_kernel<S, K, TEX><<<grid, bs, stage, st>>>(a);
 }
 

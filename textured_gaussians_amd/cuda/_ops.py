@@ -710,13 +710,13 @@ def rasterize_to_pixels_fwd_mip2_textured_gaussians(means2d, ray_transforms, col
                                                     log_texture_res, texture_range_x, texture_range_y, normals,
                                                     backgrounds, masks, image_width, image_height, tile_size,
                                                     tile_offsets, flatten_ids, gs_contrib_threshold, g_weight,
-                                                    live_entries=None):
+                                                    live_entries=None, zero_rows=None):
     """The ``mipmapped2`` forward (ext.cpp:136-137) over the pyramid ``textures``
     [N, (4^(L+1) - 1) / 3, 4]; returns the 9-tuple of the bilinear forward."""
     return _raster_fwd(True, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                        texture_range_y, normals, backgrounds, masks, image_width, image_height, tile_size,
                        tile_offsets, flatten_ids, gs_contrib_threshold, g_weight, live_entries=live_entries,
-                       mip_log_res=log_texture_res)[:9]
+                       zero_rows=zero_rows, mip_log_res=log_texture_res)[:9]
 
 
 def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, colors, opacities, textures,
@@ -725,14 +725,16 @@ def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, col
                                                     tile_size, tile_offsets, flatten_ids, g_weight, render_colors,
                                                     render_alphas, last_ids, median_ids, v_render_colors,
                                                     v_render_alphas, v_render_normals, v_render_distort,
-                                                    v_render_median, absgrad, live_entries=None):
+                                                    v_render_median, absgrad, live_entries=None, v_textures_out=None,
+                                                    zero_rows=None):
     """The ``mipmapped2`` backward (ext.cpp:139-140): the reference's rule (no
     d(sample)/d(position) term); v_textures has the pyramid's shape."""
     return _raster_bwd(True, UV_NONE, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                        texture_range_y, normals, densify, backgrounds, masks, image_width, image_height, tile_size,
                        tile_offsets, flatten_ids, g_weight, render_colors, render_alphas, last_ids, median_ids,
                        v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
-                       absgrad, live_entries=live_entries, mip_log_res=log_texture_res)
+                       absgrad, live_entries=live_entries, v_textures_out=v_textures_out, zero_rows=zero_rows,
+                       mip_log_res=log_texture_res)
 
 
 # ------------------------------------------------------------------ Adam

@@ -446,11 +446,17 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
                 texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
                 flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight):
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
+        # the pyramid's gradient, zero-filled along the rasterizer walks (listed rows by the
+        # forward, the others by the backward), as for the bilinear filter
+        ctx.v_tex = None
+        if ctx.needs_input_grad[4] and mip_textures.is_cuda:
+            v_tex = torch.empty(mip_textures.shape, dtype=means2d.dtype, device=mip_textures.device)
+            ctx.v_tex = (v_tex, _C.mark_rows(flatten_ids, mip_textures.shape[0]))
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count) = _make_lazy_cuda_func("rasterize_to_pixels_fwd_mip2_textured_gaussians")(
             means2d, ray_transforms, colors, opacities, mip_textures, log_texture_res, texture_range_x,
             texture_range_y, normals, backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids,
-            gs_contrib_threshold, g_weight, live_entries=ctx.live)
+            gs_contrib_threshold, g_weight, live_entries=ctx.live, zero_rows=ctx.v_tex)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, mip_textures, normals, densify, backgrounds,
                               masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids)
         ctx.log_texture_res = log_texture_res
@@ -472,8 +478,10 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
             ctx.texture_range[1], normals, densify, backgrounds, masks, ctx.width, ctx.height, ctx.tile_size,
             isect_offsets, flatten_ids, ctx.g_weight, render_colors, render_alphas, last_ids, median_ids,
             v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
-            v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad, live_entries=ctx.live)
+            v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad, live_entries=ctx.live,
+            v_textures_out=ctx.v_tex[0] if ctx.v_tex is not None else None, zero_rows=ctx.v_tex)
         ctx.live = None
+        ctx.v_tex = None
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
         v_backgrounds = None
