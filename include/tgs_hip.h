@@ -319,6 +319,23 @@ int tgs_generate_mipmap_fwd(int dtype, int64_t N, int32_t C, int32_t log_texture
 int tgs_generate_mipmap_bwd(int dtype, int64_t N, int32_t C, int32_t log_texture_res,
                             const void *v_mip_textures, void *v_textures, tgs_stream_t stream);
 
+/* SSIM of the trainer's loss (no C++ counterpart in the reference: it calls
+ * torchmetrics' StructuralSimilarityIndexMeasure(data_range=1.0),
+ * examples/simple_trainer_textured_gaussians.py:430, 1426-1429): 11-tap Gaussian
+ * window (sigma 1.5), c1 = 1e-4, c2 = 9e-4, mean over the windows inside the image.
+ * x, y: [B, H, W, C] channels-last (C <= 4, H, W >= 11, B * H <= 65535).
+ * tgs_ssim_fwd writes tgs_ssim_partials(...) per-block partial sums of the SSIM map
+ * (their sum / (B C (H-10) (W-10)) is the SSIM) and keeps in `work`
+ * (tgs_ssim_work_bytes) what tgs_ssim_bwd needs: dx = scale * dSSIM_sum/dx, the
+ * gradient of the sum of the map with respect to x (y held constant).  The two
+ * calls on one work buffer must be ordered; the backward overwrites the window sums. */
+size_t tgs_ssim_work_bytes(int dtype, int32_t B, int32_t H, int32_t W, int32_t C);
+int64_t tgs_ssim_partials(int32_t B, int32_t H, int32_t W, int32_t C);
+int tgs_ssim_fwd(int dtype, int32_t B, int32_t H, int32_t W, int32_t C, const void *x, const void *y,
+                 void *work, void *partials, tgs_stream_t stream);
+int tgs_ssim_bwd(int dtype, int32_t B, int32_t H, int32_t W, int32_t C, const void *x, const void *y,
+                 void *work, double scale, void *dx, tgs_stream_t stream);
+
 /* Selective Adam step of bfloat16 textures from texture-gradient records (no
  * reference counterpart; the arithmetic is the reference's `selective_adam_update`,
  * adam.cu:16-47: no bias correction, rows with visibility[n] == 0 untouched).

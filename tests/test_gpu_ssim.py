@@ -1,0 +1,43 @@
+"""The fused SSIM kernels against the torch restatement of torchmetrics' algorithm
+(``losses.ssim``, itself pinned against a brute-force window sum on the CPU): value
+and gradient, float64 and float32, including constant regions (clamped variances)."""
+import pytest
+import torch
+
+from textured_gaussians_amd import losses
+
+from gpu_util import DEV, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-5)])
+@pytest.mark.parametrize("shape", [(1, 37, 53, 3), (2, 11, 16, 4), (1, 64, 96, 1)])
+def test_fused_ssim_matches_torch(dtype, tol, shape):
+    g = torch.Generator().manual_seed(sum(shape))
+    x = torch.rand(shape, generator=g, dtype=torch.float64)
+    y = (x + 0.3 * torch.randn(shape, generator=g, dtype=torch.float64)).clamp(0, 1)
+    y[:, : shape[1] // 3] = 0.5  # a constant band
+    x[:, -3:, : shape[2] // 2] = 0.25
+    xd = x.to(DEV, dtype).requires_grad_(True)
+    s = losses.ssim_hwc(xd, y.to(DEV, dtype))
+    s.backward()
+    xr = x.to(dtype).requires_grad_(True)
+    sr = losses.ssim(xr.permute(0, 3, 1, 2), y.to(dtype).permute(0, 3, 1, 2))
+    sr.backward()
+    assert abs(float(s) - float(sr)) < tol * 10
+    assert rel_err(xd.grad, xr.grad) < tol * 100
+
+
+def test_fused_ssim_in_the_trainer_loss_at_full_hd():
+    g = torch.Generator().manual_seed(7)
+    r = torch.rand(1, 1080, 1920, 3, generator=g).to(DEV).requires_grad_(True)
+    t = torch.rand(1, 1080, 1920, 3, generator=g).to(DEV)
+    loss = losses.l1_dssim_loss(r, t)
+    loss.backward()
+    r64 = r.detach().double().cpu().requires_grad_(True)
+    ref = 0.8 * (r64 - t.double().cpu()).abs().mean() + 0.2 * (1 - losses.ssim(r64.permute(0, 3, 1, 2),
+                                                                               t.double().cpu().permute(0, 3, 1, 2)))
+    ref.backward()
+    assert abs(float(loss) - float(ref)) < 1e-5
+    assert rel_err(r.grad, r64.grad) < 1e-4

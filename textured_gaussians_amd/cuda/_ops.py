@@ -737,6 +737,37 @@ def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, col
                        mip_log_res=log_texture_res)
 
 
+# ------------------------------------------------------------------- SSIM
+def ssim_fwd(x, y):
+    """Fused SSIM of channels-last images [B, H, W, C] (the trainer's torchmetrics
+    settings): returns (ssim, work) with ``work`` kept for ``ssim_bwd``."""
+    for n, t in dict(x=x, y=y).items():
+        _check_input(t, n)
+    dt = _dtype_code(x, "x")
+    _same(x.dtype, y=y)
+    if x.dim() != 4 or x.shape != y.shape:
+        raise RuntimeError("ssim: x and y must be [B, H, W, C] of one shape")
+    B, H, W, C = x.shape
+    L = lib()
+    with torch.cuda.device(x.device):
+        work = torch.empty(max(1, L.tgs_ssim_work_bytes(dt, B, H, W, C)), dtype=torch.uint8, device=x.device)
+        part = torch.empty(max(1, L.tgs_ssim_partials(B, H, W, C)), dtype=x.dtype, device=x.device)
+        check(L.tgs_ssim_fwd(dt, B, H, W, C, _p(x), _p(y), _p(work), _p(part), _stream(x)), "ssim_fwd")
+    M = B * C * (H - 10) * (W - 10)
+    return part.sum() / M, work
+
+
+def ssim_bwd(x, y, work):
+    """d(SSIM)/dx for the ``work`` of ``ssim_fwd(x, y)`` (y held constant)."""
+    dt = _dtype_code(x, "x")
+    B, H, W, C = x.shape
+    M = B * C * (H - 10) * (W - 10)
+    with torch.cuda.device(x.device):
+        dx = torch.empty_like(x)
+        check(lib().tgs_ssim_bwd(dt, B, H, W, C, _p(x), _p(y), _p(work), 1.0 / M, _p(dx), _stream(x)), "ssim_bwd")
+    return dx
+
+
 # ------------------------------------------------------------------ Adam
 def selective_adam_update(param, param_grad, exp_avg, exp_avg_sq, tiles_touched, lr, b1, b2, eps, N, M):
     for n, t in dict(param=param, param_grad=param_grad, exp_avg=exp_avg, exp_avg_sq=exp_avg_sq,

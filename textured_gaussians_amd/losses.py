@@ -42,8 +42,34 @@ def ssim(preds: Tensor, target: Tensor, data_range: float = 1.0, kernel_size: in
     return ssim_map.reshape(B, -1).mean(-1).mean()
 
 
+class _FusedSSIM(torch.autograd.Function):
+    """SSIM of channels-last [B, H, W, C] images by the gfx950 kernels (the same
+    formula as ``ssim``); differentiable in the first argument."""
+
+    @staticmethod
+    def forward(ctx, x, y):
+        from .cuda import _ops
+        s, work = _ops.ssim_fwd(x, y)
+        ctx.save_for_backward(x, y, work)
+        return s
+
+    @staticmethod
+    def backward(ctx, g):
+        from .cuda import _ops
+        x, y, work = ctx.saved_tensors
+        return _ops.ssim_bwd(x, y, work) * g, None
+
+
+def ssim_hwc(x: Tensor, y: Tensor) -> Tensor:
+    """Mean SSIM of channels-last [B, H, W, C] images; the fused kernels on the GPU
+    (gradient with respect to ``x`` only), ``ssim`` elsewhere."""
+    if x.is_cuda and x.dtype in (torch.float32, torch.float64) and not y.requires_grad and x.shape[-1] <= 4:
+        return _FusedSSIM.apply(x.contiguous(), y.contiguous())
+    return ssim(x.permute(0, 3, 1, 2), y.permute(0, 3, 1, 2))
+
+
 def l1_dssim_loss(render: Tensor, target: Tensor, ssim_lambda: float = 0.2) -> Tensor:
     """The trainer's loss on [B, H, W, C] images: (1 - l) L1 + l (1 - SSIM)."""
     l1 = (render - target).abs().mean()
-    d = 1.0 - ssim(target.permute(0, 3, 1, 2), render.permute(0, 3, 1, 2))
+    d = 1.0 - ssim_hwc(render, target)  # SSIM is symmetric in its two images
     return l1 * (1.0 - ssim_lambda) + d * ssim_lambda
