@@ -432,11 +432,14 @@ def main():
         "mpix_per_s": round(ab["P"] / (ms * 1e-3) / 1e6, 2),
         "raster_ms": {"fwd": round(t_fwd, 3), "bwd": round(t_bwd, 3)},
         "raster_hbm_frac_fwd_bwd": round(achieved_fr / HBM_PEAK_GBS, 4),
+        # the same against the device-to-device copy measured on this box
+        "raster_frac_of_copy_fwd_bwd": round(achieved_fr / copy_gbs, 4),
         "plain_2dgs": plain,
         "copy_bandwidth_GBs": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved_bwd, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_bwd / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "raster_bwd_kernel<float,3,true>", "algorithmic_bytes": ab["bytes_bwd"]},
+                     "frac_of_copy": round(achieved_bwd / copy_gbs, 4),
+                     "kernel": "raster_bwd_kernel<float,3,1>", "algorithmic_bytes": ab["bytes_bwd"]},
     }
     if not args.no_cpu_baseline:
         print("[bench] CPU oracle baseline (bounded sample)", file=sys.stderr, flush=True)
