@@ -153,6 +153,13 @@ __global__ __launch_bounds__(256) void isect_count_kernel(int64_t n, const float
     tpg[idx] = (R.y1 - R.y0) * (R.x1 - R.x0);
 }
 
+// One wavefront per 64 consecutive Gaussians, emitting their intersections
+// together: the wave's output range is contiguous (cumulative counts), each lane
+// takes every 64th position of it and finds the Gaussian that owns the position
+// by a binary search over the lanes' start offsets (6 shuffles), so the writes
+// are coalesced and no lane waits for a large Gaussian's loop.  The keys and
+// their positions are those of a per-Gaussian loop over its tile rectangle in
+// row-major order.
 __global__ __launch_bounds__(256) void isect_fill_kernel(int64_t n, int64_t N, const int64_t *__restrict__ camera_ids,
                                                          const float *__restrict__ means2d,
                                                          const int32_t *__restrict__ radii,
@@ -160,21 +167,49 @@ __global__ __launch_bounds__(256) void isect_fill_kernel(int64_t n, int64_t N, c
                                                          const int64_t *__restrict__ cum, int32_t ts, int32_t tw,
                                                          int32_t th, int32_t tile_bits,
                                                          int64_t *__restrict__ keys, int32_t *__restrict__ vals) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    const int32_t r = radii[idx];
-    if (r <= 0) return;
-    const Rect R = tile_rect(means2d, r, idx, ts, tw, th);
-    const int64_t cam = camera_ids ? camera_ids[idx] : idx / N;  // packed entries carry their camera
-    const int64_t hi = cam << (32 + tile_bits);
-    const int64_t dbits = (int64_t)__float_as_int(depths[idx]);  // sign-extended
-    int64_t o = idx == 0 ? 0 : cum[idx - 1];
-    for (int32_t y = R.y0; y < R.y1; ++y)
-        for (int32_t x = R.x0; x < R.x1; ++x) {
-            keys[o] = hi | ((int64_t)(y * tw + x) << 32) | dbits;
-            vals[o] = (int32_t)idx;
-            ++o;
+    const int lane = threadIdx.x & 63;
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);  // the wave's first Gaussian
+    if (base >= n) return;                                                           // uniform over the wave
+    const int64_t idx = base + lane;
+    long long start = LLONG_MAX;  // lanes past n own nothing
+    int32_t x0 = 0, y0 = 0, w = 1, dbits = 0;
+    long long hi = 0;
+    if (idx < n) {
+        start = idx == 0 ? 0 : cum[idx - 1];
+        const int32_t r = radii[idx];
+        if (r > 0) {
+            const Rect R = tile_rect(means2d, r, idx, ts, tw, th);
+            x0 = R.x0;
+            y0 = R.y0;
+            w = max(R.x1 - R.x0, 1);
+            const int64_t cam = camera_ids ? camera_ids[idx] : idx / N;  // packed entries carry their camera
+            hi = (long long)(cam << (32 + tile_bits));
+            dbits = __float_as_int(depths[idx]);
         }
+    }
+    const int64_t last = min(n, base + 64) - 1;
+    const long long first = __shfl(start, 0);
+    const long long end = cum[last];
+    for (long long pos = first + lane; __ballot(pos < end) != 0; pos += 64) {
+        // owner: the last lane whose start <= pos (lanes of empty Gaussians share the
+        // next lane's start, so the last of them is the one that owns positions)
+        int owner = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const long long s = __shfl(start, owner + step);
+            if (s <= pos) owner += step;
+        }
+        const long long so = __shfl(start, owner);
+        const int32_t ox0 = __shfl(x0, owner), oy0 = __shfl(y0, owner), ow = __shfl(w, owner);
+        const long long ohi = __shfl(hi, owner);
+        const int32_t od = __shfl(dbits, owner);
+        if (pos < end) {
+            const int32_t k = (int32_t)(pos - so);
+            const int32_t dy = k / ow, dx = k - dy * ow;
+            keys[pos] = ohi | ((int64_t)((oy0 + dy) * tw + ox0 + dx) << 32) | (int64_t)od;  // depth bits sign-extended
+            vals[pos] = (int32_t)(base + owner);
+        }
+    }
 }
 
 // ------------------------------------------------------------ radix sort

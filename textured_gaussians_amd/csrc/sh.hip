@@ -175,6 +175,95 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int64_t n, int K, int degre
     }
 }
 
+// float32 with K * 3 <= 48 (SH degree 3): the same per-Gaussian arithmetic with the
+// wave's coefficient block (64 Gaussians x K * 3 values, contiguous in memory)
+// loaded and its v_coeffs block stored through LDS, so global accesses are
+// contiguous 256-byte runs instead of 64 lines 192 bytes apart.  Rows are padded
+// to an odd stride (49) against bank conflicts.
+constexpr int kShStage = 49;
+
+__global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, int degree, const float *__restrict__ dirs,
+                                                            const float *__restrict__ coeffs,
+                                                            const uint8_t *__restrict__ masks,
+                                                            const float *__restrict__ v_colors,
+                                                            float *__restrict__ v_coeffs, float *__restrict__ v_dirs) {
+    __shared__ float sm[4][64 * kShStage];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x + wv * 64;
+    if (base >= n) return;  // uniform over the wave
+    const int K3 = K * 3;
+    const int nw = (int)min((int64_t)64, n - base);
+    const int64_t tot = (int64_t)nw * K3;
+    float *st = sm[wv];
+    {
+        int t = lane / K3, k = lane - (lane / K3) * K3;  // element lane + 64 j of the block = (t, k)
+        for (int64_t e = lane; e < tot; e += 64) {
+            st[t * kShStage + k] = coeffs[base * K3 + e];
+            k += 64;
+            while (k >= K3) {
+                k -= K3;
+                ++t;
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const int64_t i = base + lane;
+    if (i < n) {
+        float *c = st + lane * kShStage;  // coefficients in, v_coeffs out (same slots)
+        if (masks && !masks[i]) {
+            for (int k = 0; k < K3; ++k) c[k] = 0.f;
+        } else {
+            const int nb = (degree + 1) * (degree + 1);
+            float x = 0, y = 0, z = 0, inv = 0;
+            if (nb > 1) {
+                const float dx = dirs[i * 3], dy = dirs[i * 3 + 1], dz = dirs[i * 3 + 2];
+                inv = tgs_rsqrt<float>(dx * dx + dy * dy + dz * dz);
+                x = dx * inv; y = dy * inv; z = dz * inv;
+            }
+            float Y[25], G[25][3];
+            const bool want_dir = v_dirs != nullptr && nb > 1;
+            sh_eval<float>(nb, x, y, z, Y, want_dir ? G : nullptr);
+            const float vr = v_colors[i * 3], vg = v_colors[i * 3 + 1], vb = v_colors[i * 3 + 2];
+            float gu = 0, gv = 0, gw = 0;
+            for (int k = 0; k < K; ++k) {
+                if (k < nb) {
+                    if (want_dir) {
+                        const float s = c[k * 3] * vr + c[k * 3 + 1] * vg + c[k * 3 + 2] * vb;
+                        gu += s * G[k][0];
+                        gv += s * G[k][1];
+                        gw += s * G[k][2];
+                    }
+                    c[k * 3] = Y[k] * vr;
+                    c[k * 3 + 1] = Y[k] * vg;
+                    c[k * 3 + 2] = Y[k] * vb;
+                } else {
+                    c[k * 3] = 0.f; c[k * 3 + 1] = 0.f; c[k * 3 + 2] = 0.f;
+                }
+            }
+            if (want_dir) {
+                const float gd = gu * x + gv * y + gw * z;
+                atomic_add(v_dirs + i * 3, (gu - gd * x) * inv);
+                atomic_add(v_dirs + i * 3 + 1, (gv - gd * y) * inv);
+                atomic_add(v_dirs + i * 3 + 2, (gw - gd * z) * inv);
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    {
+        int t = lane / K3, k = lane - (lane / K3) * K3;
+        for (int64_t e = lane; e < tot; e += 64) {
+            v_coeffs[base * K3 + e] = st[t * kShStage + k];
+            k += 64;
+            while (k >= K3) {
+                k -= K3;
+                ++t;
+            }
+        }
+    }
+}
+
 }  // namespace tgs
 
 using namespace tgs;
@@ -208,7 +297,10 @@ extern "C" int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const
     if (n <= 0) return TGS_OK;
     const unsigned nb = (unsigned)ceil_div(n, 256);
     hipStream_t st = (hipStream_t)stream;
-    if (dtype == TGS_F32)
+    if (dtype == TGS_F32 && K * 3 <= kShStage - 1)
+        sh_bwd_staged_kernel<<<nb, 256, 0, st>>>(n, K, degree, (const float *)dirs, (const float *)coeffs, masks,
+                                                 (const float *)v_colors, (float *)v_coeffs, (float *)v_dirs);
+    else if (dtype == TGS_F32)
         sh_bwd_kernel<float><<<nb, 256, 0, st>>>(n, K, degree, (const float *)dirs, (const float *)coeffs, masks,
                                                  (const float *)v_colors, (float *)v_coeffs, (float *)v_dirs);
     else if (dtype == TGS_F64)
