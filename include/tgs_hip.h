@@ -378,6 +378,11 @@ int tgs_gemm_bf16(int trans_a, int trans_b, int epilogue, int64_t M, int64_t N, 
                   float *C, int64_t ldc, const float *mask, int64_t ldm, int32_t k_splits,
                   tgs_stream_t stream);
 int tgs_sigmoid_bwd(int64_t n, const float *g_out, const float *out, float *g, tgs_stream_t stream);
+/* Sigmoid backward fused with the column sums of its result (no reference
+ * counterpart; the decoder's output layer): g = g_out * o * (1 - o) for [M, N]
+ * float32 (N a multiple of 4, 16-byte aligned), and db[N] += g.sum(0). */
+int tgs_sigmoid_bwd_colsum(int64_t M, int64_t N, const float *g_out, const float *out, float *g,
+                           float *db, tgs_stream_t stream);
 int tgs_colsum(int64_t M, int64_t N, const float *G, int64_t ldg, float *db, tgs_stream_t stream);
 
 /* Fused decoder inference: all three layers in one kernel, activations kept in

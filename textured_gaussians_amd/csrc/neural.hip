@@ -160,6 +160,42 @@ __global__ __launch_bounds__(256) void sigmoid_bwd_kernel(int64_t n, const float
     }
 }
 
+// sigmoid backward and the column sums of its result in one pass (the output
+// layer's bias gradient): g = g_out * o * (1 - o) with 4-vectors, each thread
+// keeps running sums for its 4 columns; one workgroup per (256-column strip,
+// chunk of rows), one atomic per column per workgroup.  Saves re-reading g.
+__global__ __launch_bounds__(256) void sigmoid_bwd_colsum_kernel(int M, int N4, const float4 *__restrict__ g_out,
+                                                                 const float4 *__restrict__ out,
+                                                                 float4 *__restrict__ g, int rows_per_block,
+                                                                 float *__restrict__ db) {
+    __shared__ float4 part[4][64];
+    const int t = threadIdx.x, c = t % 64, rq = t / 64;
+    const int n4 = blockIdx.x * 64 + c;
+    const int r0 = blockIdx.y * rows_per_block, r1 = min(M, r0 + rows_per_block);
+    float4 s = {0.f, 0.f, 0.f, 0.f};
+    if (n4 < N4)
+        for (int r = r0 + rq; r < r1; r += 4) {
+            const int64_t i = (int64_t)r * N4 + n4;
+            const float4 o = out[i], go = g_out[i];
+            float4 gg;
+            gg.x = go.x * o.x * (1.f - o.x);
+            gg.y = go.y * o.y * (1.f - o.y);
+            gg.z = go.z * o.z * (1.f - o.z);
+            gg.w = go.w * o.w * (1.f - o.w);
+            g[i] = gg;
+            s.x += gg.x; s.y += gg.y; s.z += gg.z; s.w += gg.w;
+        }
+    part[rq][c] = s;
+    __syncthreads();
+    if (rq == 0 && n4 < N4) {
+        const float4 a = part[0][c], b = part[1][c], cc = part[2][c], d = part[3][c];
+        atomic_add(db + 4 * n4, a.x + b.x + cc.x + d.x);
+        atomic_add(db + 4 * n4 + 1, a.y + b.y + cc.y + d.y);
+        atomic_add(db + 4 * n4 + 2, a.z + b.z + cc.z + d.z);
+        atomic_add(db + 4 * n4 + 3, a.w + b.w + cc.w + d.w);
+    }
+}
+
 // column sums of G[M,N] added into db[N] (float32, unrounded); one workgroup per
 // (64-column strip, chunk of rows), one atomic per column per workgroup
 __global__ __launch_bounds__(256) void colsum_kernel(int M, int N, const float *__restrict__ G, int64_t ldg,
@@ -458,6 +494,22 @@ extern "C" int tgs_sigmoid_bwd(int64_t n, const float *g_out, const float *out, 
     if (n == 0) return TGS_OK;
     sigmoid_bwd_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, (hipStream_t)stream>>>(n, g_out, out, g);
     TGS_CHECK_LAUNCH("sigmoid_bwd");
+    return TGS_OK;
+}
+
+extern "C" int tgs_sigmoid_bwd_colsum(int64_t M, int64_t N, const float *g_out, const float *out, float *g,
+                                      float *db, tgs_stream_t stream) {
+    TGS_CHECK_ARG(M >= 0 && N >= 0 && M < (int64_t(1) << 31) && N % 4 == 0 && (M == 0 || (g_out && out && g && db)),
+                  "sigmoid_bwd_colsum: bad arguments (N must be a multiple of 4)");
+    TGS_CHECK_ARG((((uintptr_t)g_out | (uintptr_t)out | (uintptr_t)g) % 16) == 0,
+                  "sigmoid_bwd_colsum: buffers must be 16-byte aligned");
+    if (M == 0 || N == 0) return TGS_OK;
+    const int rows = 4096;
+    const int64_t N4 = N / 4;
+    dim3 grid((unsigned)ceil_div(N4, 64), (unsigned)ceil_div(M, rows));
+    sigmoid_bwd_colsum_kernel<<<grid, 256, 0, (hipStream_t)stream>>>((int)M, (int)N4, (const float4 *)g_out,
+                                                                     (const float4 *)out, (float4 *)g, rows, db);
+    TGS_CHECK_LAUNCH("sigmoid_bwd_colsum");
     return TGS_OK;
 }
 

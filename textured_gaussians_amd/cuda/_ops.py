@@ -868,6 +868,21 @@ def sigmoid_bwd(g_out: Tensor, out: Tensor) -> Tensor:
     return g
 
 
+def sigmoid_bwd_colsum(g_out: Tensor, out: Tensor):
+    """(g_out * out * (1 - out), its column sums) for 2-D float32 [M, N] (N % 4 == 0),
+    in one pass."""
+    _f32_2d(out, "out")
+    _check_input(g_out, "g_out")
+    if g_out.shape != out.shape or g_out.dtype != torch.float32 or out.shape[1] % 4:
+        raise RuntimeError("sigmoid_bwd_colsum: g_out and out must be float32 [M, N] with N % 4 == 0")
+    g = torch.empty_like(out)
+    db = torch.zeros(out.shape[1], dtype=torch.float32, device=out.device)
+    with torch.cuda.device(out.device):
+        check(lib().tgs_sigmoid_bwd_colsum(out.shape[0], out.shape[1], _p(g_out), _p(out), _p(g), _p(db),
+                                           _stream(out)), "sigmoid_bwd_colsum")
+    return g, db
+
+
 def colsum_(g: Tensor, db: Tensor) -> Tensor:
     """db += g.sum(0) for a 2-D float32 g (float32 accumulation)."""
     _f32_2d(g, "g")

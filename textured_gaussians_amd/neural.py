@@ -71,9 +71,8 @@ class _DecodeFn(torch.autograd.Function):
         else:
             z, W1, h1, W2, h2, W3, out = ctx.saved_tensors
         ks = _k_splits(z.shape[0])
-        g3 = E.sigmoid_bwd(g_out.contiguous(), out)
+        g3, db3 = E.sigmoid_bwd_colsum(g_out.contiguous(), out)
         dW3 = E.gemm_bf16(h2, g3, trans_a=True, epilogue=E.EPI_ATOMIC, out=torch.zeros_like(W3), k_splits=ks)
-        db3 = E.colsum_(g3, torch.zeros(W3.shape[1], device=g3.device))
         g2 = E.gemm_bf16(g3, W3, trans_b=True, epilogue=E.EPI_RELU_MASK, mask=h2)
         dW2 = E.gemm_bf16(h1, g2, trans_a=True, epilogue=E.EPI_ATOMIC, out=torch.zeros_like(W2), k_splits=ks)
         db2 = E.colsum_(g2, torch.zeros(W2.shape[1], device=g2.device))
