@@ -364,7 +364,9 @@ int tgs_texture_adam_rows(int64_t N, int32_t tex_h, int32_t tex_w, const int32_t
  *
  * tgs_gemm_bf16: C[M,N] = epilogue(A[M,K] B[K,N] + bias[N]) on the matrix cores
  * (v_mfma_f32_16x16x32_bf16): operands are float32 in memory, rounded to
- * bfloat16 (nearest even) when staged, accumulated in float32.  A element
+ * bfloat16 (nearest even) when staged, accumulated in float32; an operand
+ * flagged TGS_GEMM_A_BF16 / TGS_GEMM_B_BF16 (OR-ed into `epilogue`) is
+ * bfloat16 bits in memory instead and is staged as is.  A element
  * (m,k) is A[m*lda+k] (trans_a 0) or A[k*lda+m] (trans_a 1); B element (k,n)
  * is B[k*ldb+n] (trans_b 0) or B[n*ldb+k] (trans_b 1).  Epilogues: NONE, RELU,
  * SIGMOID, RELU_MASK (keep where mask[m*ldm+n] > 0, for ReLU backward), ATOMIC
@@ -373,16 +375,19 @@ int tgs_texture_adam_rows(int64_t N, int32_t tex_h, int32_t tex_w, const int32_t
  * tgs_sigmoid_bwd: g = g_out * out * (1 - out).
  * tgs_colsum: db[n] += sum_m G[m*ldg+n] (float32).                         */
 enum { TGS_EPI_NONE = 0, TGS_EPI_RELU = 1, TGS_EPI_SIGMOID = 2, TGS_EPI_RELU_MASK = 3, TGS_EPI_ATOMIC = 4 };
+enum { TGS_GEMM_A_BF16 = 0x10, TGS_GEMM_B_BF16 = 0x20 };
 int tgs_gemm_bf16(int trans_a, int trans_b, int epilogue, int64_t M, int64_t N, int64_t K,
-                  const float *A, int64_t lda, const float *B, int64_t ldb, const float *bias,
+                  const void *A, int64_t lda, const void *B, int64_t ldb, const float *bias,
                   float *C, int64_t ldc, const float *mask, int64_t ldm, int32_t k_splits,
                   tgs_stream_t stream);
 int tgs_sigmoid_bwd(int64_t n, const float *g_out, const float *out, float *g, tgs_stream_t stream);
 /* Sigmoid backward fused with the column sums of its result (no reference
  * counterpart; the decoder's output layer): g = g_out * o * (1 - o) for [M, N]
- * float32 (N a multiple of 4, 16-byte aligned), and db[N] += g.sum(0). */
-int tgs_sigmoid_bwd_colsum(int64_t M, int64_t N, const float *g_out, const float *out, float *g,
-                           float *db, tgs_stream_t stream);
+ * float32 (N a multiple of 4, 16-byte aligned), and db[N] += g.sum(0) of the
+ * float32 values.  g is stored as float32 (g_bf16 0) or as bfloat16 bits
+ * rounded to nearest even (g_bf16 1: what tgs_gemm_bf16 would stage). */
+int tgs_sigmoid_bwd_colsum(int64_t M, int64_t N, const float *g_out, const float *out, void *g,
+                           int g_bf16, float *db, tgs_stream_t stream);
 int tgs_colsum(int64_t M, int64_t N, const float *G, int64_t ldg, float *db, tgs_stream_t stream);
 
 /* Fused decoder inference: all three layers in one kernel, activations kept in

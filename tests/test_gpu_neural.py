@@ -2,7 +2,7 @@
 
 - the BF16 matrix-core GEMM against float64 products of the same
   bfloat16-rounded operands (all transpositions, epilogues, ragged sizes,
-  split-K);
+  split-K), and bfloat16 operands bit-equal to float32 ones;
 - the decoder's forward and backward against the oracle's float64 restatement;
 - end to end on a config-1-style scene: textures refined by the plain method,
   a decoder fitted to them, PSNR of decoded vs explicit renders, memory, and the
@@ -61,12 +61,67 @@ def test_gemm_split_k_accumulates(splits):
     assert _rel(out, ref) < 1e-5
 
 
+@pytest.mark.parametrize("ta", [False, True])
+@pytest.mark.parametrize("tb", [False, True])
+@pytest.mark.parametrize("shape", [(77, 130, 45), (4099, 257, 96)])
+def test_gemm_bfloat16_operands_equal_float_operands(ta, tb, shape):
+    """An operand given as bfloat16 is staged as is; given as float32 it is rounded
+    to nearest even when staged, as torch's conversion rounds: the products agree
+    bit for bit.  Leading dimensions of 45, 130, 257 and 4099 take the element
+    loads, 96 the 16-byte loads; the second shape's tile count (65 x 5) is not a
+    multiple of 8."""
+    g = torch.Generator().manual_seed(11)
+    M, N, K = shape
+    a = torch.randn((K, M) if ta else (M, K), generator=g).to(DEV)
+    b = torch.randn((N, K) if tb else (K, N), generator=g).to(DEV)
+    mask = torch.randn(M, N, generator=g).clamp_min(0).to(DEV)
+    for epi, kw in ((_ops.EPI_NONE, {}), (_ops.EPI_RELU_MASK, dict(mask=mask))):
+        ref = _ops.gemm_bf16(a, b, trans_a=ta, trans_b=tb, epilogue=epi, **kw)
+        for a_, b_ in ((a.bfloat16(), b), (a, b.bfloat16()), (a.bfloat16(), b.bfloat16())):
+            out = _ops.gemm_bf16(a_, b_, trans_a=ta, trans_b=tb, epilogue=epi, **kw)
+            assert torch.equal(out, ref), (epi, a_.dtype, b_.dtype)
+    full = gemm_reference(a.cpu(), b.cpu(), ta, tb)
+    assert _rel(_ops.gemm_bf16(a.bfloat16(), b.bfloat16(), trans_a=ta, trans_b=tb), full) < 1e-5
+
+
+def test_gemm_split_k_bfloat16_operand_and_empty_k():
+    g = torch.Generator().manual_seed(12)
+    X = torch.randn(9000, 128, generator=g).to(DEV)
+    G = torch.randn(9000, 1024, generator=g).to(DEV)
+    ref = gemm_reference(X.cpu(), G.bfloat16().float().cpu(), True, False)
+    out = _ops.gemm_bf16(X, G.bfloat16(), trans_a=True, epilogue=_ops.EPI_ATOMIC,
+                         out=torch.zeros(128, 1024, device=DEV), k_splits=3)
+    assert _rel(out, ref) < 1e-5
+    # K = 0: the product is empty and the epilogue sees the bias alone
+    bias = torch.randn(70, generator=g).to(DEV)
+    out = _ops.gemm_bf16(torch.empty(33, 0, device=DEV), torch.empty(0, 70, device=DEV), epilogue=_ops.EPI_RELU,
+                         bias=bias)
+    assert torch.equal(out, bias.clamp_min(0).expand(33, 70))
+
+
+def test_sigmoid_bwd_colsum_bfloat16_output():
+    g = torch.Generator().manual_seed(13)
+    o = torch.rand(3000, 1024, generator=g).to(DEV)
+    go = torch.randn(3000, 1024, generator=g).to(DEV)
+    g32, db32 = _ops.sigmoid_bwd_colsum(go, o)
+    g16, db16 = _ops.sigmoid_bwd_colsum(go, o, g_dtype=torch.bfloat16)
+    assert g16.dtype == torch.bfloat16 and torch.equal(g16, g32.bfloat16())
+    assert _rel(db16, db32.double()) < 1e-6  # sums of the same float32 values, atomic order aside
+    assert _rel(db32, (go.double() * o * (1 - o)).sum(0)) < 1e-5
+
+
 def test_colsum_and_sigmoid_bwd():
     g = torch.Generator().manual_seed(3)
     G = torch.randn(9000, 130, generator=g)
     db = torch.randn(130, generator=g)
     out = _ops.colsum_(G.to(DEV), db.to(DEV).clone())
     assert _rel(out, db.double() + G.double().sum(0)) < 1e-5
+    # 4-vector path (column count a multiple of 4), rows not a multiple of the block's
+    for rows in (1, 37, 9001):
+        G4 = torch.randn(rows, 128, generator=g)
+        db4 = torch.randn(128, generator=g)
+        out = _ops.colsum_(G4.to(DEV), db4.to(DEV).clone())
+        assert _rel(out, db4.double() + G4.double().sum(0)) < 1e-5, rows
     o = torch.rand(1000, generator=g)
     go = torch.randn(1000, generator=g)
     assert _rel(_ops.sigmoid_bwd(go.to(DEV), o.to(DEV)), go * o * (1 - o)) < 1e-6

@@ -19,8 +19,15 @@
 // tile: the four waves own 32x32 quadrants, each four 16x16 MFMA tiles, over K
 // steps of 32 staged through LDS in the operand layout the MFMA reads (16 bytes
 // per lane: lane l takes row l%16 of A and column l%16 of B, k = 8(l/16)..+8).
-// Split-K (grid.z > 1) accumulates partial tiles into C with atomics (C zeroed
-// by the caller); it is used for weight gradients, whose K is the batch.
+// Split-K accumulates partial tiles into C with atomics (C zeroed by the
+// caller); it is used for weight gradients, whose K is the batch.  Either
+// operand may also be given as bfloat16 bits (staged as is; the rounding then
+// happened where it was written, to the same values).
+// Tile order: consecutive workgroup ids go to the 8 XCDs in turn, so workgroup w
+// runs tile (w % 8) * ceil(tiles / 8) + w / 8 and each XCD walks a contiguous
+// range of tiles.  Tiles are numbered with the N tile fastest when K is not split
+// (the N tiles of one row block share its A rows, read once into that XCD's L2)
+// and with the M tile fastest under split-K (the tiles of one K slice share it).
 #include "common.h"
 
 namespace tgs {
@@ -39,68 +46,100 @@ enum { EPI_NONE = 0, EPI_RELU = 1, EPI_SIGMOID = 2, EPI_RELU_MASK = 3, EPI_ATOMI
 constexpr int kBM = 64, kBN = 64, kBK = 32;
 constexpr int kLdsK = kBK + 8;  // row pitch in bf16 (80 bytes): spreads the 16 rows a wave reads over banks
 
+// Column of the 8-element (16-byte) chunk holding k in row r of a staged tile:
+// chunks are XOR-swizzled by bits 3-4 of the row, so the transposing stores
+// (8 rows 8 apart, one k) fall into four different chunk columns instead of one
+// (rows 8 apart are 160 dwords apart, the same bank).  Reads of 16-byte chunks
+// stay aligned.
+__device__ __forceinline__ int lds_col(int r, int k) { return (((k >> 3) ^ ((r >> 3) & 3)) << 3) | (k & 7); }
+
 // Stages a 64 (rows r) x 32 (k) operand tile into S[r][k] as bfloat16, zero outside
 // the matrix.  KCONTIG: element (r, k) is P[r*ld + k]; else P[k*ld + r].  Each thread
 // moves 8 elements with two 16-byte loads (when vec and in range), along k or along r.
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+// 8 consecutive operand elements at P + off as bfloat16 bits: P holds float32
+// (rounded here) or, with bf, bfloat16 bits.  full: all 8 exist and one 16-byte
+// (bf16) or two 16-byte (float32) loads may be used; else element i is loaded
+// when i < lim and is zero otherwise.
+__device__ __forceinline__ u16x8 load8_bf16(const void *__restrict__ P, bool bf, int64_t off, bool full, int lim) {
+    u16x8 v;
+    if (bf) {
+        const uint16_t *src = (const uint16_t *)P + off;
+        if (full) {
+            v = *(const u16x8 *)src;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = i < lim ? src[i] : (uint16_t)0;
+        }
+    } else {
+        const float *src = (const float *)P + off;
+        float f[8];
+        if (full) {
+            const f32x4 a = *(const f32x4 *)src, b = *(const f32x4 *)(src + 4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                f[i] = a[i];
+                f[4 + i] = b[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) f[i] = i < lim ? src[i] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = bf16_bits(f[i]);
+    }
+    return v;
+}
+
 template <bool KCONTIG>
-__device__ __forceinline__ void stage_tile(uint16_t (*S)[kLdsK], const float *__restrict__ P, int64_t ld, int r0, int R,
-                                           int kb, int k_end, bool vec, int t) {
-    float v[8];
+__device__ __forceinline__ void stage_tile(uint16_t (*S)[kLdsK], const void *__restrict__ P, bool bf, int64_t ld,
+                                           int r0, int R, int kb, int k_end, bool vec, int t) {
     if (KCONTIG) {
         const int rr = t >> 2, kk = (t & 3) * 8;
         const int gr = r0 + rr, gk = kb + kk;
-        const float *src = P + (int64_t)gr * ld + gk;
-        if (vec && gr < R && gk + 8 <= k_end) {
-            const f32x4 a = *(const f32x4 *)src, b = *(const f32x4 *)(src + 4);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                v[i] = a[i];
-                v[4 + i] = b[i];
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] = (gr < R && gk + i < k_end) ? src[i] : 0.f;
-        }
-        const bf16x8 w = {(__bf16)v[0], (__bf16)v[1], (__bf16)v[2], (__bf16)v[3],
-                          (__bf16)v[4], (__bf16)v[5], (__bf16)v[6], (__bf16)v[7]};
-        *(bf16x8 *)&S[rr][kk] = w;
+        const u16x8 v = load8_bf16(P, bf, (int64_t)gr * ld + gk, vec && gr < R && gk + 8 <= k_end,
+                                   gr < R ? k_end - gk : 0);
+        *(u16x8 *)&S[rr][lds_col(rr, kk)] = v;
     } else {
         const int kk = t >> 3, rr = (t & 7) * 8;
         const int gk = kb + kk, gr = r0 + rr;
-        const float *src = P + (int64_t)gk * ld + gr;
-        if (vec && gk < k_end && gr + 8 <= R) {
-            const f32x4 a = *(const f32x4 *)src, b = *(const f32x4 *)(src + 4);
+        const u16x8 v = load8_bf16(P, bf, (int64_t)gk * ld + gr, vec && gk < k_end && gr + 8 <= R,
+                                   gk < k_end ? R - gr : 0);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                v[i] = a[i];
-                v[4 + i] = b[i];
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] = (gk < k_end && gr + i < R) ? src[i] : 0.f;
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) S[rr + i][kk] = bf16_bits(v[i]);
+        for (int i = 0; i < 8; ++i) S[rr + i][lds_col(rr + i, kk)] = v[i];
     }
 }
 
 template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const float *__restrict__ A, int64_t lda,
-                                                        const float *__restrict__ B, int64_t ldb,
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const void *__restrict__ A, int64_t lda,
+                                                        const void *__restrict__ B, int64_t ldb, int a_bf, int b_bf,
                                                         const float *__restrict__ bias, float *__restrict__ C,
                                                         int64_t ldc, const float *__restrict__ mask, int64_t ldm,
-                                                        int k_chunk) {
+                                                        int k_chunk, int tiles_m, int tiles_n, int64_t tiles) {
     __shared__ __align__(16) uint16_t As[kBM][kLdsK];  // [m][k]
     __shared__ __align__(16) uint16_t Bs[kBN][kLdsK];  // [n][k]
+    const int64_t w = blockIdx.x, T = (w % 8) * ((tiles + 7) / 8) + w / 8;
+    if (T >= tiles) return;  // uniform over the workgroup
+    int tm, tn, tz;
+    if (k_chunk >= K) {  // K not split: N tile fastest
+        tn = (int)(T % tiles_n);
+        tm = (int)(T / tiles_n);
+        tz = 0;
+    } else {
+        tm = (int)(T % tiles_m);
+        tn = (int)((T / tiles_m) % tiles_n);
+        tz = (int)(T / ((int64_t)tiles_m * tiles_n));
+    }
     const int t = threadIdx.x, wave = t / kWave, lane = t % kWave;
-    const int m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
-    const int k_begin = blockIdx.z * k_chunk;
+    const int m0 = tm * kBM, n0 = tn * kBN;
+    const int k_begin = tz * k_chunk;
     const int k_end = min(K, k_begin + k_chunk);
     const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;  // wave quadrant
 
     // 16-byte loads when the operand allows them
-    const bool a_vec = (lda % 4) == 0 && ((uintptr_t)A % 16) == 0;
-    const bool b_vec = (ldb % 4) == 0 && ((uintptr_t)B % 16) == 0;
+    const bool a_vec = (lda % (a_bf ? 8 : 4)) == 0 && ((uintptr_t)A % 16) == 0;
+    const bool b_vec = (ldb % (b_bf ? 8 : 4)) == 0 && ((uintptr_t)B % 16) == 0;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -109,16 +148,16 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
 
     for (int kb = k_begin; kb < k_end; kb += kBK) {
         // ---- stage A tile (64 x 32) and B tile (32 x 64) as bf16, zero outside the matrices
-        stage_tile<!TA>(As, A, lda, m0, M, kb, k_end, a_vec, t);
-        stage_tile<TB>(Bs, B, ldb, n0, N, kb, k_end, b_vec, t);
+        stage_tile<!TA>(As, A, a_bf != 0, lda, m0, M, kb, k_end, a_vec, t);
+        stage_tile<TB>(Bs, B, b_bf != 0, ldb, n0, N, kb, k_end, b_vec, t);
         __syncthreads();
         // ---- 2 x 2 MFMA tiles per wave
         bf16x8 af[2], bfr[2];
         const int r = lane % 16, kq = (lane / 16) * 8;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) af[i] = *(const bf16x8 *)&As[wm + i * 16 + r][kq];
+        for (int i = 0; i < 2; ++i) af[i] = *(const bf16x8 *)&As[wm + i * 16 + r][lds_col(wm + i * 16 + r, kq)];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) bfr[j] = *(const bf16x8 *)&Bs[wn + j * 16 + r][kq];
+        for (int j = 0; j < 2; ++j) bfr[j] = *(const bf16x8 *)&Bs[wn + j * 16 + r][lds_col(wn + j * 16 + r, kq)];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -164,9 +203,12 @@ __global__ __launch_bounds__(256) void sigmoid_bwd_kernel(int64_t n, const float
 // layer's bias gradient): g = g_out * o * (1 - o) with 4-vectors, each thread
 // keeps running sums for its 4 columns; one workgroup per (256-column strip,
 // chunk of rows), one atomic per column per workgroup.  Saves re-reading g.
+// BF: g is stored as bfloat16 (nearest even: the values the GEMMs that consume
+// it would stage anyway), the sums are of the float32 values.
+template <bool BF>
 __global__ __launch_bounds__(256) void sigmoid_bwd_colsum_kernel(int M, int N4, const float4 *__restrict__ g_out,
                                                                  const float4 *__restrict__ out,
-                                                                 float4 *__restrict__ g, int rows_per_block,
+                                                                 void *__restrict__ g, int rows_per_block,
                                                                  float *__restrict__ db) {
     __shared__ float4 part[4][64];
     const int t = threadIdx.x, c = t % 64, rq = t / 64;
@@ -182,7 +224,13 @@ __global__ __launch_bounds__(256) void sigmoid_bwd_colsum_kernel(int M, int N4, 
             gg.y = go.y * o.y * (1.f - o.y);
             gg.z = go.z * o.z * (1.f - o.z);
             gg.w = go.w * o.w * (1.f - o.w);
-            g[i] = gg;
+            if (BF) {
+                const uint32_t lo = bf16_bits(gg.x) | ((uint32_t)bf16_bits(gg.y) << 16);
+                const uint32_t hi = bf16_bits(gg.z) | ((uint32_t)bf16_bits(gg.w) << 16);
+                ((uint2 *)g)[i] = uint2{lo, hi};
+            } else {
+                ((float4 *)g)[i] = gg;
+            }
             s.x += gg.x; s.y += gg.y; s.z += gg.z; s.w += gg.w;
         }
     part[rq][c] = s;
@@ -210,6 +258,55 @@ __global__ __launch_bounds__(256) void colsum_kernel(int M, int N, const float *
     part[rq][c] = s;
     __syncthreads();
     if (rq == 0 && n < N) atomic_add(db + n, part[0][c] + part[1][c] + part[2][c] + part[3][c]);
+}
+
+// The same with 4-vectors: 32 column quads by 8 row lanes per workgroup, four
+// rows in flight per lane.
+__global__ __launch_bounds__(256) void colsum4_kernel(int M, int N4, const float4 *__restrict__ G, int64_t ldg4,
+                                                      int rows_per_block, float *__restrict__ db) {
+    __shared__ float4 part[8][32];
+    const int t = threadIdx.x, c = t % 32, rq = t / 32;
+    const int n4 = blockIdx.x * 32 + c;
+    const int r0 = blockIdx.y * rows_per_block, r1 = min(M, r0 + rows_per_block);
+    float4 s = {0.f, 0.f, 0.f, 0.f};
+    if (n4 < N4) {
+        int r = r0 + rq;
+        for (; r + 24 < r1; r += 32) {
+            float4 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = G[(int64_t)(r + 8 * u) * ldg4 + n4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                s.x += x[u].x;
+                s.y += x[u].y;
+                s.z += x[u].z;
+                s.w += x[u].w;
+            }
+        }
+        for (; r < r1; r += 8) {
+            const float4 x = G[(int64_t)r * ldg4 + n4];
+            s.x += x.x;
+            s.y += x.y;
+            s.z += x.z;
+            s.w += x.w;
+        }
+    }
+    part[rq][c] = s;
+    __syncthreads();
+    if (rq == 0 && n4 < N4) {
+        float4 a = part[0][c];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) {
+            a.x += part[q][c].x;
+            a.y += part[q][c].y;
+            a.z += part[q][c].z;
+            a.w += part[q][c].w;
+        }
+        atomic_add(db + 4 * n4, a.x);
+        atomic_add(db + 4 * n4 + 1, a.y);
+        atomic_add(db + 4 * n4 + 2, a.z);
+        atomic_add(db + 4 * n4 + 3, a.w);
+    }
 }
 
 // ------------------------------------------------------------ fused decode
@@ -416,35 +513,44 @@ __global__ __launch_bounds__(256) void decode_kernel(int64_t n, const float *__r
 }
 
 template <bool TA, bool TB, int EPI>
-static void launch_gemm(int M, int N, int K, const float *A, int64_t lda, const float *B, int64_t ldb,
-                        const float *bias, float *C, int64_t ldc, const float *mask, int64_t ldm, int splits,
-                        hipStream_t st) {
-    const int chunk = ((K + splits - 1) / splits + kBK - 1) / kBK * kBK;
-    const int nz = (K + chunk - 1) / chunk;
-    dim3 grid((unsigned)ceil_div(M, kBM), (unsigned)ceil_div(N, kBN), (unsigned)max(nz, 1));
-    gemm_bf16_kernel<TA, TB, EPI><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, bias, C, ldc, mask, ldm, chunk);
+static void launch_gemm(int M, int N, int K, const void *A, int64_t lda, const void *B, int64_t ldb, int a_bf,
+                        int b_bf, const float *bias, float *C, int64_t ldc, const float *mask, int64_t ldm,
+                        int splits, hipStream_t st) {
+    const int chunk = K > 0 ? ((K + splits - 1) / splits + kBK - 1) / kBK * kBK : kBK;
+    const int nz = K > 0 ? (K + chunk - 1) / chunk : 1;
+    const int tm = (int)ceil_div(M, kBM), tn = (int)ceil_div(N, kBN);
+    const int64_t tiles = (int64_t)tm * tn * nz;
+    const int64_t grid = (tiles + 7) / 8 * 8;
+    gemm_bf16_kernel<TA, TB, EPI><<<(unsigned)grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, a_bf, b_bf, bias, C, ldc,
+                                                                   mask, ldm, chunk, tm, tn, tiles);
 }
 
 }  // namespace tgs
 
 using namespace tgs;
 
-extern "C" int tgs_gemm_bf16(int trans_a, int trans_b, int epilogue, int64_t M, int64_t N, int64_t K, const float *A,
-                             int64_t lda, const float *B, int64_t ldb, const float *bias, float *C, int64_t ldc,
+extern "C" int tgs_gemm_bf16(int trans_a, int trans_b, int epilogue, int64_t M, int64_t N, int64_t K, const void *A,
+                             int64_t lda, const void *B, int64_t ldb, const float *bias, float *C, int64_t ldc,
                              const float *mask, int64_t ldm, int32_t k_splits, tgs_stream_t stream) {
     TGS_CHECK_ARG(M >= 0 && N >= 0 && K >= 0 && M < (int64_t(1) << 31) && N < (int64_t(1) << 31) &&
                       K < (int64_t(1) << 31),
                   "gemm_bf16: bad sizes M=%lld N=%lld K=%lld", (long long)M, (long long)N, (long long)K);
+    const int a_bf = (epilogue & TGS_GEMM_A_BF16) != 0, b_bf = (epilogue & TGS_GEMM_B_BF16) != 0;
+    epilogue &= ~(TGS_GEMM_A_BF16 | TGS_GEMM_B_BF16);
     TGS_CHECK_ARG(epilogue >= EPI_NONE && epilogue <= EPI_ATOMIC, "gemm_bf16: bad epilogue %d", epilogue);
-    TGS_CHECK_ARG(A && B && C, "gemm_bf16: null operand");
+    TGS_CHECK_ARG((K == 0 || (A && B)) && (M == 0 || N == 0 || C), "gemm_bf16: null operand");
     TGS_CHECK_ARG(epilogue != EPI_RELU_MASK || mask, "gemm_bf16: the ReLU-mask epilogue needs a mask");
     TGS_CHECK_ARG(k_splits >= 1 && (k_splits == 1 || epilogue == EPI_ATOMIC),
                   "gemm_bf16: split-K needs the atomic epilogue");
     TGS_CHECK_ARG(epilogue != EPI_ATOMIC || !bias, "gemm_bf16: the atomic epilogue takes no bias");
     if (M == 0 || N == 0) return TGS_OK;
+    const int64_t tiles_mn = ceil_div(M, kBM) * ceil_div(N, kBN);  // < 2^50
+    TGS_CHECK_ARG(tiles_mn < (int64_t(1) << 31) && tiles_mn * (K / kBK + 1) < (int64_t(1) << 31),
+                  "gemm_bf16: too many tiles (M=%lld N=%lld K=%lld)", (long long)M, (long long)N, (long long)K);
     hipStream_t st = (hipStream_t)stream;
     const int m = (int)M, n = (int)N, k = (int)K;
-#define TGS_G(TA, TB, E) launch_gemm<TA, TB, E>(m, n, k, A, lda, B, ldb, bias, C, ldc, mask, ldm, k_splits, st)
+#define TGS_G(TA, TB, E) \
+    launch_gemm<TA, TB, E>(m, n, k, A, lda, B, ldb, a_bf, b_bf, bias, C, ldc, mask, ldm, k_splits, st)
 #define TGS_GE(TA, TB)                                  \
     switch (epilogue) {                                 \
         case EPI_NONE: TGS_G(TA, TB, EPI_NONE); break;  \
@@ -497,8 +603,8 @@ extern "C" int tgs_sigmoid_bwd(int64_t n, const float *g_out, const float *out, 
     return TGS_OK;
 }
 
-extern "C" int tgs_sigmoid_bwd_colsum(int64_t M, int64_t N, const float *g_out, const float *out, float *g,
-                                      float *db, tgs_stream_t stream) {
+extern "C" int tgs_sigmoid_bwd_colsum(int64_t M, int64_t N, const float *g_out, const float *out, void *g,
+                                      int g_bf16, float *db, tgs_stream_t stream) {
     TGS_CHECK_ARG(M >= 0 && N >= 0 && M < (int64_t(1) << 31) && N % 4 == 0 && (M == 0 || (g_out && out && g && db)),
                   "sigmoid_bwd_colsum: bad arguments (N must be a multiple of 4)");
     TGS_CHECK_ARG((((uintptr_t)g_out | (uintptr_t)out | (uintptr_t)g) % 16) == 0,
@@ -507,8 +613,12 @@ extern "C" int tgs_sigmoid_bwd_colsum(int64_t M, int64_t N, const float *g_out, 
     const int rows = 4096;
     const int64_t N4 = N / 4;
     dim3 grid((unsigned)ceil_div(N4, 64), (unsigned)ceil_div(M, rows));
-    sigmoid_bwd_colsum_kernel<<<grid, 256, 0, (hipStream_t)stream>>>((int)M, (int)N4, (const float4 *)g_out,
-                                                                     (const float4 *)out, (float4 *)g, rows, db);
+    if (g_bf16)
+        sigmoid_bwd_colsum_kernel<true><<<grid, 256, 0, (hipStream_t)stream>>>((int)M, (int)N4, (const float4 *)g_out,
+                                                                               (const float4 *)out, g, rows, db);
+    else
+        sigmoid_bwd_colsum_kernel<false><<<grid, 256, 0, (hipStream_t)stream>>>(
+            (int)M, (int)N4, (const float4 *)g_out, (const float4 *)out, g, rows, db);
     TGS_CHECK_LAUNCH("sigmoid_bwd_colsum");
     return TGS_OK;
 }
@@ -517,9 +627,16 @@ extern "C" int tgs_colsum(int64_t M, int64_t N, const float *G, int64_t ldg, flo
     TGS_CHECK_ARG(M >= 0 && N >= 0 && M < (int64_t(1) << 31) && N < (int64_t(1) << 31) && (M == 0 || (G && db)),
                   "colsum: bad arguments");
     if (M == 0 || N == 0) return TGS_OK;
-    const int rows = 4096;
-    dim3 grid((unsigned)ceil_div(N, 64), (unsigned)ceil_div(M, rows));
-    colsum_kernel<<<grid, 256, 0, (hipStream_t)stream>>>((int)M, (int)N, G, ldg, rows, db);
+    if (N % 4 == 0 && ldg % 4 == 0 && ((uintptr_t)G % 16) == 0) {
+        const int rows = 512;
+        dim3 grid((unsigned)ceil_div(N / 4, 32), (unsigned)ceil_div(M, rows));
+        colsum4_kernel<<<grid, 256, 0, (hipStream_t)stream>>>((int)M, (int)(N / 4), (const float4 *)G, ldg / 4, rows,
+                                                              db);
+    } else {
+        const int rows = 4096;
+        dim3 grid((unsigned)ceil_div(N, 64), (unsigned)ceil_div(M, rows));
+        colsum_kernel<<<grid, 256, 0, (hipStream_t)stream>>>((int)M, (int)N, G, ldg, rows, db);
+    }
     TGS_CHECK_LAUNCH("colsum");
     return TGS_OK;
 }

@@ -817,6 +817,7 @@ def zero_fill_rows_(t: Tensor, mask: Tensor, select: int) -> Tensor:
 
 # ------------------------------------------------- neural-texture decoder
 EPI_NONE, EPI_RELU, EPI_SIGMOID, EPI_RELU_MASK, EPI_ATOMIC = 0, 1, 2, 3, 4
+GEMM_A_BF16, GEMM_B_BF16 = 0x10, 0x20
 
 
 def _f32_2d(t: Tensor, name: str) -> None:
@@ -829,11 +830,17 @@ def gemm_bf16(a: Tensor, b: Tensor, trans_a: bool = False, trans_b: bool = False
               bias: Optional[Tensor] = None, mask: Optional[Tensor] = None, out: Optional[Tensor] = None,
               k_splits: int = 1) -> Tensor:
     """C = epilogue(op(a) @ op(b) + bias) on the matrix cores, bfloat16 operands and
-    float32 accumulation (op = transpose when trans_*).  With ``EPI_ATOMIC`` the
-    product is added into ``out`` (which must be given), optionally with K split
-    over ``k_splits`` workgroup slices."""
-    _f32_2d(a, "a")
-    _f32_2d(b, "b")
+    float32 accumulation (op = transpose when trans_*).  ``a`` and ``b`` are float32
+    (rounded to nearest even when staged) or bfloat16 (staged as is).  With
+    ``EPI_ATOMIC`` the product is added into ``out`` (which must be given),
+    optionally with K split over ``k_splits`` workgroup slices."""
+    flags = 0
+    for t, name, flag in ((a, "a", GEMM_A_BF16), (b, "b", GEMM_B_BF16)):
+        _check_input(t, name)
+        if t.dim() != 2 or t.dtype not in (torch.float32, torch.bfloat16) or t.stride(1) != 1:
+            raise RuntimeError(f"{name} must be a 2-D float32 or bfloat16 tensor with unit column stride")
+        if t.dtype == torch.bfloat16:
+            flags |= flag
     M, K = (a.shape[1], a.shape[0]) if trans_a else (a.shape[0], a.shape[1])
     Kb, N = (b.shape[1], b.shape[0]) if trans_b else (b.shape[0], b.shape[1])
     if K != Kb:
@@ -852,7 +859,7 @@ def gemm_bf16(a: Tensor, b: Tensor, trans_a: bool = False, trans_b: bool = False
         _f32_2d(out, "out")
         assert out.shape == (M, N)
     with torch.cuda.device(a.device):
-        check(lib().tgs_gemm_bf16(int(trans_a), int(trans_b), int(epilogue), M, N, K, _p(a), a.stride(0), _p(b),
+        check(lib().tgs_gemm_bf16(int(trans_a), int(trans_b), int(epilogue) | flags, M, N, K, _p(a), a.stride(0), _p(b),
                                   b.stride(0), _p(bias), _p(out), out.stride(0), _p(mask),
                                   mask.stride(0) if mask is not None else 0, int(k_splits), _stream(a)),
               "gemm_bf16")
@@ -868,18 +875,22 @@ def sigmoid_bwd(g_out: Tensor, out: Tensor) -> Tensor:
     return g
 
 
-def sigmoid_bwd_colsum(g_out: Tensor, out: Tensor):
+def sigmoid_bwd_colsum(g_out: Tensor, out: Tensor, g_dtype=torch.float32):
     """(g_out * out * (1 - out), its column sums) for 2-D float32 [M, N] (N % 4 == 0),
-    in one pass."""
+    in one pass; g stored as ``g_dtype`` (float32 or bfloat16, nearest even), the
+    sums taken over the float32 values."""
     _f32_2d(out, "out")
     _check_input(g_out, "g_out")
     if g_out.shape != out.shape or g_out.dtype != torch.float32 or out.shape[1] % 4:
         raise RuntimeError("sigmoid_bwd_colsum: g_out and out must be float32 [M, N] with N % 4 == 0")
-    g = torch.empty_like(out)
+    if g_dtype not in (torch.float32, torch.bfloat16):
+        raise RuntimeError("sigmoid_bwd_colsum: g_dtype must be float32 or bfloat16")
+    g = torch.empty(out.shape, dtype=g_dtype, device=out.device)
     db = torch.zeros(out.shape[1], dtype=torch.float32, device=out.device)
     with torch.cuda.device(out.device):
-        check(lib().tgs_sigmoid_bwd_colsum(out.shape[0], out.shape[1], _p(g_out), _p(out), _p(g), _p(db),
-                                           _stream(out)), "sigmoid_bwd_colsum")
+        check(lib().tgs_sigmoid_bwd_colsum(out.shape[0], out.shape[1], _p(g_out), _p(out), _p(g),
+                                           int(g_dtype == torch.bfloat16), _p(db), _stream(out)),
+              "sigmoid_bwd_colsum")
     return g, db
 
 

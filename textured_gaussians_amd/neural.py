@@ -71,7 +71,9 @@ class _DecodeFn(torch.autograd.Function):
         else:
             z, W1, h1, W2, h2, W3, out = ctx.saved_tensors
         ks = _k_splits(z.shape[0])
-        g3, db3 = E.sigmoid_bwd_colsum(g_out.contiguous(), out)
+        # the output-layer cotangent is stored as bfloat16: both GEMMs that read it
+        # stage it as bfloat16 anyway, so their results are unchanged and it moves half the bytes
+        g3, db3 = E.sigmoid_bwd_colsum(g_out.contiguous(), out, g_dtype=torch.bfloat16)
         dW3 = E.gemm_bf16(h2, g3, trans_a=True, epilogue=E.EPI_ATOMIC, out=torch.zeros_like(W3), k_splits=ks)
         g2 = E.gemm_bf16(g3, W3, trans_b=True, epilogue=E.EPI_RELU_MASK, mask=h2)
         dW2 = E.gemm_bf16(h1, g2, trans_a=True, epilogue=E.EPI_ATOMIC, out=torch.zeros_like(W2), k_splits=ks)
