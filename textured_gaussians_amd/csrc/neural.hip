@@ -53,59 +53,82 @@ constexpr int kLdsK = kBK + 8;  // row pitch in bf16 (80 bytes): spreads the 16 
 // stay aligned.
 __device__ __forceinline__ int lds_col(int r, int k) { return (((k >> 3) ^ ((r >> 3) & 3)) << 3) | (k & 7); }
 
-// Stages a 64 (rows r) x 32 (k) operand tile into S[r][k] as bfloat16, zero outside
-// the matrix.  KCONTIG: element (r, k) is P[r*ld + k]; else P[k*ld + r].  Each thread
-// moves 8 elements with two 16-byte loads (when vec and in range), along k or along r.
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
-// 8 consecutive operand elements at P + off as bfloat16 bits: P holds float32
-// (rounded here) or, with bf, bfloat16 bits.  full: all 8 exist and one 16-byte
-// (bf16) or two 16-byte (float32) loads may be used; else element i is loaded
-// when i < lim and is zero otherwise.
-__device__ __forceinline__ u16x8 load8_bf16(const void *__restrict__ P, bool bf, int64_t off, bool full, int lim) {
-    u16x8 v;
+// 8 consecutive operand elements as loaded: 8 float32, or 8 bfloat16 bits in `a`.
+// They stay in registers while the previous K step's products run.
+struct Raw8 {
+    f32x4 a, b;
+};
+
+// Loads the 8 elements at P + off: P holds float32 or, with bf, bfloat16 bits.
+// full: all 8 exist and 16-byte loads may be used; else element i is loaded when
+// i < lim and is zero otherwise.
+__device__ __forceinline__ Raw8 load8(const void *__restrict__ P, bool bf, int64_t off, bool full, int lim) {
+    Raw8 r;
     if (bf) {
         const uint16_t *src = (const uint16_t *)P + off;
+        u16x8 v;
         if (full) {
             v = *(const u16x8 *)src;
         } else {
 #pragma unroll
             for (int i = 0; i < 8; ++i) v[i] = i < lim ? src[i] : (uint16_t)0;
         }
+        r.a = __builtin_bit_cast(f32x4, v);
+        r.b = f32x4{0, 0, 0, 0};
     } else {
         const float *src = (const float *)P + off;
-        float f[8];
         if (full) {
-            const f32x4 a = *(const f32x4 *)src, b = *(const f32x4 *)(src + 4);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                f[i] = a[i];
-                f[4 + i] = b[i];
-            }
+            r.a = *(const f32x4 *)src;
+            r.b = *(const f32x4 *)(src + 4);
         } else {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) f[i] = i < lim ? src[i] : 0.f;
+            for (int i = 0; i < 4; ++i) {
+                r.a[i] = i < lim ? src[i] : 0.f;
+                r.b[i] = 4 + i < lim ? src[4 + i] : 0.f;
+            }
         }
+    }
+    return r;
+}
+
+// the 8 elements as bfloat16 bits (float32 rounded to nearest even)
+__device__ __forceinline__ u16x8 to_bf16x8(const Raw8 &r, bool bf) {
+    if (bf) return __builtin_bit_cast(u16x8, r.a);
+    u16x8 v;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = bf16_bits(f[i]);
+    for (int i = 0; i < 4; ++i) {
+        v[i] = bf16_bits(r.a[i]);
+        v[4 + i] = bf16_bits(r.b[i]);
     }
     return v;
 }
 
+// A 64 (rows r) x 32 (k) operand tile, staged into S[r][k] as bfloat16, zero
+// outside the matrix.  KCONTIG: element (r, k) is P[r*ld + k]; else P[k*ld + r].
+// Each thread moves 8 elements, along k or along r: stage_load issues its loads,
+// stage_store converts and writes them to LDS.
 template <bool KCONTIG>
-__device__ __forceinline__ void stage_tile(uint16_t (*S)[kLdsK], const void *__restrict__ P, bool bf, int64_t ld,
-                                           int r0, int R, int kb, int k_end, bool vec, int t) {
+__device__ __forceinline__ Raw8 stage_load(const void *__restrict__ P, bool bf, int64_t ld, int r0, int R, int kb,
+                                           int k_end, bool vec, int t) {
+    if (KCONTIG) {
+        const int gr = r0 + (t >> 2), gk = kb + (t & 3) * 8;
+        return load8(P, bf, (int64_t)gr * ld + gk, vec && gr < R && gk + 8 <= k_end, gr < R ? k_end - gk : 0);
+    } else {
+        const int gk = kb + (t >> 3), gr = r0 + (t & 7) * 8;
+        return load8(P, bf, (int64_t)gk * ld + gr, vec && gk < k_end && gr + 8 <= R, gk < k_end ? R - gr : 0);
+    }
+}
+
+template <bool KCONTIG>
+__device__ __forceinline__ void stage_store(uint16_t (*S)[kLdsK], const Raw8 &raw, bool bf, int t) {
+    const u16x8 v = to_bf16x8(raw, bf);
     if (KCONTIG) {
         const int rr = t >> 2, kk = (t & 3) * 8;
-        const int gr = r0 + rr, gk = kb + kk;
-        const u16x8 v = load8_bf16(P, bf, (int64_t)gr * ld + gk, vec && gr < R && gk + 8 <= k_end,
-                                   gr < R ? k_end - gk : 0);
         *(u16x8 *)&S[rr][lds_col(rr, kk)] = v;
     } else {
         const int kk = t >> 3, rr = (t & 7) * 8;
-        const int gk = kb + kk, gr = r0 + rr;
-        const u16x8 v = load8_bf16(P, bf, (int64_t)gk * ld + gr, vec && gk < k_end && gr + 8 <= R,
-                                   gk < k_end ? R - gr : 0);
 #pragma unroll
         for (int i = 0; i < 8; ++i) S[rr + i][lds_col(rr + i, kk)] = v[i];
     }
@@ -146,11 +169,22 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 
+    // A tile (64 x 32) and B tile (32 x 64) of the first K step in registers
+    Raw8 ra, rb;
+    if (k_begin < k_end) {
+        ra = stage_load<!TA>(A, a_bf != 0, lda, m0, M, k_begin, k_end, a_vec, t);
+        rb = stage_load<TB>(B, b_bf != 0, ldb, n0, N, k_begin, k_end, b_vec, t);
+    }
     for (int kb = k_begin; kb < k_end; kb += kBK) {
-        // ---- stage A tile (64 x 32) and B tile (32 x 64) as bf16, zero outside the matrices
-        stage_tile<!TA>(As, A, a_bf != 0, lda, m0, M, kb, k_end, a_vec, t);
-        stage_tile<TB>(Bs, B, b_bf != 0, ldb, n0, N, kb, k_end, b_vec, t);
+        // ---- stage this step's tiles as bf16, then issue the next step's loads, which
+        // are in flight during this step's products
+        stage_store<!TA>(As, ra, a_bf != 0, t);
+        stage_store<TB>(Bs, rb, b_bf != 0, t);
         __syncthreads();
+        if (kb + kBK < k_end) {
+            ra = stage_load<!TA>(A, a_bf != 0, lda, m0, M, kb + kBK, k_end, a_vec, t);
+            rb = stage_load<TB>(B, b_bf != 0, ldb, n0, N, kb + kBK, k_end, b_vec, t);
+        }
         // ---- 2 x 2 MFMA tiles per wave
         bf16x8 af[2], bfr[2];
         const int r = lane % 16, kq = (lane / 16) * 8;

@@ -66,22 +66,25 @@ class _DecodeFn(torch.autograd.Function):
         E = _ops
         if ctx.recompute:
             z, W1, b1, W2, b2, W3, out = ctx.saved_tensors
-            h1 = E.gemm_bf16(z, W1, epilogue=E.EPI_RELU, bias=b1)
-            h2 = E.gemm_bf16(h1, W2, epilogue=E.EPI_RELU, bias=b2)
         else:
             z, W1, h1, W2, h2, W3, out = ctx.saved_tensors
+        # the weights as bfloat16, the values the GEMMs would stage: half the bytes per K step
+        W1b, W2b, W3b = W1.bfloat16(), W2.bfloat16(), W3.bfloat16()
+        if ctx.recompute:
+            h1 = E.gemm_bf16(z, W1b, epilogue=E.EPI_RELU, bias=b1)
+            h2 = E.gemm_bf16(h1, W2b, epilogue=E.EPI_RELU, bias=b2)
         ks = _k_splits(z.shape[0])
         # the output-layer cotangent is stored as bfloat16: both GEMMs that read it
         # stage it as bfloat16 anyway, so their results are unchanged and it moves half the bytes
         g3, db3 = E.sigmoid_bwd_colsum(g_out.contiguous(), out, g_dtype=torch.bfloat16)
         dW3 = E.gemm_bf16(h2, g3, trans_a=True, epilogue=E.EPI_ATOMIC, out=torch.zeros_like(W3), k_splits=ks)
-        g2 = E.gemm_bf16(g3, W3, trans_b=True, epilogue=E.EPI_RELU_MASK, mask=h2)
+        g2 = E.gemm_bf16(g3, W3b, trans_b=True, epilogue=E.EPI_RELU_MASK, mask=h2)
         dW2 = E.gemm_bf16(h1, g2, trans_a=True, epilogue=E.EPI_ATOMIC, out=torch.zeros_like(W2), k_splits=ks)
         db2 = E.colsum_(g2, torch.zeros(W2.shape[1], device=g2.device))
-        g1 = E.gemm_bf16(g2, W2, trans_b=True, epilogue=E.EPI_RELU_MASK, mask=h1)
+        g1 = E.gemm_bf16(g2, W2b, trans_b=True, epilogue=E.EPI_RELU_MASK, mask=h1)
         dW1 = E.gemm_bf16(z, g1, trans_a=True, epilogue=E.EPI_ATOMIC, out=torch.zeros_like(W1), k_splits=ks)
         db1 = E.colsum_(g1, torch.zeros(W1.shape[1], device=g1.device))
-        dz = E.gemm_bf16(g1, W1, trans_b=True) if ctx.needs_input_grad[0] else None
+        dz = E.gemm_bf16(g1, W1b, trans_b=True) if ctx.needs_input_grad[0] else None
         return dz, dW1, db1, dW2, db2, dW3, db3
 
 
