@@ -197,3 +197,18 @@ def test_masks_render_background():
                                                           s.backgrounds, masks, s.width, s.height, 16, offs,
                                                           flat, 0.0, 1.0)
     assert torch.all(out[0][0, :16, :16] == 1.0) and torch.all(out[1][0, :16, :16] == 0.0)
+
+
+def test_singular_view_matrix_raises():
+    """The camera-to-world matrices come from ``torch.linalg.inv_ex`` (the routine
+    of ``torch.inverse``) with the singularity check read after the intersection
+    count: a singular view matrix still raises, as ``torch.inverse`` does in the
+    reference.  The projection reads only the top three rows, so it runs normally."""
+    sc = _scene(n=50)
+    L = _leaves(sc, DEV)
+    vm = sc.viewmats.to(DEV).clone()
+    vm[0, 3, :] = 0.0
+    with pytest.raises(RuntimeError, match="singular"):
+        tg.rasterization_textured_gaussians(
+            L["means"], L["quats"], L["scales"], L["opacities"], L["colors"], L["textures"], vm, sc.Ks.to(DEV),
+            sc.width, sc.height, sh_degree=sc.sh_degree, backgrounds=sc.backgrounds.to(DEV))

@@ -50,11 +50,40 @@ def _common_checks(means, quats, scales, opacities, colors, viewmats, Ks, render
     return N, C
 
 
+def _inverse_deferred(viewmats: Tensor):
+    """``torch.inverse(viewmats)`` by the same routine, without the host wait of its
+    singularity check: the check's status comes back by an asynchronous copy and
+    ``_check_inverse`` reads it after the intersection count read, where the host
+    waits for the device anyway (so the check costs no second wait).  The
+    reference inverts twice per render ([RD]:1573, and for the normals), each
+    time with that wait."""
+    if not viewmats.is_cuda:
+        return torch.inverse(viewmats), None
+    inv, info = torch.linalg.inv_ex(viewmats)
+    info_host = torch.empty(info.shape, dtype=info.dtype, pin_memory=True)
+    info_host.copy_(info, non_blocking=True)
+    done = torch.cuda.Event()
+    done.record()
+    return inv, (info_host, done)
+
+
+def _check_inverse(pending) -> None:
+    if pending is None:
+        return
+    info_host, done = pending
+    done.synchronize()
+    if bool((info_host != 0).any()):
+        raise torch.linalg.LinAlgError(
+            "linalg.inv: a view matrix is singular, the inversion could not be completed")
+
+
 def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, height, near_plane, far_plane,
              radius_clip, eps2d, sh_degree, packed, tile_size, sparse_grad, norm_rot_grad, render_mode,
              proj_grad):
-    """Projection, tile intersection, per-view colours ([RD]:1507-1594)."""
+    """Projection, tile intersection, per-view colours ([RD]:1507-1594).  Also
+    returns the camera-to-world matrices (the inverse view matrices)."""
     N, C = means.shape[0], viewmats.shape[0]
+    camtoworlds, pending = _inverse_deferred(viewmats)
     proj = fully_fused_projection_2dgs(
         means, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane, radius_clip, packed,
         sparse_grad, norm_rot_grad, proj_grad=proj_grad)
@@ -72,12 +101,12 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
         means2d, radii, depths, tile_size, tile_width, tile_height, packed=packed, n_cameras=C,
         camera_ids=camera_ids, gaussian_ids=gaussian_ids)
     isect_offsets = isect_offset_encode(isect_ids, C, tile_width, tile_height)
+    _check_inverse(pending)
     if not (colors.dim() == 3 and sh_degree is None):
         colors = colors[gaussian_ids] if packed else colors.expand(C, *([-1] * colors.dim()))
     elif packed:
         colors = colors[camera_ids, gaussian_ids, :]
     if sh_degree is not None:
-        camtoworlds = torch.inverse(viewmats)
         if packed:
             dirs = means[gaussian_ids, :] - camtoworlds[camera_ids, :3, 3]
         else:
@@ -95,20 +124,20 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
         "isect_ids": isect_ids, "flatten_ids": flatten_ids, "isect_offsets": isect_offsets, "width": width,
         "height": height, "tile_size": tile_size, "n_cameras": C, "gradient_2dgs": densify,
     }
-    return colors, opacities, densify, meta
+    return colors, opacities, densify, meta, camtoworlds
 
 
-def _finish(render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, viewmats, Ks):
+def _finish(render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, camtoworlds, Ks):
     render_normals_from_depth = None
     if render_mode in ["ED", "RGB+ED"]:
         render_colors = torch.cat([render_colors[..., :-1],
                                    render_colors[..., -1:] / render_alphas.clamp(min=1e-10)], dim=-1)
     if render_mode in ["RGB+ED", "RGB+D"]:
         depth_for_normal = render_colors[..., -1:] if depth_mode == "expected" else render_median
-        render_normals_from_depth = depth_to_normal(depth_for_normal, torch.linalg.inv(viewmats), Ks).squeeze(0)
+        render_normals_from_depth = depth_to_normal(depth_for_normal, camtoworlds, Ks).squeeze(0)
     # camera-frame normals -> world frame, n_w[i] = sum_j R[i, j] n[j]: a broadcast product and a
     # sum over j (two elementwise passes; a [H*W,3] x [3,3] GEMM runs far below bandwidth)
-    R = torch.linalg.inv(viewmats)[..., :3, :3]
+    R = camtoworlds[..., :3, :3]
     render_normals = (render_normals[..., None, :] * R[:, None, None]).sum(-1)
     return render_colors, render_normals, render_normals_from_depth
 
@@ -147,7 +176,7 @@ def rasterization_textured_gaussians(
         if grad_mode == "exact":
             proj_grad = "exact"
             filtering = "bilinear_exact"
-    colors, opac_c, densify, meta = _prepare(
+    colors, opac_c, densify, meta, camtoworlds = _prepare(
         means, quats, scales, opacities, colors, viewmats, Ks, width, height, near_plane, far_plane, radius_clip,
         eps2d, sh_degree, packed, tile_size, sparse_grad, norm_rot_grad, render_mode, proj_grad)
     (render_colors, render_alphas, render_normals, render_distort, render_median, gs_contrib_sum,
@@ -158,7 +187,7 @@ def rasterization_textured_gaussians(
         gs_contrib_threshold=gs_contrib_threshold, filtering=filtering, g_weight=g_weight,
         gaussian_ids=meta["gaussian_ids"], texture_adam=texture_adam)
     render_colors, render_normals, render_normals_from_depth = _finish(
-        render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, viewmats, Ks)
+        render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, camtoworlds, Ks)
     meta["render_distort"] = render_distort
     return (render_colors, render_alphas, render_normals, render_normals_from_depth, render_distort,
             render_median, gs_contrib_sum, gs_contrib_count, meta)
@@ -180,7 +209,7 @@ def rasterization_2dgs(
     if grad_mode == "exact":
         proj_grad = raster_grad = "exact"
     N, C = _common_checks(means, quats, scales, opacities, colors, viewmats, Ks, render_mode, distloss, sh_degree)
-    colors, opac_c, densify, meta = _prepare(
+    colors, opac_c, densify, meta, camtoworlds = _prepare(
         means, quats, scales, opacities, colors, viewmats, Ks, width, height, near_plane, far_plane, radius_clip,
         eps2d, sh_degree, packed, tile_size, sparse_grad, norm_rot_grad, render_mode, proj_grad)
     (render_colors, render_alphas, render_normals, render_distort, render_median, gs_contrib_sum,
@@ -190,7 +219,7 @@ def rasterization_2dgs(
         absgrad=absgrad, distloss=distloss, gs_contrib_threshold=gs_contrib_threshold, grad_mode=raster_grad,
         gaussian_ids=meta["gaussian_ids"], n_gaussians=N)
     render_colors, render_normals, render_normals_from_depth = _finish(
-        render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, viewmats, Ks)
+        render_colors, render_alphas, render_normals, render_median, render_mode, depth_mode, camtoworlds, Ks)
     meta["render_distort"] = render_distort
     return (render_colors, render_alphas, render_normals, render_normals_from_depth, render_distort,
             render_median, gs_contrib_sum, gs_contrib_count, meta)
