@@ -95,13 +95,9 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
         opacities = opacities.repeat(C, 1)
         camera_ids, gaussian_ids = None, None
     densify = torch.zeros_like(means2d, dtype=means.dtype, requires_grad=True, device=means.device)
-    tile_width = math.ceil(width / float(tile_size))
-    tile_height = math.ceil(height / float(tile_size))
-    tiles_per_gauss, isect_ids, flatten_ids = isect_tiles(
-        means2d, radii, depths, tile_size, tile_width, tile_height, packed=packed, n_cameras=C,
-        camera_ids=camera_ids, gaussian_ids=gaussian_ids)
-    isect_offsets = isect_offset_encode(isect_ids, C, tile_width, tile_height)
-    _check_inverse(pending)
+    # per-view colours before the tile intersection: they need only the projection, and the
+    # intersection count read makes the host wait for the device, so work queued before it
+    # overlaps that wait (the reference computes them after the sort, [RD]:1561-1594)
     if not (colors.dim() == 3 and sh_degree is None):
         colors = colors[gaussian_ids] if packed else colors.expand(C, *([-1] * colors.dim()))
     elif packed:
@@ -117,6 +113,13 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
         colors = torch.cat((colors, depths[..., None]), dim=-1)
     elif render_mode in ["D", "ED"]:
         colors = depths[..., None]
+    tile_width = math.ceil(width / float(tile_size))
+    tile_height = math.ceil(height / float(tile_size))
+    tiles_per_gauss, isect_ids, flatten_ids = isect_tiles(
+        means2d, radii, depths, tile_size, tile_width, tile_height, packed=packed, n_cameras=C,
+        camera_ids=camera_ids, gaussian_ids=gaussian_ids)
+    isect_offsets = isect_offset_encode(isect_ids, C, tile_width, tile_height)
+    _check_inverse(pending)
     meta = {
         "camera_ids": camera_ids, "gaussian_ids": gaussian_ids, "radii": radii, "means2d": means2d,
         "depths": depths, "ray_transforms": ray_transforms, "opacities": opacities, "normals": normals,
