@@ -38,6 +38,30 @@ def test_sh_fwd_bwd(degree, dtype):
     assert torch.all(col[~masks.to(DEV)] == 0)
 
 
+@pytest.mark.parametrize("degree", [1, 3])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_sh_bwd_exact_band_count_and_unaligned_buffers(degree, offset):
+    """K = (degree + 1)^2 exactly (K * 3 a multiple of 4: the float32 kernel moves the
+    coefficient blocks as 16-byte vectors), a ragged last wave, and coefficient
+    buffers one float off 16-byte alignment (element-wise path)."""
+    g = torch.Generator().manual_seed(10 + degree)
+    n, K = 3001, (degree + 1) ** 2
+    dirs = torch.randn(n, 3, generator=g, dtype=torch.float64)
+    coeffs = torch.randn(n, K, 3, generator=g, dtype=torch.float64)
+    masks = torch.rand(n, generator=g) > 0.2
+    vcol = torch.randn(n, 3, generator=g, dtype=torch.float64)
+    buf = torch.zeros(n * K * 3 + offset, device=DEV)
+    cg = buf[offset:].view(n, K, 3)
+    cg.copy_(coeffs)
+    vco, vdi = _ops.compute_sh_bwd(K, degree, dirs.to(DEV, torch.float32), cg, masks.to(DEV),
+                                   vcol.to(DEV, torch.float32), True)
+    d = dirs.clone().requires_grad_(True)
+    c = coeffs.clone().requires_grad_(True)
+    (sh_ref(degree, d, c, masks) * vcol).sum().backward()
+    assert rel_err(vco, c.grad) < 1e-5
+    assert rel_err(vdi, d.grad) < 1e-4
+
+
 def test_selective_adam_matches_oracle_and_torch():
     g = torch.Generator().manual_seed(0)
     N, M = 1000, 48  # M % 4 == 0: the vectorised path

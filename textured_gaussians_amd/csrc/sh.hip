@@ -28,73 +28,43 @@ namespace tgs {
 #define C4f 0.47308734787878004
 #define C4g 0.6258357354491761
 
-// Y[k] and, if G != nullptr, dY[k]/d(x,y,z) in G[k][0..2], at the unit vector (x,y,z).
-template <typename T> __device__ __forceinline__ void sh_eval(int nb, T x, T y, T z, T *Y, T (*G)[3]) {
-    Y[0] = T(C0);
-    if (G) { G[0][0] = 0; G[0][1] = 0; G[0][2] = 0; }
+// Calls f(k, Y_k, dY_k/dx, dY_k/dy, dY_k/dz) for k < nb at the unit vector (x,y,z),
+// with k a literal at each call, so callers index registers, not arrays.
+template <typename T, typename F> __device__ __forceinline__ void sh_each(int nb, T x, T y, T z, F &&f) {
+    f(0, T(C0), T(0), T(0), T(0));
     if (nb <= 1) return;
-    Y[1] = -T(C1) * y; Y[2] = T(C1) * z; Y[3] = -T(C1) * x;
-    if (G) {
-        G[1][0] = 0; G[1][1] = -T(C1); G[1][2] = 0;
-        G[2][0] = 0; G[2][1] = 0; G[2][2] = T(C1);
-        G[3][0] = -T(C1); G[3][1] = 0; G[3][2] = 0;
-    }
+    f(1, -T(C1) * y, T(0), -T(C1), T(0));
+    f(2, T(C1) * z, T(0), T(0), T(C1));
+    f(3, -T(C1) * x, -T(C1), T(0), T(0));
     if (nb <= 4) return;
     const T x2 = x * x, y2 = y * y, z2 = z * z;
-    Y[4] = T(C2a) * x * y;
-    Y[5] = -T(C2a) * y * z;
-    Y[6] = T(C2b) * (T(3) * z2 - T(1));
-    Y[7] = -T(C2a) * x * z;
-    Y[8] = T(C2c) * (x2 - y2);
-    if (G) {
-        G[4][0] = T(C2a) * y; G[4][1] = T(C2a) * x; G[4][2] = 0;
-        G[5][0] = 0; G[5][1] = -T(C2a) * z; G[5][2] = -T(C2a) * y;
-        G[6][0] = 0; G[6][1] = 0; G[6][2] = T(6 * C2b) * z;
-        G[7][0] = -T(C2a) * z; G[7][1] = 0; G[7][2] = -T(C2a) * x;
-        G[8][0] = T(2 * C2c) * x; G[8][1] = -T(2 * C2c) * y; G[8][2] = 0;
-    }
+    f(4, T(C2a) * x * y, T(C2a) * y, T(C2a) * x, T(0));
+    f(5, -T(C2a) * y * z, T(0), -T(C2a) * z, -T(C2a) * y);
+    f(6, T(C2b) * (T(3) * z2 - T(1)), T(0), T(0), T(6 * C2b) * z);
+    f(7, -T(C2a) * x * z, -T(C2a) * z, T(0), -T(C2a) * x);
+    f(8, T(C2c) * (x2 - y2), T(2 * C2c) * x, -T(2 * C2c) * y, T(0));
     if (nb <= 9) return;
-    Y[9] = -T(C3a) * y * (T(3) * x2 - y2);
-    Y[10] = T(C3b) * x * y * z;
-    Y[11] = -T(C3c) * y * (T(5) * z2 - T(1));
-    Y[12] = T(C3d) * z * (T(5) * z2 - T(3));
-    Y[13] = -T(C3c) * x * (T(5) * z2 - T(1));
-    Y[14] = T(C3e) * z * (x2 - y2);
-    Y[15] = -T(C3a) * x * (x2 - T(3) * y2);
-    if (G) {
-        G[9][0] = -T(6 * C3a) * x * y; G[9][1] = -T(3 * C3a) * (x2 - y2); G[9][2] = 0;
-        G[10][0] = T(C3b) * y * z; G[10][1] = T(C3b) * x * z; G[10][2] = T(C3b) * x * y;
-        G[11][0] = 0; G[11][1] = -T(C3c) * (T(5) * z2 - T(1)); G[11][2] = -T(10 * C3c) * y * z;
-        G[12][0] = 0; G[12][1] = 0; G[12][2] = T(C3d) * (T(15) * z2 - T(3));
-        G[13][0] = -T(C3c) * (T(5) * z2 - T(1)); G[13][1] = 0; G[13][2] = -T(10 * C3c) * x * z;
-        G[14][0] = T(2 * C3e) * x * z; G[14][1] = -T(2 * C3e) * y * z; G[14][2] = T(C3e) * (x2 - y2);
-        G[15][0] = -T(3 * C3a) * (x2 - y2); G[15][1] = T(6 * C3a) * x * y; G[15][2] = 0;
-    }
+    f(9, -T(C3a) * y * (T(3) * x2 - y2), -T(6 * C3a) * x * y, -T(3 * C3a) * (x2 - y2), T(0));
+    f(10, T(C3b) * x * y * z, T(C3b) * y * z, T(C3b) * x * z, T(C3b) * x * y);
+    f(11, -T(C3c) * y * (T(5) * z2 - T(1)), T(0), -T(C3c) * (T(5) * z2 - T(1)), -T(10 * C3c) * y * z);
+    f(12, T(C3d) * z * (T(5) * z2 - T(3)), T(0), T(0), T(C3d) * (T(15) * z2 - T(3)));
+    f(13, -T(C3c) * x * (T(5) * z2 - T(1)), -T(C3c) * (T(5) * z2 - T(1)), T(0), -T(10 * C3c) * x * z);
+    f(14, T(C3e) * z * (x2 - y2), T(2 * C3e) * x * z, -T(2 * C3e) * y * z, T(C3e) * (x2 - y2));
+    f(15, -T(C3a) * x * (x2 - T(3) * y2), -T(3 * C3a) * (x2 - y2), T(6 * C3a) * x * y, T(0));
     if (nb <= 16) return;
     const T q = T(7) * z2 - T(1), p = T(7) * z2 - T(3);
-    Y[16] = T(C4a) * x * y * (x2 - y2);
-    Y[17] = -T(C4b) * y * z * (T(3) * x2 - y2);
-    Y[18] = T(C4c) * x * y * q;
-    Y[19] = -T(C4d) * y * z * p;
-    Y[20] = T(C4e) * (T(35) * z2 * z2 - T(30) * z2 + T(3));
-    Y[21] = -T(C4d) * x * z * p;
-    Y[22] = T(C4f) * (x2 - y2) * q;
-    Y[23] = -T(C4b) * x * z * (x2 - T(3) * y2);
-    Y[24] = T(C4g) * (x2 * x2 - T(6) * x2 * y2 + y2 * y2);
-    if (G) {
-        G[16][0] = T(C4a) * (T(3) * x2 * y - y2 * y); G[16][1] = T(C4a) * (x2 * x - T(3) * x * y2); G[16][2] = 0;
-        G[17][0] = -T(6 * C4b) * x * y * z; G[17][1] = -T(3 * C4b) * (x2 - y2) * z;
-        G[17][2] = -T(C4b) * (T(3) * x2 * y - y2 * y);
-        G[18][0] = T(C4c) * y * q; G[18][1] = T(C4c) * x * q; G[18][2] = T(14 * C4c) * x * y * z;
-        G[19][0] = 0; G[19][1] = -T(C4d) * z * p; G[19][2] = -T(C4d) * y * (T(21) * z2 - T(3));
-        G[20][0] = 0; G[20][1] = 0; G[20][2] = T(C4e) * (T(140) * z2 * z - T(60) * z);
-        G[21][0] = -T(C4d) * z * p; G[21][1] = 0; G[21][2] = -T(C4d) * x * (T(21) * z2 - T(3));
-        G[22][0] = T(2 * C4f) * x * q; G[22][1] = -T(2 * C4f) * y * q; G[22][2] = T(14 * C4f) * z * (x2 - y2);
-        G[23][0] = -T(3 * C4b) * (x2 - y2) * z; G[23][1] = T(6 * C4b) * x * y * z;
-        G[23][2] = -T(C4b) * (x2 * x - T(3) * x * y2);
-        G[24][0] = T(4 * C4g) * (x2 * x - T(3) * x * y2); G[24][1] = T(4 * C4g) * (y2 * y - T(3) * x2 * y);
-        G[24][2] = 0;
-    }
+    f(16, T(C4a) * x * y * (x2 - y2), T(C4a) * (T(3) * x2 * y - y2 * y), T(C4a) * (x2 * x - T(3) * x * y2), T(0));
+    f(17, -T(C4b) * y * z * (T(3) * x2 - y2),
+      -T(6 * C4b) * x * y * z, -T(3 * C4b) * (x2 - y2) * z, -T(C4b) * (T(3) * x2 * y - y2 * y));
+    f(18, T(C4c) * x * y * q, T(C4c) * y * q, T(C4c) * x * q, T(14 * C4c) * x * y * z);
+    f(19, -T(C4d) * y * z * p, T(0), -T(C4d) * z * p, -T(C4d) * y * (T(21) * z2 - T(3)));
+    f(20, T(C4e) * (T(35) * z2 * z2 - T(30) * z2 + T(3)), T(0), T(0), T(C4e) * (T(140) * z2 * z - T(60) * z));
+    f(21, -T(C4d) * x * z * p, -T(C4d) * z * p, T(0), -T(C4d) * x * (T(21) * z2 - T(3)));
+    f(22, T(C4f) * (x2 - y2) * q, T(2 * C4f) * x * q, -T(2 * C4f) * y * q, T(14 * C4f) * z * (x2 - y2));
+    f(23, -T(C4b) * x * z * (x2 - T(3) * y2),
+      -T(3 * C4b) * (x2 - y2) * z, T(6 * C4b) * x * y * z, -T(C4b) * (x2 * x - T(3) * x * y2));
+    f(24, T(C4g) * (x2 * x2 - T(6) * x2 * y2 + y2 * y2),
+      T(4 * C4g) * (x2 * x - T(3) * x * y2), T(4 * C4g) * (y2 * y - T(3) * x2 * y), T(0));
 }
 
 template <typename T>
@@ -114,15 +84,13 @@ __global__ __launch_bounds__(256) void sh_fwd_kernel(int64_t n, int K, int degre
         const T inv = tgs_rsqrt<T>(dx * dx + dy * dy + dz * dz);
         x = dx * inv; y = dy * inv; z = dz * inv;
     }
-    T Y[25];
-    sh_eval<T>(nb, x, y, z, Y, nullptr);
     const T *c = coeffs + i * (int64_t)K * 3;
     T r = 0, g = 0, b = 0;
-    for (int k = 0; k < nb; ++k) {
-        r += Y[k] * c[k * 3];
-        g += Y[k] * c[k * 3 + 1];
-        b += Y[k] * c[k * 3 + 2];
-    }
+    sh_each<T>(nb, x, y, z, [&](int k, T v, T, T, T) {
+        r += v * c[k * 3];
+        g += v * c[k * 3 + 1];
+        b += v * c[k * 3 + 2];
+    });
     colors[i * 3] = r; colors[i * 3 + 1] = g; colors[i * 3 + 2] = b;
 }
 
@@ -145,26 +113,23 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int64_t n, int K, int degre
         inv = tgs_rsqrt<T>(dx * dx + dy * dy + dz * dz);
         x = dx * inv; y = dy * inv; z = dz * inv;
     }
-    T Y[25], G[25][3];
     const bool want_dir = v_dirs != nullptr && nb > 1;
-    sh_eval<T>(nb, x, y, z, Y, want_dir ? G : nullptr);
     const T vr = v_colors[i * 3], vg = v_colors[i * 3 + 1], vb = v_colors[i * 3 + 2];
     const T *c = coeffs + i * (int64_t)K * 3;
     T gu = 0, gv = 0, gw = 0;
-    for (int k = 0; k < K; ++k) {
-        if (k < nb) {
-            vc[k * 3] = Y[k] * vr;
-            vc[k * 3 + 1] = Y[k] * vg;
-            vc[k * 3 + 2] = Y[k] * vb;
-            if (want_dir) {
-                const T s = c[k * 3] * vr + c[k * 3 + 1] * vg + c[k * 3 + 2] * vb;
-                gu += s * G[k][0];
-                gv += s * G[k][1];
-                gw += s * G[k][2];
-            }
-        } else {
-            vc[k * 3] = 0; vc[k * 3 + 1] = 0; vc[k * 3 + 2] = 0;
+    sh_each<T>(nb, x, y, z, [&](int k, T v, T gx, T gy, T gz) {  // nb <= K (checked by the entry point)
+        vc[k * 3] = v * vr;
+        vc[k * 3 + 1] = v * vg;
+        vc[k * 3 + 2] = v * vb;
+        if (want_dir) {
+            const T s = c[k * 3] * vr + c[k * 3 + 1] * vg + c[k * 3 + 2] * vb;
+            gu += s * gx;
+            gv += s * gy;
+            gw += s * gz;
         }
+    });
+    for (int k = nb; k < K; ++k) {
+        vc[k * 3] = 0; vc[k * 3 + 1] = 0; vc[k * 3 + 2] = 0;
     }
     if (want_dir) {
         // through u = d / |d|: dL/dd = (g - (g.u) u) / |d|
@@ -179,10 +144,13 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int64_t n, int K, int degre
 // wave's coefficient block (64 Gaussians x K * 3 values, contiguous in memory)
 // loaded and its v_coeffs block stored through LDS, so global accesses are
 // contiguous 256-byte runs instead of 64 lines 192 bytes apart.  Rows are padded
-// to an odd stride (49) against bank conflicts.
+// to an odd stride (49) against bank conflicts.  The basis size NB is a template
+// argument, so the basis and its gradient stay in registers (indexed by unrolled
+// loops) instead of private memory.
 constexpr int kShStage = 49;
 
-__global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, int degree, const float *__restrict__ dirs,
+template <int NB>
+__global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, const float *__restrict__ dirs,
                                                             const float *__restrict__ coeffs,
                                                             const uint8_t *__restrict__ masks,
                                                             const float *__restrict__ v_colors,
@@ -195,15 +163,25 @@ __global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, in
     const int nw = (int)min((int64_t)64, n - base);
     const int64_t tot = (int64_t)nw * K3;
     float *st = sm[wv];
+    // the block moves as 16-byte vectors when K3 is a multiple of 4 (the four values of a
+    // vector then share a Gaussian) and both buffers are 16-byte aligned (then so is every
+    // block: base % 64 == 0), the rest element by element; element e is (e / K3, e % K3)
+    const bool vec = K3 % 4 == 0 && (((uintptr_t)coeffs | (uintptr_t)v_coeffs) & 15) == 0;
+    const int64_t tot4 = vec ? tot / 4 : 0;
     {
-        int t = lane / K3, k = lane - (lane / K3) * K3;  // element lane + 64 j of the block = (t, k)
-        for (int64_t e = lane; e < tot; e += 64) {
+        const float4 *src = (const float4 *)(coeffs + base * K3);
+        for (int64_t e4 = lane; e4 < tot4; e4 += 64) {
+            const float4 v = src[e4];
+            const int t = (int)(e4 * 4 / K3), k = (int)(e4 * 4 - (int64_t)t * K3);
+            float *d = st + t * kShStage + k;
+            d[0] = v.x;
+            d[1] = v.y;
+            d[2] = v.z;
+            d[3] = v.w;
+        }
+        for (int64_t e = tot4 * 4 + lane; e < tot; e += 64) {
+            const int t = (int)(e / K3), k = (int)(e - (int64_t)t * K3);
             st[t * kShStage + k] = coeffs[base * K3 + e];
-            k += 64;
-            while (k >= K3) {
-                k -= K3;
-                ++t;
-            }
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -214,32 +192,35 @@ __global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, in
         if (masks && !masks[i]) {
             for (int k = 0; k < K3; ++k) c[k] = 0.f;
         } else {
-            const int nb = (degree + 1) * (degree + 1);
             float x = 0, y = 0, z = 0, inv = 0;
-            if (nb > 1) {
+            if (NB > 1) {
                 const float dx = dirs[i * 3], dy = dirs[i * 3 + 1], dz = dirs[i * 3 + 2];
                 inv = tgs_rsqrt<float>(dx * dx + dy * dy + dz * dz);
                 x = dx * inv; y = dy * inv; z = dz * inv;
             }
-            float Y[25], G[25][3];
-            const bool want_dir = v_dirs != nullptr && nb > 1;
-            sh_eval<float>(nb, x, y, z, Y, want_dir ? G : nullptr);
+            const bool want_dir = v_dirs != nullptr && NB > 1;
             const float vr = v_colors[i * 3], vg = v_colors[i * 3 + 1], vb = v_colors[i * 3 + 2];
             float gu = 0, gv = 0, gw = 0;
-            for (int k = 0; k < K; ++k) {
-                if (k < nb) {
-                    if (want_dir) {
-                        const float s = c[k * 3] * vr + c[k * 3 + 1] * vg + c[k * 3 + 2] * vb;
-                        gu += s * G[k][0];
-                        gv += s * G[k][1];
-                        gw += s * G[k][2];
-                    }
-                    c[k * 3] = Y[k] * vr;
-                    c[k * 3 + 1] = Y[k] * vg;
-                    c[k * 3 + 2] = Y[k] * vb;
-                } else {
-                    c[k * 3] = 0.f; c[k * 3 + 1] = 0.f; c[k * 3 + 2] = 0.f;
-                }
+            // NB <= K (checked by the entry point); c[k] is read before it is overwritten
+            if (want_dir) {
+                sh_each<float>(NB, x, y, z, [&](int k, float v, float gx, float gy, float gz) {
+                    const float s = c[k * 3] * vr + c[k * 3 + 1] * vg + c[k * 3 + 2] * vb;
+                    gu += s * gx;
+                    gv += s * gy;
+                    gw += s * gz;
+                    c[k * 3] = v * vr;
+                    c[k * 3 + 1] = v * vg;
+                    c[k * 3 + 2] = v * vb;
+                });
+            } else {
+                sh_each<float>(NB, x, y, z, [&](int k, float v, float, float, float) {
+                    c[k * 3] = v * vr;
+                    c[k * 3 + 1] = v * vg;
+                    c[k * 3 + 2] = v * vb;
+                });
+            }
+            for (int k = NB; k < K; ++k) {
+                c[k * 3] = 0.f; c[k * 3 + 1] = 0.f; c[k * 3 + 2] = 0.f;
             }
             if (want_dir) {
                 const float gd = gu * x + gv * y + gw * z;
@@ -252,14 +233,15 @@ __global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, in
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
     {
-        int t = lane / K3, k = lane - (lane / K3) * K3;
-        for (int64_t e = lane; e < tot; e += 64) {
+        float4 *dst = (float4 *)(v_coeffs + base * K3);
+        for (int64_t e4 = lane; e4 < tot4; e4 += 64) {
+            const int t = (int)(e4 * 4 / K3), k = (int)(e4 * 4 - (int64_t)t * K3);
+            const float *q = st + t * kShStage + k;
+            dst[e4] = float4{q[0], q[1], q[2], q[3]};
+        }
+        for (int64_t e = tot4 * 4 + lane; e < tot; e += 64) {
+            const int t = (int)(e / K3), k = (int)(e - (int64_t)t * K3);
             v_coeffs[base * K3 + e] = st[t * kShStage + k];
-            k += 64;
-            while (k >= K3) {
-                k -= K3;
-                ++t;
-            }
         }
     }
 }
@@ -297,9 +279,19 @@ extern "C" int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const
     if (n <= 0) return TGS_OK;
     const unsigned nb = (unsigned)ceil_div(n, 256);
     hipStream_t st = (hipStream_t)stream;
-    if (dtype == TGS_F32 && K * 3 <= kShStage - 1)
-        sh_bwd_staged_kernel<<<nb, 256, 0, st>>>(n, K, degree, (const float *)dirs, (const float *)coeffs, masks,
-                                                 (const float *)v_colors, (float *)v_coeffs, (float *)v_dirs);
+    if (dtype == TGS_F32 && K * 3 <= kShStage - 1) {
+#define TGS_SHB(NBV)                                                                                              \
+    sh_bwd_staged_kernel<NBV><<<nb, 256, 0, st>>>(n, K, (const float *)dirs, (const float *)coeffs, masks,        \
+                                                  (const float *)v_colors, (float *)v_coeffs, (float *)v_dirs)
+        switch (degree) {
+            case 0: TGS_SHB(1); break;
+            case 1: TGS_SHB(4); break;
+            case 2: TGS_SHB(9); break;
+            case 3: TGS_SHB(16); break;
+            default: TGS_SHB(25); break;
+        }
+#undef TGS_SHB
+    }
     else if (dtype == TGS_F32)
         sh_bwd_kernel<float><<<nb, 256, 0, st>>>(n, K, degree, (const float *)dirs, (const float *)coeffs, masks,
                                                  (const float *)v_colors, (float *)v_coeffs, (float *)v_dirs);
