@@ -87,6 +87,31 @@ def test_packed_render_and_gradients_match_grid(textured, grad_mode, sparse):
         assert rel_err(g1[k], g0[k]) < 1e-10, k
 
 
+@pytest.mark.parametrize("kind", ["rgb", "per_view"])
+def test_packed_plain_and_per_view_colours_match_grid(kind):
+    """Colours without SH, [N, 3] or per view [C, N, 3]: the packed layout gathers
+    them per entry (and scatters their gradients back), the grid layout expands them."""
+    s = _scene(torch.float64)
+    C, N = s.viewmats.shape[0], s.means.shape[0]
+    g = torch.Generator().manual_seed(5)
+    cols = torch.rand((N, 3) if kind == "rgb" else (C, N, 3), generator=g, dtype=torch.float64).to(DEV)
+    res = {}
+    for packed in (False, True):
+        L = {k: getattr(s, k).detach().clone().requires_grad_(True)
+             for k in ["means", "quats", "scales", "opacities", "textures"]}
+        L["colors"] = cols.clone().requires_grad_(True)
+        out = tg.rasterization_textured_gaussians(L["means"], L["quats"], L["scales"], L["opacities"], L["colors"],
+                                                  L["textures"], s.viewmats, s.Ks, s.width, s.height,
+                                                  backgrounds=s.backgrounds, packed=packed)
+        gw = torch.Generator(device=DEV).manual_seed(1)
+        (out[0] * torch.rand(out[0].shape, device=DEV, dtype=out[0].dtype, generator=gw)).sum().backward()
+        res[packed] = (out[0].detach(), {k: v.grad for k, v in L.items()})
+    (o0, g0), (o1, g1) = res[False], res[True]
+    assert rel_err(o1, o0) < 1e-12
+    for k in g0:
+        assert rel_err(g1[k], g0[k]) < 1e-10, k
+
+
 def test_packed_textures_sample_each_entrys_gaussian():
     """A Gaussian visible only in the second camera: its packed entry index differs from its
     Gaussian id, and its texture (not the texture at its entry index) must colour it."""

@@ -89,7 +89,10 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
         sparse_grad, norm_rot_grad, proj_grad=proj_grad)
     if packed:
         camera_ids, gaussian_ids, radii, means2d, depths, ray_transforms, normals = proj
-        opacities = opacities[gaussian_ids]
+        # per-entry gathers by index_select (the reference indexes, opacities[gaussian_ids]):
+        # the same values, and a backward that scatter-adds with atomics instead of
+        # sorting the indices first (torch's advanced-indexing backward)
+        opacities = opacities.index_select(0, gaussian_ids)
     else:
         radii, means2d, depths, ray_transforms, normals = proj
         opacities = opacities.repeat(C, 1)
@@ -99,12 +102,12 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
     # intersection count read makes the host wait for the device, so work queued before it
     # overlaps that wait (the reference computes them after the sort, [RD]:1561-1594)
     if not (colors.dim() == 3 and sh_degree is None):
-        colors = colors[gaussian_ids] if packed else colors.expand(C, *([-1] * colors.dim()))
+        colors = colors.index_select(0, gaussian_ids) if packed else colors.expand(C, *([-1] * colors.dim()))
     elif packed:
-        colors = colors[camera_ids, gaussian_ids, :]
+        colors = colors.reshape(C * N, -1).index_select(0, camera_ids * N + gaussian_ids)
     if sh_degree is not None:
         if packed:
-            dirs = means[gaussian_ids, :] - camtoworlds[camera_ids, :3, 3]
+            dirs = means.index_select(0, gaussian_ids) - camtoworlds[camera_ids, :3, 3]
         else:
             dirs = means[None, :, :] - camtoworlds[:, None, :3, 3]
         colors = spherical_harmonics(sh_degree, dirs, colors, masks=radii > 0)
