@@ -188,6 +188,19 @@ int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const void *dirs
                const void *coeffs, const uint8_t *masks, const void *v_colors,
                void *v_coeffs, void *v_dirs, tgs_stream_t stream);
 
+/* Several cameras over one coefficient set (no reference counterpart: the
+ * reference expands coeffs to [C, N, K, 3] and calls compute_sh_fwd/bwd on
+ * C*N rows, rendering.py:1573-1583).  dirs [C,n,3], masks uint8[C,n] or
+ * NULL, colors / v_colors [C,n,3]; coeffs and v_coeffs [n,K,3]; v_coeffs is
+ * written with the sum over the cameras; v_dirs [C,n,3] (may be NULL) is
+ * written in full (zero where masked).  float32 with K*3 <= 48 only
+ * (TGS_ERR_UNSUPPORTED otherwise).                                          */
+int tgs_sh_fwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, const void *dirs,
+                    const void *coeffs, const uint8_t *masks, void *colors, tgs_stream_t stream);
+int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, const void *dirs,
+                    const void *coeffs, const uint8_t *masks, const void *v_colors, void *v_coeffs,
+                    void *v_dirs, tgs_stream_t stream);
+
 /* --------------------------------------------------------------- rasterizer
  * Replaces `rasterize_to_pixels_fwd_textured_gaussians` (ext.cpp:86-88,
  * bindings.h:509-538) when textures != NULL, and `rasterize_to_pixels_fwd_2dgs`

@@ -62,6 +62,45 @@ def test_sh_bwd_exact_band_count_and_unaligned_buffers(degree, offset):
     assert rel_err(vdi, d.grad) < 1e-4
 
 
+@pytest.mark.parametrize("degree,extra", [(0, 0), (1, 1), (2, 0), (3, 0), (3, 1)])
+def test_sh_views_kernel_equals_the_expanded_form(degree, extra):
+    """One coefficient set seen from C views: the views kernel against
+    ``spherical_harmonics`` of the coefficients expanded to every view (what the
+    reference computes) and against the oracle.  Colours are bit-equal (same
+    per-(view, Gaussian) arithmetic); gradients agree to float32 rounding (the
+    views are summed in the kernel instead of by the expand's backward)."""
+    from textured_gaussians_amd.cuda import spherical_harmonics, spherical_harmonics_views
+    g = torch.Generator().manual_seed(20 + degree)
+    C, n, K = 3, 2049, (degree + 1) ** 2 + extra
+    dirs = torch.randn(C, n, 3, generator=g, dtype=torch.float64)
+    coeffs = torch.randn(n, K, 3, generator=g, dtype=torch.float64)
+    masks = torch.rand(C, n, generator=g) > 0.2
+    vcol = torch.randn(C, n, 3, generator=g, dtype=torch.float64)
+    outs = {}
+    for fused in (True, False):
+        d = dirs.to(DEV, torch.float32).requires_grad_(True)
+        c = coeffs.to(DEV, torch.float32).requires_grad_(True)
+        if fused:
+            col = spherical_harmonics_views(degree, d, c, masks.to(DEV))
+        else:
+            col = spherical_harmonics(degree, d, c.expand(C, -1, -1, -1), masks.to(DEV))
+        (col * vcol.to(DEV, torch.float32)).sum().backward()
+        outs[fused] = (col.detach(), c.grad, d.grad)
+    assert torch.equal(outs[True][0], outs[False][0])
+    assert rel_err(outs[True][1], outs[False][1]) < 1e-6
+    assert rel_err(outs[True][2], outs[False][2]) < 1e-6
+    dd = dirs.clone().requires_grad_(True)
+    cc = coeffs.clone().requires_grad_(True)
+    ref = sh_ref(degree, dd, cc.expand(C, -1, -1, -1), masks)
+    (ref * vcol).sum().backward()
+    assert rel_err(outs[True][0], ref) < 1e-5
+    assert rel_err(outs[True][1], cc.grad) < 1e-5
+    if degree > 0:
+        assert rel_err(outs[True][2], dd.grad) < 1e-4
+    else:
+        assert torch.all(outs[True][2] == 0)
+
+
 def test_selective_adam_matches_oracle_and_torch():
     g = torch.Generator().manual_seed(0)
     N, M = 1000, 48  # M % 4 == 0: the vectorised path

@@ -149,6 +149,42 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int64_t n, int K, int degre
 // loops) instead of private memory.
 constexpr int kShStage = 49;
 
+// A wave's coefficient block (the tot = nw * K3 values of nw consecutive Gaussians,
+// contiguous at src) into st[t * kShStage + k] and back: element e is (e / K3, e % K3).
+// With vec (K3 a multiple of 4, the block 16-byte aligned) it moves as 16-byte vectors,
+// whose four values then share a Gaussian.
+__device__ __forceinline__ void sh_stage_in(float *st, const float *__restrict__ src, int64_t tot, int K3, bool vec,
+                                            int lane) {
+    const int64_t tot4 = vec ? tot / 4 : 0;
+    for (int64_t e4 = lane; e4 < tot4; e4 += 64) {
+        const float4 v = ((const float4 *)src)[e4];
+        const int t = (int)(e4 * 4 / K3), k = (int)(e4 * 4 - (int64_t)t * K3);
+        float *d = st + t * kShStage + k;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+    }
+    for (int64_t e = tot4 * 4 + lane; e < tot; e += 64) {
+        const int t = (int)(e / K3), k = (int)(e - (int64_t)t * K3);
+        st[t * kShStage + k] = src[e];
+    }
+}
+
+__device__ __forceinline__ void sh_stage_out(float *__restrict__ dst, const float *st, int64_t tot, int K3, bool vec,
+                                             int lane) {
+    const int64_t tot4 = vec ? tot / 4 : 0;
+    for (int64_t e4 = lane; e4 < tot4; e4 += 64) {
+        const int t = (int)(e4 * 4 / K3), k = (int)(e4 * 4 - (int64_t)t * K3);
+        const float *q = st + t * kShStage + k;
+        ((float4 *)dst)[e4] = float4{q[0], q[1], q[2], q[3]};
+    }
+    for (int64_t e = tot4 * 4 + lane; e < tot; e += 64) {
+        const int t = (int)(e / K3), k = (int)(e - (int64_t)t * K3);
+        dst[e] = st[t * kShStage + k];
+    }
+}
+
 template <int NB>
 __global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, const float *__restrict__ dirs,
                                                             const float *__restrict__ coeffs,
@@ -163,27 +199,9 @@ __global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, co
     const int nw = (int)min((int64_t)64, n - base);
     const int64_t tot = (int64_t)nw * K3;
     float *st = sm[wv];
-    // the block moves as 16-byte vectors when K3 is a multiple of 4 (the four values of a
-    // vector then share a Gaussian) and both buffers are 16-byte aligned (then so is every
-    // block: base % 64 == 0), the rest element by element; element e is (e / K3, e % K3)
+    // 16-byte moves when both buffers are 16-byte aligned (then so is every block: base % 64 == 0)
     const bool vec = K3 % 4 == 0 && (((uintptr_t)coeffs | (uintptr_t)v_coeffs) & 15) == 0;
-    const int64_t tot4 = vec ? tot / 4 : 0;
-    {
-        const float4 *src = (const float4 *)(coeffs + base * K3);
-        for (int64_t e4 = lane; e4 < tot4; e4 += 64) {
-            const float4 v = src[e4];
-            const int t = (int)(e4 * 4 / K3), k = (int)(e4 * 4 - (int64_t)t * K3);
-            float *d = st + t * kShStage + k;
-            d[0] = v.x;
-            d[1] = v.y;
-            d[2] = v.z;
-            d[3] = v.w;
-        }
-        for (int64_t e = tot4 * 4 + lane; e < tot; e += 64) {
-            const int t = (int)(e / K3), k = (int)(e - (int64_t)t * K3);
-            st[t * kShStage + k] = coeffs[base * K3 + e];
-        }
-    }
+    sh_stage_in(st, coeffs + base * K3, tot, K3, vec, lane);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     const int64_t i = base + lane;
@@ -232,18 +250,122 @@ __global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, co
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    {
-        float4 *dst = (float4 *)(v_coeffs + base * K3);
-        for (int64_t e4 = lane; e4 < tot4; e4 += 64) {
-            const int t = (int)(e4 * 4 / K3), k = (int)(e4 * 4 - (int64_t)t * K3);
-            const float *q = st + t * kShStage + k;
-            dst[e4] = float4{q[0], q[1], q[2], q[3]};
+    sh_stage_out(v_coeffs + base * K3, st, tot, K3, vec, lane);
+}
+
+
+// Several cameras over one set of coefficients, float32, K * 3 <= 48: the grid layout
+// with C > 1 views.  The reference expands the coefficients to [C, N, K, 3] (a copy)
+// and evaluates C * N rows; here a lane owns one Gaussian, stages its coefficients
+// once (as the staged backward does) and loops over the cameras.  dirs, masks,
+// colors, v_colors and v_dirs are [C, N, ...]; coeffs and v_coeffs [N, K, 3].
+// Per (camera, Gaussian) the arithmetic is that of sh_fwd_kernel / the staged
+// backward; v_coeffs sums the cameras in order 0..C-1.
+template <int NB>
+__global__ __launch_bounds__(256) void sh_fwd_cams_kernel(int64_t C, int64_t n, int K, const float *__restrict__ dirs,
+                                                          const float *__restrict__ coeffs,
+                                                          const uint8_t *__restrict__ masks,
+                                                          float *__restrict__ colors) {
+    __shared__ float sm[4][64 * kShStage];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x + wv * 64;
+    if (base >= n) return;  // uniform over the wave
+    const int K3 = K * 3;
+    const int nw = (int)min((int64_t)64, n - base);
+    float *st = sm[wv];
+    sh_stage_in(st, coeffs + base * K3, (int64_t)nw * K3, K3, K3 % 4 == 0 && ((uintptr_t)coeffs & 15) == 0, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const int64_t i = base + lane;
+    if (i >= n) return;
+    const float *c = st + lane * kShStage;
+    for (int64_t cam = 0; cam < C; ++cam) {
+        const int64_t ci = cam * n + i;
+        float r = 0, g = 0, b = 0;
+        if (!masks || masks[ci]) {
+            float x = 0, y = 0, z = 0;
+            if (NB > 1) {
+                const float dx = dirs[ci * 3], dy = dirs[ci * 3 + 1], dz = dirs[ci * 3 + 2];
+                const float inv = tgs_rsqrt<float>(dx * dx + dy * dy + dz * dz);
+                x = dx * inv; y = dy * inv; z = dz * inv;
+            }
+            sh_each<float>(NB, x, y, z, [&](int k, float v, float, float, float) {
+                r += v * c[k * 3];
+                g += v * c[k * 3 + 1];
+                b += v * c[k * 3 + 2];
+            });
         }
-        for (int64_t e = tot4 * 4 + lane; e < tot; e += 64) {
-            const int t = (int)(e / K3), k = (int)(e - (int64_t)t * K3);
-            v_coeffs[base * K3 + e] = st[t * kShStage + k];
-        }
+        colors[ci * 3] = r; colors[ci * 3 + 1] = g; colors[ci * 3 + 2] = b;
     }
+}
+
+// v_dirs (optional) is written in full, zero where masked: one lane per (camera, Gaussian).
+template <int NB>
+__global__ __launch_bounds__(256) void sh_bwd_cams_kernel(int64_t C, int64_t n, int K, const float *__restrict__ dirs,
+                                                          const float *__restrict__ coeffs,
+                                                          const uint8_t *__restrict__ masks,
+                                                          const float *__restrict__ v_colors,
+                                                          float *__restrict__ v_coeffs, float *__restrict__ v_dirs) {
+    __shared__ float sm[4][64 * kShStage];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x + wv * 64;
+    if (base >= n) return;  // uniform over the wave
+    const int K3 = K * 3;
+    const int nw = (int)min((int64_t)64, n - base);
+    const int64_t tot = (int64_t)nw * K3;
+    float *st = sm[wv];
+    const bool vec = K3 % 4 == 0 && (((uintptr_t)coeffs | (uintptr_t)v_coeffs) & 15) == 0;
+    sh_stage_in(st, coeffs + base * K3, tot, K3, vec, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const int64_t i = base + lane;
+    if (i < n) {
+        float *c = st + lane * kShStage;  // coefficients in, v_coeffs out (same slots)
+        float acc[NB * 3];
+#pragma unroll
+        for (int q = 0; q < NB * 3; ++q) acc[q] = 0.f;
+        const bool want_dir = v_dirs != nullptr && NB > 1;
+        for (int64_t cam = 0; cam < C; ++cam) {
+            const int64_t ci = cam * n + i;
+            if (masks && !masks[ci]) {
+                if (v_dirs) { v_dirs[ci * 3] = 0.f; v_dirs[ci * 3 + 1] = 0.f; v_dirs[ci * 3 + 2] = 0.f; }
+                continue;
+            }
+            float x = 0, y = 0, z = 0, inv = 0;
+            if (NB > 1) {
+                const float dx = dirs[ci * 3], dy = dirs[ci * 3 + 1], dz = dirs[ci * 3 + 2];
+                inv = tgs_rsqrt<float>(dx * dx + dy * dy + dz * dz);
+                x = dx * inv; y = dy * inv; z = dz * inv;
+            }
+            const float vr = v_colors[ci * 3], vg = v_colors[ci * 3 + 1], vb = v_colors[ci * 3 + 2];
+            float gu = 0, gv = 0, gw = 0;
+            sh_each<float>(NB, x, y, z, [&](int k, float v, float gx, float gy, float gz) {
+                if (want_dir) {
+                    const float s = c[k * 3] * vr + c[k * 3 + 1] * vg + c[k * 3 + 2] * vb;
+                    gu += s * gx;
+                    gv += s * gy;
+                    gw += s * gz;
+                }
+                acc[k * 3] += v * vr;
+                acc[k * 3 + 1] += v * vg;
+                acc[k * 3 + 2] += v * vb;
+            });
+            if (want_dir) {
+                const float gd = gu * x + gv * y + gw * z;
+                v_dirs[ci * 3] = (gu - gd * x) * inv;
+                v_dirs[ci * 3 + 1] = (gv - gd * y) * inv;
+                v_dirs[ci * 3 + 2] = (gw - gd * z) * inv;
+            } else if (v_dirs) {
+                v_dirs[ci * 3] = 0.f; v_dirs[ci * 3 + 1] = 0.f; v_dirs[ci * 3 + 2] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NB * 3; ++q) c[q] = acc[q];  // NB <= K (checked by the entry point)
+        for (int q = NB * 3; q < K3; ++q) c[q] = 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    sh_stage_out(v_coeffs + base * K3, st, tot, K3, vec, lane);
 }
 
 }  // namespace tgs
@@ -303,5 +425,60 @@ extern "C" int tgs_sh_bwd(int dtype, int64_t n, int32_t K, int32_t degree, const
         return TGS_ERR_UNSUPPORTED;
     }
     TGS_CHECK_LAUNCH("sh_bwd");
+    return TGS_OK;
+}
+
+// degree -> basis size template argument
+#define TGS_SH_NB_SWITCH(degree, M)    \
+    switch (degree) {                  \
+        case 0: M(1); break;           \
+        case 1: M(4); break;           \
+        case 2: M(9); break;           \
+        case 3: M(16); break;          \
+        default: M(25); break;         \
+    }
+
+extern "C" int tgs_sh_fwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, const void *dirs,
+                               const void *coeffs, const uint8_t *masks, void *colors, tgs_stream_t stream) {
+    TGS_CHECK_ARG(degree >= 0 && degree <= 4, "sh_fwd_cams: degree %d not in [0, 4]", degree);
+    TGS_CHECK_ARG(K >= (degree + 1) * (degree + 1), "sh_fwd_cams: K=%d < (degree+1)^2", K);
+    TGS_CHECK_ARG(C >= 0 && n >= 0, "sh_fwd_cams: bad sizes C=%lld n=%lld", (long long)C, (long long)n);
+    if (dtype != TGS_F32 || K * 3 > kShStage - 1) {
+        set_error("sh_fwd_cams: float32 with K * 3 <= %d only (got dtype %d, K=%d)", kShStage - 1, dtype, K);
+        return TGS_ERR_UNSUPPORTED;
+    }
+    if (n == 0 || C == 0) return TGS_OK;
+    TGS_CHECK_ARG(dirs && coeffs && colors, "sh_fwd_cams: null buffer");
+    const unsigned nb = (unsigned)ceil_div(n, 256);
+    hipStream_t st = (hipStream_t)stream;
+#define TGS_SHF(NBV)                                                                                     \
+    sh_fwd_cams_kernel<NBV><<<nb, 256, 0, st>>>(C, n, K, (const float *)dirs, (const float *)coeffs, masks, \
+                                                (float *)colors)
+    TGS_SH_NB_SWITCH(degree, TGS_SHF)
+#undef TGS_SHF
+    TGS_CHECK_LAUNCH("sh_fwd_cams");
+    return TGS_OK;
+}
+
+extern "C" int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, const void *dirs,
+                               const void *coeffs, const uint8_t *masks, const void *v_colors, void *v_coeffs,
+                               void *v_dirs, tgs_stream_t stream) {
+    TGS_CHECK_ARG(degree >= 0 && degree <= 4, "sh_bwd_cams: degree %d not in [0, 4]", degree);
+    TGS_CHECK_ARG(K >= (degree + 1) * (degree + 1), "sh_bwd_cams: K=%d < (degree+1)^2", K);
+    TGS_CHECK_ARG(C >= 0 && n >= 0, "sh_bwd_cams: bad sizes C=%lld n=%lld", (long long)C, (long long)n);
+    if (dtype != TGS_F32 || K * 3 > kShStage - 1) {
+        set_error("sh_bwd_cams: float32 with K * 3 <= %d only (got dtype %d, K=%d)", kShStage - 1, dtype, K);
+        return TGS_ERR_UNSUPPORTED;
+    }
+    if (n == 0) return TGS_OK;
+    TGS_CHECK_ARG(dirs && coeffs && v_colors && v_coeffs, "sh_bwd_cams: null buffer");
+    const unsigned nb = (unsigned)ceil_div(n, 256);
+    hipStream_t st = (hipStream_t)stream;
+#define TGS_SHB(NBV)                                                                                      \
+    sh_bwd_cams_kernel<NBV><<<nb, 256, 0, st>>>(C, n, K, (const float *)dirs, (const float *)coeffs, masks,  \
+                                                (const float *)v_colors, (float *)v_coeffs, (float *)v_dirs)
+    TGS_SH_NB_SWITCH(degree, TGS_SHB)
+#undef TGS_SHB
+    TGS_CHECK_LAUNCH("sh_bwd_cams");
     return TGS_OK;
 }

@@ -8,5 +8,6 @@ from ._wrapper import (  # noqa: F401
     rasterize_to_pixels_textured_gaussians,
     selective_adam_update,
     spherical_harmonics,
+    spherical_harmonics_views,
     tgs_fns,
 )

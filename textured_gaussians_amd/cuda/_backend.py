@@ -51,6 +51,8 @@ _PROTOS = {
                                                vp, vp, vp, vp, vp, c_int, vp, vp, vp, vp, vp]),
     "tgs_sh_fwd": (c_int, [c_int, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp]),
     "tgs_sh_bwd": (c_int, [c_int, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
+    "tgs_sh_fwd_cams": (c_int, [c_int, c_i64, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp]),
+    "tgs_sh_bwd_cams": (c_int, [c_int, c_i64, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
     "tgs_rasterize_fwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
                                   vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl, c_dbl,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),

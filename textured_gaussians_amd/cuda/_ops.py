@@ -341,6 +341,46 @@ def compute_sh_bwd(K, degrees_to_use, dirs, coeffs, masks, v_colors, compute_v_d
     return v_coeffs, v_dirs
 
 
+def sh_cams_supported(coeffs) -> bool:
+    """Whether ``compute_sh_{fwd,bwd}_cams`` take these coefficients (float32, K*3 <= 48)."""
+    return coeffs.dtype == torch.float32 and coeffs.dim() == 3 and coeffs.shape[1] * 3 <= 48
+
+
+def compute_sh_fwd_cams(degrees_to_use, dirs, coeffs, masks):
+    """colors [C, N, 3] of dirs [C, N, 3] against one coefficient set coeffs [N, K, 3]
+    (masks [C, N] or None): ``compute_sh_fwd`` of the coefficients expanded to every
+    camera, without the expanded copy."""
+    for t, name in ((dirs, "dirs"), (coeffs, "coeffs")):
+        _check_input(t, name)
+    if masks is not None:
+        _check_input(masks, "masks")
+    C, N = dirs.shape[0], dirs.shape[1]
+    if dirs.shape != (C, N, 3) or coeffs.dim() != 3 or coeffs.shape[0] != N or coeffs.shape[2] != 3:
+        raise RuntimeError(f"sh_fwd_cams: dirs [C, N, 3] and coeffs [N, K, 3] expected ({dirs.shape}, {coeffs.shape})")
+    with torch.cuda.device(dirs.device):
+        colors = torch.empty_like(dirs)
+        check(lib().tgs_sh_fwd_cams(_dtype_code(dirs, "dirs"), C, N, coeffs.shape[1], degrees_to_use, _p(dirs),
+                                    _p(coeffs), _p(masks.view(torch.uint8)) if masks is not None else None,
+                                    _p(colors), _stream(dirs)), "compute_sh_fwd_cams")
+    return colors
+
+
+def compute_sh_bwd_cams(degrees_to_use, dirs, coeffs, masks, v_colors, compute_v_dirs):
+    """(v_coeffs [N, K, 3] summed over the cameras, v_dirs [C, N, 3] or None)."""
+    for t, name in ((dirs, "dirs"), (coeffs, "coeffs"), (v_colors, "v_colors")):
+        _check_input(t, name)
+    if masks is not None:
+        _check_input(masks, "masks")
+    C, N = dirs.shape[0], dirs.shape[1]
+    with torch.cuda.device(dirs.device):
+        v_coeffs = torch.empty_like(coeffs)
+        v_dirs = torch.empty_like(dirs) if compute_v_dirs else None
+        check(lib().tgs_sh_bwd_cams(_dtype_code(dirs, "dirs"), C, N, coeffs.shape[1], degrees_to_use, _p(dirs),
+                                    _p(coeffs), _p(masks.view(torch.uint8)) if masks is not None else None,
+                                    _p(v_colors), _p(v_coeffs), _p(v_dirs), _stream(dirs)), "compute_sh_bwd_cams")
+    return v_coeffs, v_dirs
+
+
 # --------------------------------------------------------------- rasterizer
 def _raster_workspace(dt, C, N, dev):
     nbytes = lib().tgs_rasterize_workspace_bytes(dt, C, N)

@@ -52,6 +52,43 @@ def spherical_harmonics(degrees_to_use: int, dirs: Tensor, coeffs: Tensor,
     return _SphericalHarmonics.apply(degrees_to_use, dirs.contiguous(), coeffs.contiguous(), masks)
 
 
+def spherical_harmonics_views(degrees_to_use: int, dirs: Tensor, coeffs: Tensor,
+                              masks: Optional[Tensor] = None) -> Tensor:
+    """Colours of one coefficient set coeffs [N, K, 3] seen from C views, dirs
+    [C, N, 3] (masks [C, N]) -> [C, N, 3]: what the reference computes as
+    ``spherical_harmonics(d, dirs, coeffs.expand(C, ...), masks)``
+    (rendering.py:1573-1583).  float32 with K <= 16 runs one kernel that loops over
+    the views (no [C, N, K, 3] copy, gradients summed over the views in the kernel);
+    anything else takes that expanded form."""
+    C = dirs.shape[0]
+    if C > 1 and dirs.is_cuda and dirs.dtype == torch.float32 and _C.sh_cams_supported(coeffs):
+        assert (degrees_to_use + 1) ** 2 <= coeffs.shape[-2], coeffs.shape
+        assert dirs.shape == (C, coeffs.shape[0], 3), (dirs.shape, coeffs.shape)
+        if masks is not None:
+            assert masks.shape == dirs.shape[:-1], masks.shape
+            masks = masks.contiguous()
+        return _SphericalHarmonicsViews.apply(degrees_to_use, dirs.contiguous(), coeffs.contiguous(), masks)
+    return spherical_harmonics(degrees_to_use, dirs, coeffs.expand(C, *([-1] * coeffs.dim())), masks)
+
+
+class _SphericalHarmonicsViews(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sh_degree: int, dirs: Tensor, coeffs: Tensor, masks: Tensor) -> Tensor:
+        colors = _C.compute_sh_fwd_cams(sh_degree, dirs, coeffs, masks)
+        ctx.save_for_backward(dirs, coeffs, masks)
+        ctx.sh_degree = sh_degree
+        return colors
+
+    @staticmethod
+    def backward(ctx, v_colors: Tensor):
+        dirs, coeffs, masks = ctx.saved_tensors
+        v_coeffs, v_dirs = _C.compute_sh_bwd_cams(ctx.sh_degree, dirs, coeffs, masks, v_colors.contiguous(),
+                                                    ctx.needs_input_grad[1])
+        if not ctx.needs_input_grad[2]:
+            v_coeffs = None
+        return None, v_dirs, v_coeffs, None
+
+
 class _SphericalHarmonics(torch.autograd.Function):
     @staticmethod
     def forward(ctx, sh_degree: int, dirs: Tensor, coeffs: Tensor, masks: Tensor) -> Tensor:

@@ -24,6 +24,7 @@ from .cuda._wrapper import (
     rasterize_to_pixels_2dgs,
     rasterize_to_pixels_textured_gaussians,
     spherical_harmonics,
+    spherical_harmonics_views,
 )
 from .utils import depth_to_normal
 
@@ -101,17 +102,21 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
     # per-view colours before the tile intersection: they need only the projection, and the
     # intersection count read makes the host wait for the device, so work queued before it
     # overlaps that wait (the reference computes them after the sort, [RD]:1561-1594)
-    if not (colors.dim() == 3 and sh_degree is None):
-        colors = colors.index_select(0, gaussian_ids) if packed else colors.expand(C, *([-1] * colors.dim()))
-    elif packed:
-        colors = colors.reshape(C * N, -1).index_select(0, camera_ids * N + gaussian_ids)
-    if sh_degree is not None:
-        if packed:
-            dirs = means.index_select(0, gaussian_ids) - camtoworlds[camera_ids, :3, 3]
-        else:
-            dirs = means[None, :, :] - camtoworlds[:, None, :3, 3]
-        colors = spherical_harmonics(sh_degree, dirs, colors, masks=radii > 0)
+    if sh_degree is not None and not packed:
+        # grid layout: one coefficient set seen from C views (the reference expands the
+        # coefficients to [C, N, K, 3] first; the views kernel loops over them instead)
+        dirs = means[None, :, :] - camtoworlds[:, None, :3, 3]
+        colors = spherical_harmonics_views(sh_degree, dirs, colors, masks=radii > 0)
         colors = torch.clamp_min(colors + 0.5, 0.0)
+    else:
+        if not (colors.dim() == 3 and sh_degree is None):
+            colors = colors.index_select(0, gaussian_ids) if packed else colors.expand(C, *([-1] * colors.dim()))
+        elif packed:
+            colors = colors.reshape(C * N, -1).index_select(0, camera_ids * N + gaussian_ids)
+        if sh_degree is not None:
+            dirs = means.index_select(0, gaussian_ids) - camtoworlds[camera_ids, :3, 3]
+            colors = spherical_harmonics(sh_degree, dirs, colors, masks=radii > 0)
+            colors = torch.clamp_min(colors + 0.5, 0.0)
     if render_mode in ["RGB+D", "RGB+ED"]:
         colors = torch.cat((colors, depths[..., None]), dim=-1)
     elif render_mode in ["D", "ED"]:
