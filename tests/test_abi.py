@@ -98,6 +98,18 @@ def test_host_validation_errors_without_gpu():
     assert rc != 0 and b"32 bits" in L.tgs_last_error()
     rc = L.tgs_sh_fwd(0, 10, 4, 5, null, null, null, null, null)
     assert rc != 0 and b"degree" in L.tgs_last_error()
+    # the views kernels: float32 with K * 3 <= 48 only
+    rc = L.tgs_sh_fwd_cams(1, 2, 10, 16, 3, null, null, null, null, null)
+    assert rc == 3 and b"float32" in L.tgs_last_error()
+    rc = L.tgs_sh_bwd_cams(0, 2, 10, 17, 3, null, null, null, null, null, null, null)
+    assert rc == 3 and b"K * 3" in L.tgs_last_error()
+    # decoder pieces: operand flags OR-ed into the epilogue, argument rules
+    rc = L.tgs_gemm_bf16(0, 0, 7 | 0x10, 4, 4, 4, null, 4, null, 4, null, null, 4, null, 0, 1, null)
+    assert rc == 1 and b"epilogue" in L.tgs_last_error()
+    rc = L.tgs_gemm_bf16(0, 0, 0x30, 4, 4, 4, null, 4, null, 4, null, null, 4, null, 0, 1, null)
+    assert rc == 1 and b"null operand" in L.tgs_last_error()
+    rc = L.tgs_sigmoid_bwd_colsum(10, 6, null, null, null, 1, null, null)
+    assert rc == 1 and b"multiple of 4" in L.tgs_last_error()
 
 
 def test_ops_reject_cpu_and_noncontiguous_tensors():
