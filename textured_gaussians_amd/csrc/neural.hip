@@ -294,8 +294,9 @@ __global__ __launch_bounds__(256) void colsum_kernel(int M, int N, const float *
     if (rq == 0 && n < N) atomic_add(db + n, part[0][c] + part[1][c] + part[2][c] + part[3][c]);
 }
 
-// The same with 4-vectors: 32 column quads by 8 row lanes per workgroup, four
-// rows in flight per lane.
+// The same with 4-vectors: 32 column quads by 8 row lanes per workgroup, eight
+// rows in flight per lane, and row chunks long enough that few workgroups add
+// into each column (the final atomics to one address serialise).
 __global__ __launch_bounds__(256) void colsum4_kernel(int M, int N4, const float4 *__restrict__ G, int64_t ldg4,
                                                       int rows_per_block, float *__restrict__ db) {
     __shared__ float4 part[8][32];
@@ -305,12 +306,12 @@ __global__ __launch_bounds__(256) void colsum4_kernel(int M, int N4, const float
     float4 s = {0.f, 0.f, 0.f, 0.f};
     if (n4 < N4) {
         int r = r0 + rq;
-        for (; r + 24 < r1; r += 32) {
-            float4 x[4];
+        for (; r + 56 < r1; r += 64) {
+            float4 x[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = G[(int64_t)(r + 8 * u) * ldg4 + n4];
+            for (int u = 0; u < 8; ++u) x[u] = G[(int64_t)(r + 8 * u) * ldg4 + n4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 s.x += x[u].x;
                 s.y += x[u].y;
                 s.z += x[u].z;
@@ -662,7 +663,7 @@ extern "C" int tgs_colsum(int64_t M, int64_t N, const float *G, int64_t ldg, flo
                   "colsum: bad arguments");
     if (M == 0 || N == 0) return TGS_OK;
     if (N % 4 == 0 && ldg % 4 == 0 && ((uintptr_t)G % 16) == 0) {
-        const int rows = 512;
+        const int rows = 2048;
         dim3 grid((unsigned)ceil_div(N / 4, 32), (unsigned)ceil_div(M, rows));
         colsum4_kernel<<<grid, 256, 0, (hipStream_t)stream>>>((int)M, (int)(N / 4), (const float4 *)G, ldg / 4, rows,
                                                               db);
