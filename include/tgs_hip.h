@@ -214,7 +214,8 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * entry i samples texture row gaussian_ids[i] (the reference indexes the
  * textures with the entry index there, Q3).  N is then the number of
  * Gaussians (texture rows, contribution counters), and the workspace is
- * tgs_rasterize_workspace_bytes(dtype, 1, nnz).
+ * tgs_rasterize_workspace_bytes(dtype, 1, nnz) (0 bytes, may be NULL, when
+ * nnz = 0; gaussian_ids must still be non-NULL then, it is not read).
  * Outputs (written): render_colors[C,H,W,K] render_alphas[C,H,W]
  * render_normals[C,H,W,3] render_distort[C,H,W] render_median[C,H,W]
  * last_ids i32[C,H,W] median_ids i32[C,H,W].

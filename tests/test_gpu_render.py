@@ -212,3 +212,20 @@ def test_singular_view_matrix_raises():
         tg.rasterization_textured_gaussians(
             L["means"], L["quats"], L["scales"], L["opacities"], L["colors"], L["textures"], vm, sc.Ks.to(DEV),
             sc.width, sc.height, sh_degree=sc.sh_degree, backgrounds=sc.backgrounds.to(DEV))
+
+
+@pytest.mark.parametrize("packed", [False, True])
+def test_all_gaussians_culled(packed):
+    """Every Gaussian behind the camera: no intersections.  The image is the
+    background with zero alpha, and every gradient is zero."""
+    sc = _scene(n=80).to(dtype=torch.float32)
+    sc.means[:] = torch.tensor([3.5, 0.3, 0.4]) * 3.0
+    L = _leaves(sc, DEV)
+    out = _gpu_render(sc, L, packed=packed)
+    assert out[-1]["isect_ids"].numel() == 0
+    bg = sc.backgrounds.to(DEV)
+    assert torch.equal(out[0][0], bg[0].expand_as(out[0][0]))
+    assert torch.all(out[1] == 0)
+    (out[0].sum() + out[1].sum()).backward()
+    for k, v in L.items():
+        assert v.grad is None or torch.all(v.grad == 0), k

@@ -1297,6 +1297,7 @@ template <typename S> static size_t abc_bytes(int64_t C, int64_t N) {
 template <typename S, int K>
 static void launch_pack(const RasterArgs<S> &a, hipStream_t st) {
     const int64_t n = a.E;
+    if (n == 0) return;  // packed layout without entries
     pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.N, a.gids, a.means2d, a.rt, a.opac, a.colors,
                                                                     a.normals, (S *)a.rec);
 }
@@ -1543,7 +1544,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
     if (rc) return rc;
     const int64_t E = gaussian_ids ? nnz : C * N;  // per-entry inputs
     TGS_CHECK_ARG(!gaussian_ids || (nnz >= 0 && nnz < (int64_t(1) << 31)), "rasterize_fwd: bad nnz");
-    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, 1, E),
+    TGS_CHECK_ARG((workspace || workspace_bytes(dtype, 1, E) == 0) && workspace_size >= workspace_bytes(dtype, 1, E),
                   "rasterize_fwd: workspace of %zu bytes required", workspace_bytes(dtype, 1, E));
     rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
     if (rc) return rc;
@@ -1611,7 +1612,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                   "two rows fit in LDS)");
     const int64_t E = gaussian_ids ? nnz : C * N;  // per-entry inputs
     TGS_CHECK_ARG(!gaussian_ids || (nnz >= 0 && nnz < (int64_t(1) << 31)), "rasterize_bwd: bad nnz");
-    TGS_CHECK_ARG(workspace && workspace_size >= workspace_bytes(dtype, 1, E),
+    TGS_CHECK_ARG((workspace || workspace_bytes(dtype, 1, E) == 0) && workspace_size >= workspace_bytes(dtype, 1, E),
                   "rasterize_bwd: workspace of %zu bytes required", workspace_bytes(dtype, 1, E));
     rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
     if (rc) return rc;
@@ -1646,7 +1647,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.g_col = (S *)v_colors; a.g_op = (S *)v_opacities; a.g_tex = (S *)v_textures; a.g_nrm = (S *)v_normals; \
         const int64_t n = E;                                                                                    \
         const unsigned nb = (unsigned)ceil_div(n, 256);                                                         \
-        zero_kernel<S><<<(unsigned)ceil_div(n * kAbc, 256), 256, 0, st>>>(n * kAbc, a.g_abc);                   \
+        if (n > 0) zero_kernel<S><<<(unsigned)ceil_div(n * kAbc, 256), 256, 0, st>>>(n * kAbc, a.g_abc);        \
         TGS_CHECK_LAUNCH("rasterize_bwd zero");                                                                 \
         if (n_isects > 0) {                                                                                     \
             rc = dispatch<S, true>(a, K, tex_mode, st);                                                         \
@@ -1658,9 +1659,9 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
             rc = launch_gather<S>(N, tex_h, tex_w, a, zero_rows_dst ? zero_rows_mask : nullptr, st);             \
             if (rc) return rc;                                                                                  \
         }                                                                                                       \
-        rt_grad_kernel<S><<<nb, 256, 0, st>>>(n, (const S *)ray_transforms, (const S *)means2d, a.g_abc,        \
-                                              (S *)v_ray_transforms,                                            \
-                                              (S *)v_densify);                                                  \
+        if (n > 0)                                                                                              \
+            rt_grad_kernel<S><<<nb, 256, 0, st>>>(n, (const S *)ray_transforms, (const S *)means2d, a.g_abc,    \
+                                                  (S *)v_ray_transforms, (S *)v_densify);                       \
         TGS_CHECK_LAUNCH("rasterize_bwd ray-transform gradient");                                               \
         return TGS_OK;                                                                                          \
     }

@@ -428,6 +428,10 @@ def _packed_args(means2d, gaussian_ids, textures, n_gaussians):
         N = int(n_gaussians)
     else:
         N = int(gaussian_ids.max().item()) + 1 if gaussian_ids.numel() else 1
+    if gaussian_ids.numel() == 0:
+        # no entries: the C ABI tells the packed layout by a non-null gaussian_ids, and an
+        # empty tensor has none; the one placeholder id is never read (nnz = 0)
+        gaussian_ids = torch.zeros(1, dtype=torch.int64, device=means2d.device)
     return gaussian_ids, means2d.size(0), N
 
 
