@@ -229,3 +229,17 @@ def test_all_gaussians_culled(packed):
     (out[0].sum() + out[1].sum()).backward()
     for k, v in L.items():
         assert v.grad is None or torch.all(v.grad == 0), k
+
+
+@pytest.mark.parametrize("packed", [False, True])
+def test_no_gaussians(packed):
+    """N = 0: the image is the background, alphas are zero, the gradients are empty."""
+    sc = _scene(n=20).to(dtype=torch.float32)
+    L = {k: v[:0].contiguous().detach().requires_grad_(True) for k, v in _leaves(sc, DEV).items()}
+    out = _gpu_render(sc, L, packed=packed)
+    bg = sc.backgrounds.to(DEV)
+    assert torch.equal(out[0][0], bg[0].expand_as(out[0][0]))
+    assert torch.all(out[1] == 0)
+    (out[0].sum() + out[1].sum()).backward()
+    for k, v in L.items():
+        assert v.grad is None or v.grad.numel() == 0, k

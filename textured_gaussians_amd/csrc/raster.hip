@@ -1401,7 +1401,7 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
 static int check_common(int64_t C, int64_t N, int32_t K, const void *textures, int64_t tex_h, int64_t tex_w,
                         double trx, double try_, int32_t W, int32_t H, int32_t ts, int32_t tw, int32_t th,
                         int64_t n_isects) {
-    TGS_CHECK_ARG(C >= 0 && N > 0 && C * N < (int64_t(1) << 31), "rasterize: bad sizes C=%lld N=%lld",
+    TGS_CHECK_ARG(C >= 0 && N >= 0 && C * N < (int64_t(1) << 31), "rasterize: bad sizes C=%lld N=%lld",
                   (long long)C, (long long)N);
     TGS_CHECK_ARG(K >= 1 && K <= 4, "rasterize: Unsupported number of channels: %d", K);
     TGS_CHECK_ARG(ts == 8 || ts == 16, "rasterize: tile_size %d unsupported (8 or 16)", ts);

@@ -385,7 +385,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             listed = flatten_ids if gaussian_ids is None else gaussian_ids[flatten_ids.long()].int()
             ctx.adam = (texture_adam, _C.mark_rows(listed, textures.shape[0]))
             ctx.tex_counts = _C.tex_counts_buffer(textures)
-        elif ctx.needs_input_grad[6] and textures.is_cuda:
+        elif ctx.needs_input_grad[6] and textures.is_cuda and textures.shape[0] > 0:
             # float gradient (also for bfloat16 textures; autograd casts it to theirs)
             v_tex = torch.empty(textures.shape, dtype=means2d.dtype, device=textures.device)
             # texture rows present in the tile lists (packed: the entries' Gaussians)
