@@ -54,8 +54,9 @@ def _gpu_fwd(st, colors, opac, textures, bg, dtype, g_weight=1.0, thr=0.0):
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("textured", [True, False])
 @pytest.mark.parametrize("K", [1, 3, 4])
-def test_raster_fwd(dtype, textured, K):
-    sc, st, colors, opac, bg = _inputs(dtype, K=K)
+@pytest.mark.parametrize("size", [(80, 64), (75, 53)])  # the second is ragged: partial edge tiles
+def test_raster_fwd(dtype, textured, K, size):
+    sc, st, colors, opac, bg = _inputs(dtype, K=K, w=size[0], h=size[1])
     tex = sc.textures.double() if textured else None
     out = _gpu_fwd(st, colors, opac, tex, bg, dtype)
     ref = _oracle_fwd(st, colors, opac, tex, bg)
