@@ -103,8 +103,8 @@ def test_raster_fwd_contrib_threshold_and_gweight():
     assert torch.equal(out[8].cpu().double(), ref["contrib_count"])
 
 
-def _bwd_case(dtype, uv, textured=True, K=3, use_bg=True, seed=0):
-    sc, st, colors, opac, bg = _inputs(dtype, K=K, seed=seed)
+def _bwd_case(dtype, uv, textured=True, K=3, use_bg=True, seed=0, size=(80, 64)):
+    sc, st, colors, opac, bg = _inputs(dtype, K=K, seed=seed, w=size[0], h=size[1])
     if not use_bg:
         bg = None
     s = st["scene"]
@@ -205,6 +205,11 @@ def _bwd_case(dtype, uv, textured=True, K=3, use_bg=True, seed=0):
 @pytest.mark.parametrize("uv", ["none", "exact", "double"])
 def test_raster_bwd_textured_f64(uv):
     _bwd_case(torch.float64, uv)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_raster_bwd_textured_ragged_image(dtype):
+    _bwd_case(dtype, "none", size=(75, 53))
 
 
 @pytest.mark.parametrize("uv", ["none", "exact"])
