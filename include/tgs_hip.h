@@ -223,6 +223,11 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * render_accum_depth[C,H,W] (may be NULL; no reference counterpart): the
  * forward's distortion accumulator sum_i w_i colors[i, K-1], for the exact
  * distortion gradient.
+ * render_trans[C,H,W] (may be NULL; no reference counterpart): the final
+ * transmittance T of each pixel.  The reference's backward starts from
+ * 1 - render_alphas ([TB]:456-457), which in float32 keeps T only to an
+ * absolute 6e-8 (6e-4 relative at the 1e-4 stop); given to
+ * tgs_rasterize_bwd, the backward starts from T itself.
  * live_entries (may be NULL; no reference counterpart): uint8[waves][n_isects]
  * with waves = ceil(tile_size^2 / 64), the wavefronts of a tile; the forward
  * sets byte [w][i] to 1 where some pixel of wave w of entry i's tile
@@ -240,10 +245,13 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * zero_rows_dst (may be NULL; no reference counterpart): a buffer of N rows of
  * zero_row_bytes bytes (16-byte aligned, a multiple of 16) whose rows with
  * zero_rows_mask[n] == 1 this call zero-fills along the way, using memory
- * bandwidth the rasterizer leaves idle; tgs_rasterize_bwd fills the rows with
- * mask 0.  With the mask of tgs_mark_rows this prepares the texture gradient:
- * the rows the backward accumulates into are zero when it starts, the other
- * rows when it ends.                                                      */
+ * bandwidth the rasterizer leaves idle.  zero_rows_select picks the rows: its
+ * low byte is the mask value to fill (0 or 1), bits 8-15 a set of residues of
+ * row mod 8 (0 = every row).  With the mask of tgs_mark_rows this prepares the
+ * texture gradient: the forward fills the rows with mask 1 (those the backward
+ * accumulates into) and the backward those with mask 0, or, with the records
+ * of tgs_rasterize_bwd, the rows with mask 0 are split between forward and
+ * backward (disjoint residues) and the gather writes the rows with mask 1.  */
 size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
 /* Bytes of one texture-gradient record (see tgs_rasterize_bwd). */
 size_t tgs_tex_record_bytes(int dtype);
@@ -265,9 +273,10 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *render_alphas, void *render_normals, void *render_distort,
                       void *render_median, int32_t *last_ids, int32_t *median_ids,
                       void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                      uint8_t *live_entries, int32_t *tex_counts, void *zero_rows_dst,
-                      const uint8_t *zero_rows_mask, size_t zero_row_bytes, void *workspace,
-                      size_t workspace_size, tgs_stream_t stream);
+                      void *render_trans, uint8_t *live_entries, int32_t *tex_counts,
+                      void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
+                      int zero_rows_select, void *workspace, size_t workspace_size,
+                      tgs_stream_t stream);
 
 /* Replaces `rasterize_to_pixels_bwd_textured_gaussians` /
  * `rasterize_to_pixels_bwd2_textured_gaussians` (ext.cpp:89-94,
@@ -278,17 +287,18 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
  * accumulator with render_colors[..., K-1] (that includes the texture colour
  * and T*background of the channel); non-NULL gives the true gradient.
  * live_entries: NULL (visit every entry) or the array the forward of the same
- * inputs wrote.
+ * inputs wrote.  render_trans: NULL (start from 1 - render_alphas, as the
+ * reference) or the forward's final transmittance.
  * Texture-gradient records (all NULL, or all given; no reference counterpart):
  * tex_counts from the forward of the same inputs, tex_rec_incl int64[N] its
  * inclusive prefix sum (tgs_prefix_sum_i32), tex_rec_cursor int32[N] zeroed,
- * tex_records of tex_rec_incl[N-1] * tgs_tex_record_bytes(dtype) bytes.  The
+ * tex_records of max(1, tex_rec_incl[N-1]) * tgs_tex_record_bytes(dtype) bytes.  The
  * backward then writes one record per pair instead of adding to v_textures in
  * L2, and a gather kernel sums each texture row's records in LDS and writes the
  * row once: rows with zero_rows_mask == 1 are written in full (so they need no
- * zero-fill beforehand; the rows with mask 0 are zero-filled by this call), or,
- * with zero_rows_dst NULL, the sums are added to v_textures.  Rows larger than
- * 64 KB fall back to direct accumulation.  With v_textures NULL (and
+ * zero-fill beforehand; the rows with mask 0 that zero_rows_select names are
+ * zero-filled by this call), or, with zero_rows_dst NULL, the sums are added
+ * to v_textures.  Rows larger than 64 KB fall back to direct accumulation.  With v_textures NULL (and
  * zero_rows_dst NULL) only the records are written, for tgs_texture_adam_rows.
  * dtype | TGS_TEX_BF16: textures are bfloat16 (float32 compute); the gradient
  * outputs stay float32.
@@ -310,8 +320,8 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const int32_t *flatten_ids, int64_t n_isects, double g_weight,
                       const void *render_colors, const void *render_alphas,
                       const int32_t *last_ids, const int32_t *median_ids,
-                      const void *render_accum_depth, const uint8_t *live_entries,
-                      const int32_t *tex_counts, const int64_t *tex_rec_incl,
+                      const void *render_accum_depth, const void *render_trans,
+                      const uint8_t *live_entries, const int32_t *tex_counts, const int64_t *tex_rec_incl,
                       int32_t *tex_rec_cursor, void *tex_records, const void *v_render_colors,
                       const void *v_render_alphas,
                       const void *v_render_normals, const void *v_render_distort,
@@ -319,7 +329,8 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *v_means2d, void *v_ray_transforms, void *v_colors,
                       void *v_opacities, void *v_textures, void *v_normals, void *v_densify,
                       void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
-                      void *workspace, size_t workspace_size, tgs_stream_t stream);
+                      int zero_rows_select, void *workspace, size_t workspace_size,
+                      tgs_stream_t stream);
 
 /* Mipmap pyramid for the `mipmapped2` filter (generate_mipmap_fwd.cu,
  * generate_mipmap_bwd.cu; ext.cpp:188-193): textures [N, R, R, C] (R = 2^L,

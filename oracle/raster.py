@@ -175,7 +175,11 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     per Gaussian, the largest condition number |hu||hv|/|cross.z| of the splat
     coordinate over its kept pairs (large for splats seen edge-on);
     ``texel_edge`` [C,N] marks Gaussians with a kept pair whose texture
-    coordinate is within 1e-4 texel of a texel edge.
+    coordinate is within 1e-4 texel of a texel edge.  ``fragile_rows`` [C,N]
+    marks Gaussians with a fragile pair or a valid pair in a fragile pixel: the
+    rows whose gradient a flipped float32 decision can move; ``fragile_own`` [C,N]
+    the subset with a fragile pair of their own (a flip there adds or removes that
+    pair's whole term from the Gaussian's gradient).
 
     ``mip_log_res``: ``textures`` is the flat pyramid [N, T, 4] of ``generate_mipmap``
     over 2^mip_log_res textures and sampling is the reference's ``mipmapped2`` filter
@@ -207,6 +211,8 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     fragile = torch.zeros(C, H, W, dtype=torch.bool)
     cond_max = torch.zeros(C * N, dtype=torch.float64)
     texel_edge = torch.zeros(C * N, dtype=torch.bool)
+    fragile_rows = torch.zeros(C * N, dtype=torch.bool)
+    fragile_own = torch.zeros(C * N, dtype=torch.bool)
     offs = isect_offsets.reshape(-1).tolist()
     fl = flatten_ids.long()
     mflat = means2d.reshape(C * N, 2)
@@ -315,12 +321,21 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
             dist = dist + extra - extra.detach()
         out_d[c, yy, xx] = dist[:, None]
         with torch.no_grad():
-            fr = okz & ((alpha - 1.0 / 255.0).abs() < 1e-6)
-            fr |= (Tincl / 1e-4 - 1).abs() < 1e-4
-            fr |= (Texcl - 0.5).abs() < 1e-5
+            # margins: a few times the float32 error of each quantity (alpha to ~1e-6
+            # relative, T to ~1e-6 relative per entry, s to ~1e-6 for well-conditioned splats)
+            fr = okz & ((alpha - 1.0 / 255.0).abs() < 2e-7)
+            fr |= (Tincl / 1e-4 - 1).abs() < 2e-5
+            fr |= (Texcl - 0.5).abs() < 2e-6
             if textured and mip_log_res is None:
-                fr |= ((s[..., 0].abs() - rx).abs() < 1e-4) | ((s[..., 1].abs() - ry).abs() < 1e-4)
-            fragile[c, yy, xx] = fr.any(1)
+                fr |= ((s[..., 0].abs() - rx).abs() < 2e-5) | ((s[..., 1].abs() - ry).abs() < 2e-5)
+            fr_pix = fr.any(1)
+            fragile[c, yy, xx] = fr_pix
+            # Gaussians whose gradient a flipped decision can move: the fragile pair's own
+            # Gaussian, and every Gaussian with a valid pair in a fragile pixel (a flip there
+            # changes that pixel's transmittance and accumulated colour behind them)
+            fr_g = (fr | (valid & fr_pix[:, None])).any(0)
+            fragile_rows.index_put_((g,), fr_g, accumulate=True)
+            fragile_own.index_put_((g,), fr.any(0), accumulate=True)
             # conditioning of s = cross.xy / cross.z for kept pairs: |hu| |hv| / |cross.z|
             cnd = (hu.norm(dim=-1) * hv.norm(dim=-1) / cz.abs().clamp_min(1e-300)).to(torch.float64)
             cnd = torch.where(inc, cnd, torch.zeros_like(cnd)).amax(0)
@@ -351,7 +366,8 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
     return {"render_colors": out_c, "render_alphas": out_a, "render_normals": out_n,
             "render_distort": out_d, "render_median": out_m, "last_ids": last_ids,
             "median_ids": median_ids, "contrib_sum": csum, "contrib_count": ccnt, "fragile": fragile,
-            "cond_max": cond_max.reshape(C, N), "texel_edge": texel_edge.reshape(C, N)}
+            "cond_max": cond_max.reshape(C, N), "texel_edge": texel_edge.reshape(C, N),
+            "fragile_rows": fragile_rows.reshape(C, N), "fragile_own": fragile_own.reshape(C, N)}
 
 
 def rasterize_pixel_loop(means2d, ray_transforms, colors, opacities, normals, textures,

@@ -82,16 +82,16 @@ def test_host_validation_errors_without_gpu():
     # 5 channels are rejected before any device work (the reference dispatches 1..5 but
     # its python layer pads 5 -> 8, which it rejects)
     rc = L.tgs_rasterize_fwd(0, 1, 10, 5, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 16, 1, 1, null, null,
-                             0, 0.0, 1.0, *([null] * 14), 0, null, 0, null)
+                             0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
     assert rc != 0 and b"channels" in L.tgs_last_error()
     rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 12, 2, 2, null, null,
-                             0, 0.0, 1.0, *([null] * 14), 0, null, 0, null)
+                             0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
     assert rc != 0 and b"tile_size" in L.tgs_last_error()
     # the packed-record workspace is required and sized by tgs_rasterize_workspace_bytes
     assert L.tgs_rasterize_workspace_bytes(0, 2, 1000) >= 2 * 1000 * 20 * 4
     assert L.tgs_rasterize_workspace_bytes(1, 2, 1000) >= 2 * 1000 * 20 * 8
     rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 16, 1, 1, null, null,
-                             0, 0.0, 1.0, *([null] * 14), 0, null, 0, null)
+                             0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
     assert rc != 0 and b"workspace" in L.tgs_last_error()
     # camera and tile ids must fit in 32 bits
     rc = L.tgs_isect_tiles_count(1 << 20, 1, null, null, 16, 4096, 4096, null, null, null, 0, null)

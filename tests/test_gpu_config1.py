@@ -1,0 +1,44 @@
+"""The CPU-runnable workload, whole frame: N = 10,000 Gaussians, 16x16 RGBA
+textures, SH degree 0, one 256x256 camera (fov 40) at (4, 0, 0), 16x16 tiles,
+white background.  Every tile of the frame goes through the oracle, in float64
+and float32, for the forward and for the backward under the three texture
+gradient rules (the reference's ``bilinear`` and ``bilinear_bwd2``, and the
+exact one), with random upstream gradients on all five rendered outputs.
+Tolerances are the ones stated in gpu_util."""
+import pytest
+import torch
+
+from textured_gaussians_amd import scenes
+
+from gpu_util import DEV, all_tiles, gpu_stages, raster_tile_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _stages(dtype):
+    sc = scenes.config1()
+    st = gpu_stages(sc, dtype)
+    s = st["scene"]
+    # SH degree 0: colour = max(0.2820948 c + 0.5, 0), one view
+    st["colors"] = torch.clamp_min(0.28209479177387814 * s.colors[None, :, 0, :] + 0.5, 0.0).contiguous()
+    st["opac"] = s.opacities[None].contiguous()
+    return st
+
+
+@pytest.mark.parametrize("dtype,uv,records", [(torch.float32, "none", False), (torch.float32, "double", False),
+                                              (torch.float32, "exact", False), (torch.float64, "none", False),
+                                              (torch.float64, "exact", False), (torch.float32, "none", True),
+                                              (torch.float64, "exact", True)])
+def test_config1_full_frame(dtype, uv, records, monkeypatch):
+    """records: texture gradients through per-pair records and the row gather (with the
+    unlisted-row fill split between forward and backward) instead of float atomics."""
+    from textured_gaussians_amd.cuda import _wrapper
+    monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", records)
+    monkeypatch.setattr(_wrapper, "FWD_FILL_RESIDUES", 0x0F)
+    st = _stages(dtype)
+    assert st["flatten_ids"].numel() > 50_000
+    s = st["scene"]
+    tiles = all_tiles(st)
+    stats = raster_tile_parity(st, st["colors"], st["opac"], s.textures, s.backgrounds, dtype, tiles, uv=uv)
+    print("config1", dtype, uv, stats)
+    assert stats["tiles"] == 256

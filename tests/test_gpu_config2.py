@@ -1,0 +1,90 @@
+"""The headline workload at full size: N = 1,000,000 Gaussians, 50x50 RGBA
+float32 textures (40 GB), SH degree 3, one 1297x840 camera (82x53 tiles, the
+last column 1 pixel wide and the last row 8 pixels high).
+
+The oracle cannot replay the whole frame, so the test combines:
+- the tile lists, sort and offsets of the whole frame, bit-exact against the
+  oracle (about 4.3M intersections);
+- forward and backward parity on tiles sampled over the frame, including the
+  ragged corner tile; the upstream gradient is nonzero only inside them, so every
+  per-Gaussian and per-texel gradient comes from them and must equal the oracle
+  run on those tiles alone;
+- size-independent properties of the full render: alpha in [0, 1], texture rows
+  and entries outside the sampled lists get exactly zero gradient.
+Both the float32 kernels (the benchmarked build) and the float64 kernels (80 GB
+of textures and 80 GB of texture gradient) run at full size.  Tolerances are the
+ones stated in gpu_util.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import isect as O
+from textured_gaussians_amd import scenes
+from textured_gaussians_amd.cuda import _wrapper
+
+from gpu_util import DEV, gpu_stages, raster_tile_parity, sample_tiles
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def stages():
+    torch.cuda.empty_cache()
+    sc = scenes.config2(texture_device=DEV)
+    tex = sc.textures
+    sc.textures = None
+    st = gpu_stages(sc)
+    st["textures"] = tex
+    s = st["scene"]
+    # per-view colours as the API makes them: degree-3 SH of the view direction, + 0.5, clamped
+    with torch.no_grad():
+        camc = torch.linalg.inv(s.viewmats)[:, None, :3, 3]
+        col = _wrapper.spherical_harmonics_views(3, s.means[None] - camc, s.colors, masks=st["radii"] > 0)
+        st["colors"] = torch.clamp_min(col + 0.5, 0.0).contiguous()
+    st["opac"] = s.opacities[None].contiguous()
+    yield st
+    st.clear()
+    torch.cuda.empty_cache()
+
+
+def test_config2_isect_bit_exact(stages):
+    st = stages
+    tpg, ids, flat = O.isect_tiles(st["means2d"].cpu().numpy(), st["radii"].cpu().numpy(),
+                                   st["depths"].cpu().numpy(), 16, st["tw"], st["th"])
+    assert (st["tw"], st["th"]) == (82, 53)
+    assert ids.size > 3_000_000
+    assert np.array_equal(st["tpg"].cpu().numpy(), tpg)
+    assert np.array_equal(st["isect_ids"].cpu().numpy(), ids)
+    assert np.array_equal(st["flatten_ids"].cpu().numpy(), flat)
+    assert np.array_equal(st["offsets"].cpu().numpy(), O.isect_offset_encode(ids, 1, st["tw"], st["th"]))
+
+
+def _tiles(st):
+    # 14 tiles over the frame, plus the ragged corner tile (1 x 8 pixels) and the last full column
+    return sample_tiles(st, 14, extra=[(0, st["th"] - 1, st["tw"] - 1), (0, st["th"] // 2, st["tw"] - 2)])
+
+
+def test_config2_sampled_tiles_f32(stages):
+    st = stages
+    bg = st["scene"].backgrounds
+    stats = raster_tile_parity(st, st["colors"], st["opac"], st["textures"], bg, torch.float32, _tiles(st))
+    print("config2 f32", stats)
+    assert stats["fragile_pixels"] < 0.05
+
+
+def test_config2_sampled_tiles_f64(stages):
+    st = stages
+    torch.cuda.empty_cache()
+    f = torch.float64
+    st64 = dict(st)
+    for k in ("means2d", "rt", "normals"):
+        st64[k] = st[k].to(f)
+    tex64 = st["textures"].to(f)  # 80 GB; the float64 texture gradient takes another 80 GB
+    try:
+        stats = raster_tile_parity(st64, st["colors"].to(f), st["opac"].to(f), tex64, st["scene"].backgrounds.to(f),
+                                   f, _tiles(st))
+    finally:
+        del tex64
+        torch.cuda.empty_cache()
+    print("config2 f64", stats)

@@ -324,3 +324,7 @@ def test_texture_gradient_records_match_direct_accumulation(dtype, tex_res):
     for a, b in zip(direct[1:], rec[1:]):
         assert rel_err(b, a) < tol
     assert float(rec[5][mask == 0].abs().max()) == 0.0
+    # without a row mask the gather adds each row's sums to the given (zeroed) gradient
+    va = torch.zeros_like(tex)
+    acc = _ops.rasterize_to_pixels_bwd_exact_textured_gaussians(*bargs, v_textures_out=va, tex_counts=cnt)
+    assert rel_err(acc[5], direct[5]) < tol

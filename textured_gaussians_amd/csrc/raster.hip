@@ -107,6 +107,7 @@ template <typename S> struct RasterArgs {
     // forward outputs
     S *out_c, *out_a, *out_n, *out_d, *out_m, *csum, *ccnt;
     S *out_accd;  // optional: sum_i w_i depth_i per pixel (the forward's distortion accumulator)
+    S *out_T;     // optional: the final transmittance per pixel (the backward's starting T)
     int32_t *last_ids, *median_ids;
     // optional: per wave of a tile, one byte per list entry, nonzero where some pixel of
     // the wave composited the entry ([waves per tile][n_isects]; forward writes, backward reads)
@@ -123,15 +124,17 @@ template <typename S> struct RasterArgs {
     // backward inputs
     const S *r_c, *r_a, *v_c, *v_a, *v_n, *v_d, *v_m;
     const S *r_accd;  // optional: see out_accd; NULL = the reference's seed, render_colors[..., K-1]
+    const S *r_T;     // optional: see out_T; NULL = 1 - render_alphas (cancels in float32 where T is small)
     const int32_t *r_last, *r_med;
     int uv_mode;
     // backward outputs (g_abc: scratch, [C*N][kAbc], turned into g_rt by the post-pass)
     S *g_xy_abs, *g_xy, *g_rt, *g_col, *g_op, *g_tex, *g_nrm, *g_abc;
     // optional fused zero-fill of rows of a [N][zrow16 x 16 bytes] buffer with zmask[row] == zsel
+    // and bit (row & 7) of zres set
     void *zdst;
     const uint8_t *zmask;
     int64_t zrow16;
-    int32_t zsel;
+    int32_t zsel, zres;
 };
 
 template <typename S> __device__ __forceinline__ S rec_int(int32_t v);
@@ -443,18 +446,18 @@ struct RowZeroer {
     const uint8_t *mask;
     uint64_t bits;
     int32_t N, row16, stride, rb;  // rb: row of bit 0 of `bits`
-    int32_t o, sel;
+    int32_t o, sel, res;
     __device__ __forceinline__ void reload(int32_t lane) {
         // next chunk of 64 rows with at least one selected row (bounded by N / stride chunks)
         while (bits == 0 && rb < N) {
             const int64_t r = (int64_t)rb + (int64_t)lane * stride;
-            const bool s = r < N && (int32_t)mask[r] == sel;
+            const bool s = r < N && (int32_t)mask[r] == sel && ((res >> (r & 7)) & 1);
             bits = __ballot(s);
             if (bits == 0) rb = (int32_t)min((int64_t)N, (int64_t)rb + 64 * (int64_t)stride);
         }
     }
-    __device__ __forceinline__ void init(void *d, const uint8_t *m, int64_t n, int64_t r16, int32_t selv, int32_t wave,
-                                         int32_t waves, int32_t lane) {
+    __device__ __forceinline__ void init(void *d, const uint8_t *m, int64_t n, int64_t r16, int32_t selv,
+                                         int32_t resv, int32_t wave, int32_t waves, int32_t lane) {
         dst = (z32x4 *)d;
         mask = m;
         N = d ? (int32_t)n : 0;
@@ -464,6 +467,7 @@ struct RowZeroer {
         bits = 0;
         o = 0;
         sel = selv;
+        res = resv;
         reload(lane);
     }
     __device__ __forceinline__ void step(int32_t lane) {
@@ -488,7 +492,7 @@ struct RowZeroer {
 
 template <typename S> __device__ __forceinline__ void zeroer_init(RowZeroer &z, const RasterArgs<S> &a, int lane) {
     const int32_t wpb = blockDim.x / kWave;
-    z.init(a.zdst, a.zmask, a.N, a.zrow16, a.zsel, (int32_t)(blockIdx.x * wpb + threadIdx.x / kWave),
+    z.init(a.zdst, a.zmask, a.N, a.zrow16, a.zsel, a.zres, (int32_t)(blockIdx.x * wpb + threadIdx.x / kWave),
            (int32_t)(gridDim.x * wpb), lane);
 }
 
@@ -521,6 +525,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
             a.last_ids[pix] = 0;
             a.median_ids[pix] = 0;
             if (a.out_accd) a.out_accd[pix] = 0;
+            if (a.out_T) a.out_T[pix] = 1;
         }
         zr.drain(lane);
         return;
@@ -636,6 +641,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
         a.out_m[pix] = median;
         a.median_ids[pix] = median_id;
         if (a.out_accd) a.out_accd[pix] = acc_vd;
+        if (a.out_T) a.out_T[pix] = T;
     }
 }
 
@@ -670,8 +676,9 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
     const int32_t start = as_const(a.offsets)[tile_lin];
     const int32_t end = (tile_lin == a.C * tiles_per_cam - 1) ? (int32_t)a.n_isects : as_const(a.offsets)[tile_lin + 1];
 
-    // transmittance after the last contributing entry; walked back by division
-    const S T_final = S(1) - a.r_a[pix];
+    // transmittance after the last contributing entry, as the forward left it (1 - alpha
+    // loses the small T of saturated pixels to cancellation); walked back by division
+    const S T_final = a.r_T ? a.r_T[pix] : S(1) - a.r_a[pix];
     S T = T_final;
     S buf[K], bufn[3] = {0, 0, 0};
 #pragma unroll
@@ -1028,107 +1035,233 @@ __global__ __launch_bounds__(256) void rt_grad_kernel(int64_t n, const S *__rest
     }
 }
 
-// Texture-gradient gather.  Each 512-thread workgroup owns a contiguous range of
-// texture rows and sums each row's records in LDS (float atomics there instead of
-// in L2), then writes the row once.  The workgroup is split by role: waves 0-3
-// zero a row buffer and add the row's records into it, waves 4-7 write the
-// previous row out of the other buffer meanwhile.  On this architecture one
-// counter covers a wave's loads and stores, so a wave that loaded records after
-// issuing stores would wait for the stores; with the roles split, the record loads
-// never wait for the write-out and the write-out never waits at all.
-// overwrite: rows with mask[row] != 0 are written in full (they were not
-// zero-filled beforehand); otherwise the sums are added to the row in place and
-// rows without records are left alone.
+// Texture-gradient gather: per texture row, the sum of its records (one per
+// (pixel, Gaussian) pair the backward found in the texture range), written once.
+// Float atomics into L2 would cost one request per 32 bytes of texel row per
+// pair; here the additions happen in LDS and HBM sees each row once, so the
+// kernel streams at write bandwidth.
+//
+// gather_list_kernel first compacts the rows to write into a list of
+// (row, records, first record) entries.  The gather then takes list entries in
+// a stride over its workgroups (balanced whatever the rows' order) and reads
+// them with scalar loads.
+//
+// A wave's vector loads and stores share one completion counter on this
+// architecture: a wave that waits for a vector load also waits for every store
+// it issued before it, and a __syncthreads() (a release fence) waits for all of
+// its stores.  So the 512-thread workgroup splits by role, with two LDS row
+// buffers and an LDS-only barrier:
+//   waves 0-3 (loaders) add the records of entry k into buffer k & 1 with LDS
+//   float atomics, after requesting the records of entry k + 1 (every lane
+//   loads, so the number of loads in flight is fixed and the wait for entry k's
+//   records does not wait for them); they store nothing to global memory;
+//   waves 4-7 (writers) meanwhile read buffer (k - 1) & 1, zero it and store it
+//   non-temporally; their only global loads are scalar, so nothing waits for
+//   their stores.
+// overwrite: the listed rows (mask[row] == 1) are written in full, so they need
+// no zero-fill beforehand; other rows are not touched.  accumulate: the listed
+// rows (those with records) get their sums added.
+struct __align__(16) GatherEntry {
+    int32_t row, n;  // texture row, records actually written for it
+    int64_t start;   // its first record
+};
+
+__global__ __launch_bounds__(256) void gather_list_kernel(int64_t N, const uint8_t *__restrict__ mask,
+                                                          const int32_t *__restrict__ cnt,
+                                                          const int32_t *__restrict__ cur,
+                                                          const int64_t *__restrict__ incl,
+                                                          GatherEntry *__restrict__ list, int64_t cap,
+                                                          uint32_t *__restrict__ count) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool sel = false;
+    int32_t n = 0;
+    if (r < N) {
+        n = min(cnt[r], cur[r]);  // records actually written
+        sel = mask ? mask[r] == 1 : n > 0;
+    }
+    const uint64_t bits = __ballot(sel);
+    if (bits == 0) return;
+    uint32_t base = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) base = atomicAdd(count, (uint32_t)__popcll(bits));
+    base = __builtin_amdgcn_readfirstlane(base);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bits >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bits, 0u));
+    // the list has room for every row a mask of tgs_mark_rows can select; rows past `cap`
+    // (another mask) are not listed, and the gather reads at most `cap` entries
+    if (sel && (int64_t)base + rank < cap) list[base + rank] = GatherEntry{(int32_t)r, n, r ? incl[r - 1] : 0};
+}
+
+__device__ __forceinline__ GatherEntry load_entry(cptr<GatherEntry> L, int32_t k) {
+    GatherEntry g;
+    g.row = L[k].row;
+    g.n = L[k].n;
+    g.start = L[k].start;
+    return g;
+}
+
+// Loader lanes take runs of kGatherRun consecutive records of a row.  Consecutive
+// records are neighbouring pixels of one Gaussian (the backward writes them in pixel
+// order), and where the splat is magnified they hit the same four texels: the run's
+// weighted cotangents are summed in registers while the texel quad stays the same, and
+// LDS float atomics (which serialize on equal addresses) are issued only when it changes.
+template <typename S> constexpr int gather_run() { return sizeof(S) == 4 ? 4 : 2; }  // registers: 32 / 32 dwords
+template <typename S> struct RecRun {
+    TexRecord<S> r[gather_run<S>()];
+};
+
 template <typename S>
-__global__ __launch_bounds__(512) void tex_gather_kernel(int64_t N, int64_t rows_per_block, int32_t tex_w,
-                                                         int64_t row_elems, const int64_t *__restrict__ rec_incl,
-                                                         const int32_t *__restrict__ rec_cnt,
-                                                         const int32_t *__restrict__ rec_cur,
-                                                         const TexRecord<S> *__restrict__ recs,
-                                                         const uint8_t *__restrict__ mask, int overwrite,
+__device__ __forceinline__ void load_run(RecRun<S> &run, const TexRecord<S> *recs, const GatherEntry &g,
+                                         int32_t first) {
+    // every lane loads kGatherRun records (record 0 where it has none), so the number of
+    // loads in flight is fixed for the waits that follow
+#pragma unroll
+    for (int j = 0; j < gather_run<S>(); ++j) {
+        const int32_t q = first + j;
+        run.r[j] = recs[q < g.n ? g.start + q : 0];
+    }
+}
+
+template <typename S>
+__device__ __forceinline__ void flush_quad(S *acc, const uint32_t e[4], const S sum[16]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (sum[4 * k + c] != S(0)) atomicAdd(acc + e[k] + c, sum[4 * k + c]);
+}
+
+// records first .. first + gather_run<S>() - 1 (those below n) of a row into the LDS row acc
+template <typename S>
+__device__ __forceinline__ void add_run(S *acc, const RecRun<S> &run, int32_t first, int32_t n, int32_t tex_w) {
+    const uint32_t tb = 4 * sizeof(S);
+    S sum[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) sum[k] = S(0);
+    uint32_t e[4] = {0, 0, 0, 0};
+    uint32_t key = 0xffffffffu;  // texel (u0, v0) and the two neighbour flags of the summed quad
+#pragma unroll
+    for (int j = 0; j < gather_run<S>(); ++j) {
+        if (first + j >= n) break;
+        const TexRecord<S> &rec = run.r[j];
+        Bilerp<S> bl;
+        bl.wu = rec.wu;
+        bl.wv = rec.wv;
+        bl.o00 = rec.o00;
+        bl.du = (rec.flags & 1u) ? tb : 0u;
+        bl.dv = (rec.flags & 2u) ? (uint32_t)tex_w * tb : 0u;
+        const uint32_t k2 = (rec.o00 << 2) | (rec.flags & 3u);
+        if (k2 != key) {
+            if (key != 0xffffffffu) {
+                flush_quad(acc, e, sum);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) sum[k] = S(0);
+            }
+            key = k2;
+            e[0] = bl.o00 / sizeof(S);
+            e[1] = (bl.o00 + bl.du) / sizeof(S);
+            e[2] = (bl.o00 + bl.dv) / sizeof(S);
+            e[3] = (bl.o00 + bl.du + bl.dv) / sizeof(S);
+        }
+        const BWeights<S> bw = bweights(bl);
+        const S w[4] = {bw.w00, bw.w10, bw.w01, bw.w11};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) sum[4 * k + c] += w[k] * rec.q[c];
+    }
+    if (key != 0xffffffffu) flush_quad(acc, e, sum);
+}
+
+// Workgroup barrier for LDS traffic only: this wave's LDS operations complete, then
+// s_barrier.  Unlike __syncthreads() it does not wait for the wave's global stores.
+// The empty asm statements keep the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    asm volatile("" ::: "memory");
+}
+
+template <typename S>
+__global__ __launch_bounds__(512, 4) void tex_gather_kernel(int32_t tex_w, int64_t row_vec,
+                                                         const GatherEntry *__restrict__ list, int64_t cap,
+                                                         const uint32_t *__restrict__ count,
+                                                         const TexRecord<S> *__restrict__ recs, int accumulate,
                                                          S *__restrict__ g_tex) {
     typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
     extern __shared__ __align__(16) char smem[];
-    const int64_t row_vec = row_elems * (int64_t)sizeof(S) / 16;  // 16-byte units per row
-    // two row buffers, addressed from the LDS base every time so accesses stay ds_* (a
-    // pointer kept in a register would be generic and turn them into flat accesses)
+    // two row buffers, addressed from the LDS base every time so accesses stay ds_*
     S *const acc = (S *)smem;
     u32x4_t *const acc4 = (u32x4_t *)smem;
-    const int t = threadIdx.x, tt = t & 255;
-    const bool writer = t >= 256;  // uniform per wave
-    const uint32_t tb = 4 * sizeof(S);
-    cptr<int32_t> cnt = as_const(rec_cnt), cur = as_const(rec_cur);
-    cptr<int64_t> incl = as_const(rec_incl);
+    constexpr int kT = 256;                     // threads per role
+    const int64_t e = 16 / (int64_t)sizeof(S);  // elements per 16-byte piece
+    const int t = threadIdx.x, tt = t & (kT - 1);
+    const bool writer = t >= kT;  // uniform per wave
+    cptr<GatherEntry> L = as_const(list);
+    const int64_t total = min((int64_t)as_const(count)[0], cap);
+    const int64_t G = gridDim.x;
+    for (int64_t i = t; i < 2 * row_vec; i += 2 * kT) acc4[i] = u32x4_t{0, 0, 0, 0};
 
-    // writer waves: half h of row `row` out of buffer b
-    auto write_half = [&](int b, int64_t row, int h) {
-        const int64_t lo = h ? row_vec / 2 : 0, hi = h ? row_vec : row_vec / 2;
-        if (overwrite) {
-            u32x4_t *dst = (u32x4_t *)(g_tex + row * row_elems);
-            for (int64_t i = lo + tt; i < hi; i += 256) __builtin_nontemporal_store(acc4[b * row_vec + i], dst + i);
-        } else {
-            S *d = g_tex + row * row_elems;
-            const int64_t e = 16 / (int64_t)sizeof(S);
-            for (int64_t i = lo * e + tt; i < hi * e; i += 256) d[i] += acc[b * row_elems + i];
-        }
-    };
-
-    const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
-    const int64_t r_end = min(N, r_begin + rows_per_block);
-    int64_t prev = -1;
-    int k = 0;
-    for (int64_t row = r_begin; row < r_end; ++row) {
-        const int32_t nrec = min(cnt[row], cur[row]);  // records actually written
-        bool skip = nrec == 0;
-        if (overwrite) {
-            // the mask byte through a scalar load of its (4-byte aligned) word, but the last
-            // partial word, which may end the buffer
-            const uint32_t word = (row | 3) < N ? as_const((const uint32_t *)mask)[row >> 2]
-                                                : (uint32_t)mask[row] << (8 * (row & 3));
-            skip = ((word >> (8 * (row & 3))) & 0xffu) == 0;
-        }
-        if (skip) continue;  // uniform over the workgroup
-        const int bc = k & 1, bp = bc ^ 1;
-        if (writer) {
-            if (prev >= 0) write_half(bp, prev, 0);
-        } else {
-            for (int64_t i = tt; i < row_vec; i += 256) acc4[bc * row_vec + i] = u32x4_t{0, 0, 0, 0};
-        }
-        __syncthreads();
-        if (writer) {
-            if (prev >= 0) write_half(bp, prev, 1);
-        } else {
-            const TexRecord<S> *rr = recs + (row ? incl[row - 1] : 0);
-            for (int32_t r = tt; r < nrec; r += 256) {
-                const TexRecord<S> rec = rr[r];
-                Bilerp<S> bl;
-                bl.wu = rec.wu;
-                bl.wv = rec.wv;
-                bl.o00 = rec.o00;
-                bl.du = (rec.flags & 1u) ? tb : 0u;
-                bl.dv = (rec.flags & 2u) ? (uint32_t)tex_w * tb : 0u;
-                const BWeights<S> bw = bweights(bl);
-                const uint32_t e00 = bl.o00 / sizeof(S), e10 = (bl.o00 + bl.du) / sizeof(S);
-                const uint32_t e01 = (bl.o00 + bl.dv) / sizeof(S), e11 = (bl.o00 + bl.du + bl.dv) / sizeof(S);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const S qc = rec.q[c];
-                    if (qc == S(0)) continue;
-                    const int64_t o = bc * row_elems + c;
-                    atomicAdd(acc + o + e00, bw.w00 * qc);
-                    atomicAdd(acc + o + e10, bw.w10 * qc);
-                    atomicAdd(acc + o + e01, bw.w01 * qc);
-                    atomicAdd(acc + o + e11, bw.w11 * qc);
+    // loaders: entry k of this workgroup is list[blockIdx.x + k G]; each loader lane takes
+    // the run of R = gather_run<S>() records at R tt of each chunk of kT R records.  The
+    // first chunk of the current and the next entry swap roles between the two halves of
+    // the unrolled loop (copying a register whose load is in flight would wait for it).
+    constexpr int32_t kR = gather_run<S>(), kChunk = kT * kR;
+    RecRun<S> ra, rb;
+    int32_t ek = blockIdx.x;  // wave-uniform; kept in an SGPR so the list reads are scalar
+    GatherEntry cur{-1, 0, 0};
+    if (!writer) {
+        if (ek < total) cur = load_entry(L, ek);
+        load_run(ra, recs, cur, tt * kR);
+    }
+    lds_barrier();
+    // step s: loaders add entry s into buffer s & 1; writers drain entry s - 1 from the
+    // other buffer.  The loop ends after the step in which the loaders had no entry.
+    auto step = [&](int64_t s, RecRun<S> &rc, RecRun<S> &rn) -> bool {
+        const int b = (int)(s & 1);
+        if (!writer) {
+            const int32_t e2 = __builtin_amdgcn_readfirstlane(ek + (int32_t)G);
+            GatherEntry nxt{-1, 0, 0};
+            if (e2 < total) nxt = load_entry(L, e2);
+            load_run(rn, recs, nxt, tt * kR);
+            if (ek < total) {
+                S *ab = acc + b * row_vec * e;
+                add_run(ab, rc, tt * kR, cur.n, tex_w);
+                // rows of more than kChunk records: further chunks in rc's registers (its
+                // first chunk is added; the next step loads into it afresh)
+                for (int32_t q = kChunk + tt * kR; q < cur.n; q += kChunk) {
+                    load_run(rc, recs, cur, q);
+                    add_run(ab, rc, q, cur.n, tex_w);
+                }
+            }
+            ek = e2;
+            cur = nxt;
+        } else if (s >= 1) {
+            const int32_t ew = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x + (s - 1) * G));
+            if (ew < total) {
+                const int64_t r = L[ew].row;
+                const int64_t o = (int64_t)(b ^ 1) * row_vec;
+                if (!accumulate) {
+                    u32x4_t *dst = (u32x4_t *)(g_tex + r * row_vec * e);
+#pragma unroll 4
+                    for (int64_t i = tt; i < row_vec; i += kT) {
+                        const u32x4_t v = acc4[o + i];
+                        acc4[o + i] = u32x4_t{0, 0, 0, 0};
+                        __builtin_nontemporal_store(v, dst + i);
+                    }
+                } else {
+                    // accumulate mode reads the destination: these writers do wait for loads
+                    S *d = g_tex + r * row_vec * e;
+                    for (int64_t i = tt; i < row_vec * e; i += kT) {
+                        d[i] += acc[o * e + i];
+                        acc[o * e + i] = S(0);
+                    }
                 }
             }
         }
-        __syncthreads();
-        prev = row;
-        ++k;
-    }
-    if (writer && prev >= 0) {
-        write_half((k + 1) & 1, prev, 0);
-        write_half((k + 1) & 1, prev, 1);
+        lds_barrier();
+        return blockIdx.x + s * G >= total;  // the loaders had no entry in this step
+    };
+    for (int64_t s = 0;; s += 2) {
+        if (step(s, ra, rb) || step(s + 1, rb, ra)) break;
     }
 }
 
@@ -1366,13 +1499,15 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
                         const int32_t *flatten_ids, int64_t n_isects, double g_weight, void *workspace,
                         void *zdst, const uint8_t *zmask, size_t zrow_bytes, int zsel, const int64_t *gids,
                         int64_t nnz) {
+    // zsel: low byte the mask value to fill, bits 8-15 the rows (by row & 7) to take; 0 there = all
     memset(&a, 0, sizeof(a));
     a.gids = gids;
     a.E = gids ? nnz : C * N;
     a.zdst = zdst;
     a.zmask = zmask;
     a.zrow16 = (int64_t)(zrow_bytes / 16);
-    a.zsel = zsel;
+    a.zsel = zsel & 0xff;
+    a.zres = (zsel >> 8) & 0xff ? (zsel >> 8) & 0xff : 0xff;
     a.C = C; a.N = N; a.n_isects = n_isects;
     a.means2d = (const S *)means2d; a.rt = (const S *)rt; a.colors = (const S *)colors;
     a.opac = (const S *)opac; a.normals = (const S *)normals; a.textures = (const S *)textures;
@@ -1442,6 +1577,7 @@ static int check_zero_rows(const void *dst, const uint8_t *mask, size_t row_byte
 
 constexpr size_t kGatherMaxLds = 160 * 1024;  // LDS per workgroup: two texture rows in the gather
 
+// two texture rows fit in one CU's LDS
 template <typename S> static bool gather_fits(int64_t tex_h, int64_t tex_w) {
     return 2 * (size_t)(tex_h * tex_w * 4) * sizeof(S) <= kGatherMaxLds;
 }
@@ -1449,22 +1585,32 @@ template <typename S> static bool gather_fits(int64_t tex_h, int64_t tex_w) {
 template <typename S>
 static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterArgs<S> &a, const uint8_t *mask,
                          hipStream_t st) {
-    const int64_t row_elems = tex_h * tex_w * 4;
-    const size_t lds = 2 * (size_t)row_elems * sizeof(S);
+    const int64_t row_bytes = tex_h * tex_w * 4 * (int64_t)sizeof(S);
+    const size_t lds = 2 * (size_t)row_bytes;  // two row buffers
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute((const void *)tex_gather_kernel<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess) {
         set_error("rasterize_bwd: cannot reserve %zu bytes of LDS for the texture gather", lds);
         return TGS_ERR_LAUNCH;
     }
+    // the row list goes where the per-entry records were: the backward kernel, their only
+    // reader, has run (16 bytes per row; with the mask of tgs_mark_rows the rows to write
+    // are at most the entries, 80 bytes each or more)
+    if (a.E == 0) return TGS_OK;  // no entries: no listed rows, no records
+    uint32_t *count = (uint32_t *)a.rec;
+    GatherEntry *list = (GatherEntry *)((char *)a.rec + 16);
+    const int64_t cap = ((int64_t)rec_bytes<S>(1, a.E) - 16) / (int64_t)sizeof(GatherEntry);
+    zero_kernel<uint32_t><<<1, 64, 0, st>>>(1, count);
+    if (N > 0)
+        gather_list_kernel<<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(N, mask, a.rec_cnt, a.rec_cur, a.rec_incl, list,
+                                                                       cap, count);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, kGatherMaxLds / std::max<size_t>(lds, 1)));
+    // as many 512-thread workgroups per CU as their buffers allow (at most 2: 16 waves)
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, kGatherMaxLds / std::max<size_t>(lds, 1)));
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
-    const int64_t rows_per_block = (N + blocks - 1) / blocks;
-    tex_gather_kernel<S><<<(unsigned)blocks, 512, lds, st>>>(N, rows_per_block, (int32_t)tex_w, row_elems, a.rec_incl,
-                                                             a.rec_cnt, a.rec_cur, (const TexRecord<S> *)a.recs, mask,
-                                                             mask ? 1 : 0, a.g_tex);
+    tex_gather_kernel<S><<<(unsigned)blocks, 512, lds, st>>>((int32_t)tex_w, row_bytes / 16, list, cap, count,
+                                                             (const TexRecord<S> *)a.recs, mask ? 0 : 1, a.g_tex);
     TGS_CHECK_LAUNCH("rasterize_bwd texture gather");
     return TGS_OK;
 }
@@ -1531,9 +1677,11 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  double g_weight, void *render_colors, void *render_alphas, void *render_normals,
                                  void *render_distort, void *render_median, int32_t *last_ids, int32_t *median_ids,
                                  void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                                 uint8_t *live_entries, int32_t *tex_counts, void *zero_rows_dst,
+                                 void *render_trans, uint8_t *live_entries, int32_t *tex_counts,
+                                 void *zero_rows_dst,
                                  const uint8_t *zero_rows_mask,
-                                 size_t zero_row_bytes, void *workspace, size_t workspace_size, tgs_stream_t stream) {
+                                 size_t zero_row_bytes, int zero_rows_select, void *workspace, size_t workspace_size,
+                                 tgs_stream_t stream) {
     const int tex_mode = tex_mode_of(textures, dtype);
     dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2);
     TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_fwd: bfloat16 textures need float32 compute");
@@ -1555,13 +1703,14 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
                        tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
-                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 1, gaussian_ids, nnz);                    \
+                       zero_rows_dst, zero_rows_mask, zero_row_bytes, zero_rows_select, gaussian_ids, nnz);     \
         set_mip<S>(a, tex_mode, tex_h, tex_w);                                                                 \
         a.thr = (S)gs_contrib_threshold;                                                                        \
         a.out_c = (S *)render_colors; a.out_a = (S *)render_alphas; a.out_n = (S *)render_normals;              \
         a.out_d = (S *)render_distort; a.out_m = (S *)render_median; a.last_ids = last_ids;                     \
         a.median_ids = median_ids; a.csum = (S *)contrib_sum; a.ccnt = (S *)contrib_count;                      \
         a.out_accd = (S *)render_accum_depth;                                                                   \
+        a.out_T = (S *)render_trans;                                                                            \
         a.live = live_entries;                                                                                  \
         a.tex_cnt = textures ? tex_counts : nullptr;                                                            \
         return dispatch<S, false>(a, K, tex_mode, st);                                                          \
@@ -1583,15 +1732,16 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const int32_t *flatten_ids, int64_t n_isects, double g_weight,
                                  const void *render_colors, const void *render_alphas, const int32_t *last_ids,
                                  const int32_t *median_ids, const void *render_accum_depth,
-                                 const uint8_t *live_entries, const int32_t *tex_counts,
+                                 const void *render_trans, const uint8_t *live_entries,
+                                 const int32_t *tex_counts,
                                  const int64_t *tex_rec_incl, int32_t *tex_rec_cursor, void *tex_records,
                                  const void *v_render_colors, const void *v_render_alphas,
                                  const void *v_render_normals, const void *v_render_distort,
                                  const void *v_render_median, int uv_grad_mode, void *v_means2d_abs, void *v_means2d,
                                  void *v_ray_transforms, void *v_colors, void *v_opacities, void *v_textures,
                                  void *v_normals, void *v_densify, void *zero_rows_dst,
-                                 const uint8_t *zero_rows_mask, size_t zero_row_bytes, void *workspace,
-                                 size_t workspace_size, tgs_stream_t stream) {
+                                 const uint8_t *zero_rows_mask, size_t zero_row_bytes, int zero_rows_select,
+                                 void *workspace, size_t workspace_size, tgs_stream_t stream) {
     const int tex_mode = tex_mode_of(textures, dtype);
     dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2);
     TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_bwd: bfloat16 textures need float32 compute");
@@ -1623,11 +1773,12 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         fill_common<S>(a, C, N, means2d, ray_transforms, colors, opacities, normals, textures, tex_h, tex_w,     \
                        texture_range_x, texture_range_y, backgrounds, masks, width, height, tile_size,          \
                        tile_width, tile_height, tile_offsets, flatten_ids, n_isects, g_weight, workspace,        \
-                       zero_rows_dst, zero_rows_mask, zero_row_bytes, 0, gaussian_ids, nnz);                    \
+                       zero_rows_dst, zero_rows_mask, zero_row_bytes, zero_rows_select, gaussian_ids, nnz);     \
         set_mip<S>(a, tex_mode, tex_h, tex_w);                                                                 \
         a.r_c = (const S *)render_colors; a.r_a = (const S *)render_alphas; a.r_last = last_ids;                \
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \
+        a.r_T = (const S *)render_trans;                                                                        \
         a.live = (uint8_t *)live_entries;                                                                       \
         const bool recs = textures && tex_records && tex_counts && tex_rec_incl && tex_rec_cursor &&           \
                           gather_fits<S>(tex_h, tex_w) &&                                                        \
@@ -1652,7 +1803,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         if (n_isects > 0) {                                                                                     \
             rc = dispatch<S, true>(a, K, tex_mode, st);                                                         \
         } else if (zero_rows_dst) {                                                                             \
-            rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, 0, st);                   \
+            rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, zero_rows_select & 0xff, st);\
         }                                                                                                       \
         if (rc) return rc;                                                                                      \
         if (recs && !records_only) {                                                                            \

@@ -55,7 +55,8 @@ _PROTOS = {
     "tgs_sh_bwd_cams": (c_int, [c_int, c_i64, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
     "tgs_rasterize_fwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
                                   vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl, c_dbl,
-                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, c_int, vp,
+                                  c_size, vp]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
     "tgs_tex_record_bytes": (c_size, [c_int]),
     "tgs_mipmap_texels": (c_i64, [c_i32]),
@@ -71,8 +72,8 @@ _PROTOS = {
     "tgs_prefix_sum_i32": (c_int, [c_i64, vp, vp, vp, c_size, vp]),
     "tgs_rasterize_bwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
                                   vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl,
-                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
-                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, vp, c_size, vp]),
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, c_int, vp, c_size, vp]),
     "tgs_gemm_bf16": (c_int, [c_int, c_int, c_int, c_i64, c_i64, c_i64, vp, c_i64, vp, c_i64, vp, vp, c_i64, vp,
                               c_i64, c_i32, vp]),
     "tgs_sigmoid_bwd": (c_int, [c_i64, vp, vp, vp, vp]),

@@ -387,15 +387,19 @@ def _raster_workspace(dt, C, N, dev):
     return torch.empty((nbytes,), dtype=torch.uint8, device=dev)
 
 
-def _zero_rows_args(zero_rows, N):
-    """(dst ptr, mask ptr, row bytes) for the fused row fill, or nulls."""
+def _zero_rows_args(zero_rows, N, select):
+    """(dst ptr, mask ptr, row bytes, selection) for the fused row fill, or nulls.
+    ``zero_rows`` = (dst, mask) or (dst, mask, selection): the rows with mask value
+    ``selection & 0xff`` are filled, of those only the rows r with bit (r & 7) of
+    ``selection >> 8`` set when that byte is nonzero; ``select`` is the default."""
     if zero_rows is None:
-        return None, None, 0
-    dst, mask = zero_rows
+        return None, None, 0, 0
+    dst, mask = zero_rows[0], zero_rows[1]
+    sel = int(zero_rows[2]) if len(zero_rows) > 2 else select
     _check_input(dst, "zero_rows dst")
     _check_input(mask, "zero_rows mask")
     assert dst.shape[0] == N and mask.numel() >= N and mask.dtype == torch.uint8
-    return _p(dst), _p(mask), dst[0].numel() * dst.element_size()
+    return _p(dst), _p(mask), dst[0].numel() * dst.element_size(), sel
 
 
 def _tex_dims(textured, textures, mip_log_res):
@@ -472,8 +476,9 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         csum = torch.zeros((N,), dtype=fdt, device=dev)
         ccnt = torch.zeros((N,), dtype=fdt, device=dev)
         accd = torch.empty((C, H, W), dtype=fdt, device=dev)
+        trans = torch.empty((C, H, W), dtype=fdt, device=dev)
         ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
-        zd, zm, zb = _zero_rows_args(zero_rows, N)
+        zd, zm, zb, zs = _zero_rows_args(zero_rows, N, 1)
         with _timed("raster_fwd", means2d):
             check(lib().tgs_rasterize_fwd(
               dt | tex_flag, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
@@ -481,11 +486,11 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
               _p(masks.view(torch.uint8)) if masks is not None else None, _p(gids), nnz, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
-              _p(csum), _p(ccnt), _p(accd), _live_arg(live_entries, flatten_ids, tile_size),
-              _counts_arg(tex_counts, textures.shape[0] if textured else 0), zd, zm, zb, _p(ws),
+              _p(csum), _p(ccnt), _p(accd), _p(trans), _live_arg(live_entries, flatten_ids, tile_size),
+              _counts_arg(tex_counts, textures.shape[0] if textured else 0), zd, zm, zb, zs, _p(ws),
               ws.numel(), _stream(means2d)),
               "rasterize_to_pixels_fwd")
-    return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd
+    return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd, trans
 
 
 def rasterize_to_pixels_fwd_textured_gaussians(means2d, ray_transforms, colors, opacities, textures,
@@ -529,19 +534,43 @@ def _counts_arg(tex_counts, N):
     return _p(tex_counts)
 
 
+class TexRecordPlan:
+    """Sizing of the texture-gradient records, started right after the forward: the
+    inclusive prefix sum of the forward's per-row pair counts, and its total copied to
+    pinned host memory asynchronously, so the backward reads it without waiting for
+    the device (no reference counterpart)."""
+
+    def __init__(self, tex_counts: Tensor):
+        N = tex_counts.numel()
+        dev = tex_counts.device
+        self.counts = tex_counts
+        self.incl = torch.empty(N, dtype=torch.int64, device=dev)
+        temp = torch.empty(max(1, lib().tgs_isect_count_temp_bytes(N)), dtype=torch.uint8, device=dev)
+        check(lib().tgs_prefix_sum_i32(N, _p(tex_counts), _p(self.incl), _p(temp), temp.numel(), _stream(tex_counts)),
+              "prefix_sum")
+        self.total_host = torch.zeros(1, dtype=torch.int64, pin_memory=True)
+        if N > 0:
+            self.total_host.copy_(self.incl[-1:], non_blocking=True)
+        self.done = torch.cuda.Event()
+        self.done.record(torch.cuda.current_stream(dev))
+        self._temp = temp  # kept until the copy has run
+
+    def total(self) -> int:
+        self.done.synchronize()
+        return int(self.total_host[0])
+
+
 def _tex_record_args(tex_counts, dt, N, dev):
-    """(counts, inclusive offsets, cursor, records) for the backward's record path."""
+    """(counts, inclusive offsets, cursor, records) for the backward's record path;
+    ``tex_counts`` is the forward's count tensor or a ``TexRecordPlan`` of it."""
     if tex_counts is None:
         return None, None, None, None, None
-    _counts_arg(tex_counts, N)
-    incl = torch.empty(N, dtype=torch.int64, device=dev)
-    temp = torch.empty(max(1, lib().tgs_isect_count_temp_bytes(N)), dtype=torch.uint8, device=dev)
-    check(lib().tgs_prefix_sum_i32(N, _p(tex_counts), _p(incl), _p(temp), temp.numel(), _stream(tex_counts)),
-          "prefix_sum")
-    total = int(incl[-1].item()) if N > 0 else 0  # one device->host read, like isect_tiles
+    plan = tex_counts if isinstance(tex_counts, TexRecordPlan) else TexRecordPlan(tex_counts)
+    _counts_arg(plan.counts, N)
+    total = plan.total()
     recs = torch.empty(max(1, total) * lib().tgs_tex_record_bytes(dt), dtype=torch.uint8, device=dev)
     cursor = torch.zeros(N, dtype=torch.int32, device=dev)
-    return tex_counts, incl, cursor, recs, temp
+    return plan.counts, plan.incl, cursor, recs, None
 
 
 def texture_adam_rows(records, visibility, tex_hi, tex_lo, exp_avg, exp_avg_sq, lr, b1, b2, eps, seed):
@@ -569,7 +598,8 @@ def texture_adam_rows(records, visibility, tex_hi, tex_lo, exp_avg, exp_avg_sq, 
 def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None, tex_counts=None,
                                                     gaussian_ids=None):
     """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
-    per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W].
+    per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W], and an 11th,
+    the final transmittance T [C,H,W] (for the backward's ``render_trans``).
     ``zero_rows`` = (buffer [N, ...], uint8 mask [N]): rows with mask 1 are
     zero-filled by the forward kernel (see ``mark_rows``).  ``live_entries``
     (from ``live_entries_buffer``) receives the entries each wavefront composited,
@@ -602,7 +632,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
                 v_textures_out=None, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None,
-                n_gaussians=None, mip_log_res=None):
+                n_gaussians=None, mip_log_res=None, render_trans=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -648,7 +678,9 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
         v_densify = torch.zeros_like(densify)
         ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
-        zd, zm, zb = _zero_rows_args(zero_rows, N)
+        zd, zm, zb, zs = _zero_rows_args(zero_rows, N, 0)
+        if render_trans is not None:
+            _check_input(render_trans, "render_trans")
         rc_, ri_, rcur_, rrec_, _tmp = _tex_record_args(tex_counts if textured else None, dt, textures.shape[0]
                                                         if textured else 0, dev)
         with _timed("raster_bwd", means2d):
@@ -659,12 +691,13 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
               image_height, tile_size, tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(),
               float(g_weight),
               _p(render_colors), _p(render_alphas), _p(last_ids), _p(median_ids), _p(render_accum_depth),
-              _live_arg(live_entries, flatten_ids, tile_size), _p(rc_), _p(ri_), _p(rcur_), _p(rrec_),
+              _p(render_trans), _live_arg(live_entries, flatten_ids, tile_size), _p(rc_), _p(ri_), _p(rcur_),
+              _p(rrec_),
               _p(v_render_colors),
               _p(v_render_alphas), _p(v_render_normals), _p(v_render_distort), _p(v_render_median), uv_mode,
               _p(v_means2d_abs), _p(v_means2d), _p(v_rt), _p(v_colors), _p(v_opacities), _p(v_textures),
               _p(v_normals), _p(v_densify) if densify.shape == means2d.shape else None,
-              zd, zm, zb, _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_bwd")
+              zd, zm, zb, zs, _p(ws), ws.numel(), _stream(means2d)), "rasterize_to_pixels_bwd")
     if records_only:
         # (counts, inclusive offsets, cursor, records) in place of v_textures
         return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, (rc_, ri_, rcur_, rrec_), v_normals, v_densify
@@ -673,43 +706,38 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
     return v_means2d_abs, v_means2d, v_rt, v_colors, v_opacities, v_normals, v_densify
 
 
-def rasterize_to_pixels_bwd_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                               live_entries=None, tex_counts=None, gaussian_ids=None):
+# Keyword options of the textured backward ops (no reference counterparts):
+# render_accum_depth, render_trans, v_textures_out, zero_rows, live_entries,
+# tex_counts, gaussian_ids (see _raster_bwd and include/tgs_hip.h).
+def rasterize_to_pixels_bwd_textured_gaussians(*args, **opts):
     """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
-    return _raster_bwd(True, UV_NONE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
-                       tex_counts=tex_counts, gaussian_ids=gaussian_ids)
+    return _raster_bwd(True, UV_NONE, *args, **opts)
 
 
-def rasterize_to_pixels_bwd2_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                                live_entries=None, tex_counts=None, gaussian_ids=None):
+def rasterize_to_pixels_bwd2_textured_gaussians(*args, **opts):
     """Reference ``bilinear_bwd2`` rule (du/ds = (res-1)/range)."""
-    return _raster_bwd(True, UV_DOUBLE, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
-                       tex_counts=tex_counts, gaussian_ids=gaussian_ids)
+    return _raster_bwd(True, UV_DOUBLE, *args, **opts)
 
 
-def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, render_accum_depth=None, v_textures_out=None, zero_rows=None,
-                                                     live_entries=None, tex_counts=None, gaussian_ids=None):
+def rasterize_to_pixels_bwd_exact_textured_gaussians(*args, **opts):
     """True derivative: du/ds = (res-1)/(2 range), and the forward's own
     distortion accumulator when ``render_accum_depth`` is given (no reference
     counterpart)."""
-    return _raster_bwd(True, UV_EXACT, *args, render_accum_depth=render_accum_depth,
-                       v_textures_out=v_textures_out, zero_rows=zero_rows, live_entries=live_entries,
-                       tex_counts=tex_counts, gaussian_ids=gaussian_ids)
+    return _raster_bwd(True, UV_EXACT, *args, **opts)
 
 
 def rasterize_to_pixels_bwd_2dgs(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
                                  image_width, image_height, tile_size, tile_offsets, flatten_ids, render_colors,
                                  render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                                  v_render_normals, v_render_distort, v_render_median, absgrad,
-                                 render_accum_depth=None, live_entries=None, gaussian_ids=None, n_gaussians=None):
+                                 render_accum_depth=None, live_entries=None, gaussian_ids=None, n_gaussians=None,
+                                 render_trans=None):
     return _raster_bwd(False, UV_NONE, means2d, ray_transforms, colors, opacities, None, 1.0, 1.0, normals,
                        densify, backgrounds, masks, image_width, image_height, tile_size, tile_offsets,
                        flatten_ids, 1.0, render_colors, render_alphas, last_ids, median_ids, v_render_colors,
                        v_render_alphas, v_render_normals, v_render_distort, v_render_median, absgrad,
                        render_accum_depth=render_accum_depth, live_entries=live_entries, gaussian_ids=gaussian_ids,
-                       n_gaussians=n_gaussians)
+                       n_gaussians=n_gaussians, render_trans=render_trans)
 
 
 # ---------------------------------------------------------------- mipmaps
@@ -757,10 +785,23 @@ def rasterize_to_pixels_fwd_mip2_textured_gaussians(means2d, ray_transforms, col
                                                     live_entries=None, zero_rows=None):
     """The ``mipmapped2`` forward (ext.cpp:136-137) over the pyramid ``textures``
     [N, (4^(L+1) - 1) / 3, 4]; returns the 9-tuple of the bilinear forward."""
+    return rasterize_to_pixels_fwd_mip2_textured_gaussians_full(
+        means2d, ray_transforms, colors, opacities, textures, log_texture_res, texture_range_x, texture_range_y,
+        normals, backgrounds, masks, image_width, image_height, tile_size, tile_offsets, flatten_ids,
+        gs_contrib_threshold, g_weight, live_entries=live_entries, zero_rows=zero_rows)[:9]
+
+
+def rasterize_to_pixels_fwd_mip2_textured_gaussians_full(means2d, ray_transforms, colors, opacities, textures,
+                                                         log_texture_res, texture_range_x, texture_range_y, normals,
+                                                         backgrounds, masks, image_width, image_height, tile_size,
+                                                         tile_offsets, flatten_ids, gs_contrib_threshold, g_weight,
+                                                         live_entries=None, zero_rows=None):
+    """As above plus the distortion accumulator and the final transmittance (11 outputs,
+    as ``rasterize_to_pixels_fwd_textured_gaussians_full``)."""
     return _raster_fwd(True, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                        texture_range_y, normals, backgrounds, masks, image_width, image_height, tile_size,
                        tile_offsets, flatten_ids, gs_contrib_threshold, g_weight, live_entries=live_entries,
-                       zero_rows=zero_rows, mip_log_res=log_texture_res)[:9]
+                       zero_rows=zero_rows, mip_log_res=log_texture_res)
 
 
 def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, colors, opacities, textures,
@@ -770,7 +811,7 @@ def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, col
                                                     render_alphas, last_ids, median_ids, v_render_colors,
                                                     v_render_alphas, v_render_normals, v_render_distort,
                                                     v_render_median, absgrad, live_entries=None, v_textures_out=None,
-                                                    zero_rows=None):
+                                                    zero_rows=None, render_trans=None):
     """The ``mipmapped2`` backward (ext.cpp:139-140): the reference's rule (no
     d(sample)/d(position) term); v_textures has the pyramid's shape."""
     return _raster_bwd(True, UV_NONE, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
@@ -778,7 +819,7 @@ def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, col
                        tile_offsets, flatten_ids, g_weight, render_colors, render_alphas, last_ids, median_ids,
                        v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
                        absgrad, live_entries=live_entries, v_textures_out=v_textures_out, zero_rows=zero_rows,
-                       mip_log_res=log_texture_res)
+                       mip_log_res=log_texture_res, render_trans=render_trans)
 
 
 # ------------------------------------------------------------------- SSIM

@@ -356,9 +356,20 @@ def rasterize_to_pixels_textured_gaussians(
 
 # Texture gradients through per-pair records and a per-row LDS gather (True), or
 # float atomics into a v_textures whose listed rows the forward zero-fills (False).
-# Measured on the 1M-Gaussian scene: atomics 21.9 ms/step, records 24.1 ms/step (the
-# gather is bound by per-row latency); at 16x16 textures 43.0 vs 45.7 ms.
+# Measured on the 1M-Gaussian scene (one MI355X): atomics 19.1-21.0 ms/step; records
+# 20.8-22.3 ms/step.  The gather streams the listed rows at 4.6 TB/s but its LDS float
+# atomics (71M records x 16 values) add about 3 ms, and the unlisted-row fill no longer
+# hides behind the atomic-bound backward.
 TEXTURE_GRAD_RECORDS = False
+# With records, the rows absent from the tile lists are zero-filled along the
+# rasterizer walks: the forward takes the rows whose index mod 8 is in this set
+# (bit r), the backward the others; the gather writes the listed rows.
+FWD_FILL_RESIDUES = 0xFF
+
+
+def _records_fit(textures: Tensor, dtype: torch.dtype) -> bool:
+    """The gather keeps one texture row in LDS: at most 80 KB."""
+    return textures.shape[1] * textures.shape[2] * 4 * torch.tensor([], dtype=dtype).element_size() <= 80 * 1024
 
 
 class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
@@ -367,12 +378,12 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
                 texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
                 flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight, gaussian_ids, texture_adam):
         # The texture gradient is as large as the textures (40 GB at 1M x 50x50) and is never
-        # zero-filled on its own.  The backward zeroes the rows of Gaussians absent from the
-        # tile lists along its walk.  The rows of listed Gaussians are zeroed by the forward
-        # along its walk and accumulated into by the backward, or, with records, written in
-        # full by the texture-gradient gather: the forward counts per row the pairs that will
-        # contribute, the backward writes one record per pair, the gather sums each row's
-        # records in LDS and writes the row once.
+        # zero-filled on its own.  With records (the default), the forward counts per row the
+        # pairs that will contribute, the backward writes one record per pair, and the gather
+        # sums each listed row's records in LDS and writes the row once; the rows absent from
+        # the tile lists are zeroed along the forward and backward walks.  Without records, the
+        # forward zeroes the listed rows along its walk, the backward accumulates into them
+        # with float atomics and zeroes the other rows.
         ctx.v_tex = None
         ctx.tex_counts = None
         ctx.adam = None
@@ -391,22 +402,31 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             # texture rows present in the tile lists (packed: the entries' Gaussians)
             listed = flatten_ids if gaussian_ids is None else gaussian_ids[flatten_ids.long()].int()
             rows = _C.mark_rows(listed, textures.shape[0])
-            ctx.v_tex = (v_tex, rows)
-            if TEXTURE_GRAD_RECORDS:
+            if TEXTURE_GRAD_RECORDS and _records_fit(textures, means2d.dtype):
                 ctx.tex_counts = _C.tex_counts_buffer(textures)
+                fwd_res = FWD_FILL_RESIDUES & 0xFF
+                if fwd_res:
+                    zero_rows = (v_tex, rows, fwd_res << 8)  # mask value 0, these residues
+                bwd_res = 0xFF & ~fwd_res
+                # the backward's share; mask value 2 (no row) when the forward took every residue
+                ctx.v_tex = (v_tex, rows, (bwd_res << 8) if bwd_res else 2)
             else:  # the forward zero-fills the listed rows and the backward adds in place
-                zero_rows = ctx.v_tex
+                zero_rows = (v_tex, rows, 1)
+                ctx.v_tex = (v_tex, rows, 0)
         # the forward records which list entries each wavefront composited; the backward
         # visits only those
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
-         gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func(fwd_fn + "_full")(
+         gs_contrib_sum, gs_contrib_count, accum_depth, trans) = _make_lazy_cuda_func(fwd_fn + "_full")(
             means2d, ray_transforms, colors, opacities, textures, texture_range_x, texture_range_y, normals,
             backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
             g_weight, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids)
+        if ctx.tex_counts is not None:
+            # record sizing starts now; the backward reads its total without waiting
+            ctx.tex_counts = _C.TexRecordPlan(ctx.tex_counts)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds,
                               masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
-                              median_ids, accum_depth, gaussian_ids)
+                              median_ids, accum_depth, gaussian_ids, trans)
         ctx.bwd_fn = bwd_fn
         ctx.texture_range = (texture_range_x, texture_range_y)
         ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
@@ -420,7 +440,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
                  v_gs_contrib_sum, v_gs_contrib_count):
         (means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds, masks, isect_offsets,
          flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth,
-         gaussian_ids) = ctx.saved_tensors
+         gaussian_ids, trans) = ctx.saved_tensors
         exact = ctx.bwd_fn == tgs_fns["bilinear_exact"][1]
         v_tex_buf, zero_rows = None, None
         if ctx.adam is not None:
@@ -436,8 +456,8 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             ctx.g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors.contiguous(),
             v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
             v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None,
-            v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts,
-            gaussian_ids=gaussian_ids)
+            render_trans=trans, v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live,
+            tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids)
         ctx.live = None
         ctx.tex_counts = None
         if ctx.adam is not None:
@@ -490,12 +510,14 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
             v_tex = torch.empty(mip_textures.shape, dtype=means2d.dtype, device=mip_textures.device)
             ctx.v_tex = (v_tex, _C.mark_rows(flatten_ids, mip_textures.shape[0]))
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
-         gs_contrib_sum, gs_contrib_count) = _make_lazy_cuda_func("rasterize_to_pixels_fwd_mip2_textured_gaussians")(
+         gs_contrib_sum, gs_contrib_count, _, trans) = _make_lazy_cuda_func(
+            "rasterize_to_pixels_fwd_mip2_textured_gaussians_full")(
             means2d, ray_transforms, colors, opacities, mip_textures, log_texture_res, texture_range_x,
             texture_range_y, normals, backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids,
             gs_contrib_threshold, g_weight, live_entries=ctx.live, zero_rows=ctx.v_tex)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, mip_textures, normals, densify, backgrounds,
-                              masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids)
+                              masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids,
+                              trans)
         ctx.log_texture_res = log_texture_res
         ctx.texture_range = (texture_range_x, texture_range_y)
         ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
@@ -508,7 +530,7 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
     def backward(ctx, v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
                  v_gs_contrib_sum, v_gs_contrib_count):
         (means2d, ray_transforms, colors, opacities, mip_textures, normals, densify, backgrounds, masks,
-         isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids) = ctx.saved_tensors
+         isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids, trans) = ctx.saved_tensors
         (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, v_normals,
          v_densify) = _make_lazy_cuda_func("rasterize_to_pixels_bwd_mip2_textured_gaussians")(
             means2d, ray_transforms, colors, opacities, mip_textures, ctx.log_texture_res, ctx.texture_range[0],
@@ -516,7 +538,7 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
             isect_offsets, flatten_ids, ctx.g_weight, render_colors, render_alphas, last_ids, median_ids,
             v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
             v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad, live_entries=ctx.live,
-            v_textures_out=ctx.v_tex[0] if ctx.v_tex is not None else None, zero_rows=ctx.v_tex)
+            v_textures_out=ctx.v_tex[0] if ctx.v_tex is not None else None, zero_rows=ctx.v_tex, render_trans=trans)
         ctx.live = None
         ctx.v_tex = None
         if ctx.absgrad:
@@ -583,12 +605,13 @@ class _RasterizeToPixels2DGS(torch.autograd.Function):
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
         ctx.packed = dict(gaussian_ids=gaussian_ids, n_gaussians=n_gaussians)
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
-         gs_contrib_sum, gs_contrib_count, accum_depth) = _make_lazy_cuda_func("rasterize_to_pixels_fwd_2dgs_full")(
+         gs_contrib_sum, gs_contrib_count, accum_depth, trans) = _make_lazy_cuda_func(
+            "rasterize_to_pixels_fwd_2dgs_full")(
             means2d, ray_transforms, colors, opacities, normals, backgrounds, masks, width, height, tile_size,
             isect_offsets, flatten_ids, gs_contrib_threshold, live_entries=ctx.live, **ctx.packed)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks,
                               isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids,
-                              accum_depth)
+                              accum_depth, trans)
         ctx.width, ctx.height, ctx.tile_size, ctx.absgrad = width, height, tile_size, absgrad
         ctx.exact = exact
         ctx.mark_non_differentiable(gs_contrib_sum, gs_contrib_count)
@@ -599,14 +622,15 @@ class _RasterizeToPixels2DGS(torch.autograd.Function):
     def backward(ctx, v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
                  v_gs_contrib_sum, v_gs_contrib_count):
         (means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks, isect_offsets,
-         flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth) = ctx.saved_tensors
+         flatten_ids, render_colors, render_alphas, last_ids, median_ids, accum_depth, trans) = ctx.saved_tensors
         (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_normals,
          v_densify) = _make_lazy_cuda_func("rasterize_to_pixels_bwd_2dgs")(
             means2d, ray_transforms, colors, opacities, normals, densify, backgrounds, masks, ctx.width,
             ctx.height, ctx.tile_size, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids,
             median_ids, v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
             v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad,
-            render_accum_depth=accum_depth if ctx.exact else None, live_entries=ctx.live, **ctx.packed)
+            render_accum_depth=accum_depth if ctx.exact else None, live_entries=ctx.live, render_trans=trans,
+            **ctx.packed)
         ctx.live = None
         if ctx.absgrad:
             means2d.absgrad = v_means2d_abs
