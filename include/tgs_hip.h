@@ -41,6 +41,11 @@ enum { TGS_TEX_BF16 = 0x100 };
  * tex_h = tex_w = 2^L (L <= 12) textures; the backward takes uv_grad_mode
  * TGS_UV_NONE (the reference's rule) and v_textures of the pyramid's shape. */
 enum { TGS_TEX_MIP2 = 0x200 };
+/* OR-ed into the dtype of tgs_rasterize_fwd (no reference counterpart): visit every
+ * list entry.  By default each wavefront first tests 64 entries at a time against its
+ * pixels with a conservative bound (outside the texture range and op * vis below
+ * 1/255 everywhere) and walks only the candidates; the outputs are the same. */
+enum { TGS_FWD_NO_CULL = 0x400 };
 enum { TGS_OK = 0, TGS_ERR_ARG = 1, TGS_ERR_LAUNCH = 2, TGS_ERR_UNSUPPORTED = 3 };
 
 /* Gradient rule for the bilinear texture lookup w.r.t. the splat coordinate.

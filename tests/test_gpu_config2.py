@@ -88,3 +88,11 @@ def test_config2_sampled_tiles_f64(stages):
         del tex64
         torch.cuda.empty_cache()
     print("config2 f64", stats)
+
+
+def test_config2_forward_candidate_prepass_is_exact(stages, monkeypatch):
+    """At the headline size the candidate pre-pass leaves every forward output, the
+    live-entry bytes and the texture pair counts bit-identical."""
+    from test_gpu_raster import _fwd_pair, assert_prepass_exact
+    st = stages
+    assert_prepass_exact(_fwd_pair(st, st["colors"], st["opac"], st["textures"], monkeypatch, 16))

@@ -328,3 +328,55 @@ def test_texture_gradient_records_match_direct_accumulation(dtype, tex_res):
     va = torch.zeros_like(tex)
     acc = _ops.rasterize_to_pixels_bwd_exact_textured_gaussians(*bargs, v_textures_out=va, tex_counts=cnt)
     assert rel_err(acc[5], direct[5]) < tol
+
+
+def _fwd_pair(st, colors, opac, tex, monkeypatch, tile_size):
+    """The forward with and without the candidate pre-pass; the live-entry bytes start
+    as 0xFF so entries a wave never reached compare equal too."""
+    s = st["scene"]
+    waves = (tile_size * tile_size + 63) // 64
+    res = {}
+    for cull in (True, False):
+        monkeypatch.setattr(_ops, "FWD_CULL", cull)
+        live = torch.full((waves, st["flatten_ids"].numel()), 255, dtype=torch.uint8, device=DEV)
+        if tex is not None:
+            cnt = _ops.tex_counts_buffer(tex)
+            fo = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(
+                st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], None, None, s.width, s.height,
+                tile_size, st["offsets"], st["flatten_ids"], 0.0, 1.0, live_entries=live, tex_counts=cnt)
+        else:
+            cnt = None
+            fo = _ops.rasterize_to_pixels_fwd_2dgs_full(st["means2d"], st["rt"], colors, opac, st["normals"], None,
+                                                        None, s.width, s.height, tile_size, st["offsets"],
+                                                        st["flatten_ids"], 0.0, live_entries=live)
+        res[cull] = (fo, live, cnt)
+    return res
+
+
+def assert_prepass_exact(res):
+    (fa, la, ca), (fb, lb, cb) = res[True], res[False]
+    for i, (x, y) in enumerate(zip(fa, fb)):
+        if i == 7:  # contribution sums: float atomics from several waves, in any order
+            assert rel_err(x, y) < 1e-6
+        else:
+            assert torch.equal(x, y), i
+    assert torch.equal(la, lb)
+    if ca is not None:
+        assert torch.equal(ca, cb)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("textured", [True, False])
+@pytest.mark.parametrize("tile_size", [16, 8])
+def test_forward_candidate_prepass_is_exact(dtype, textured, tile_size, monkeypatch):
+    """The forward's candidate pre-pass skips only list entries no pixel of a wave can
+    keep: every output, the live-entry bytes and the texture pair counts are bit-identical
+    to the walk over every entry (the contribution sums to float atomic ordering)."""
+    sc = scenes.config1(n=2500, width=80, height=64, tex_res=12, seed=3, sh_degree=None)
+    st = gpu_stages(sc, dtype, tile_size=tile_size)
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(4)
+    colors = torch.rand(C, N, 3, generator=g, dtype=torch.float64).to(DEV, dtype)
+    opac = sc.opacities.double()[None].repeat(C, 1).to(DEV, dtype)
+    tex = sc.textures.to(DEV, dtype) if textured else None
+    assert_prepass_exact(_fwd_pair(st, colors, opac, tex, monkeypatch, tile_size))

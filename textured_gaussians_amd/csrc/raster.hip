@@ -101,6 +101,7 @@ template <typename S> struct RasterArgs {
     int64_t tex_stride;                   // elements per texture (tex_h * tex_w * 4)
     S trx, try_, su, sv, uh, vh, umax, vmax;  // u = s.x su + uh, v = s.y sv + vh, clamped to [0, umax/vmax]
     int32_t mip_log;                          // mip2 filter: log2 of the finest resolution
+    int32_t cull;                             // forward: skip list entries no pixel of a wave can keep
     S g_weight, vis_bias, thr;
     int32_t W, H, ts, tw, th;
     const int32_t *offsets, *flatten_ids;
@@ -497,6 +498,83 @@ template <typename S> __device__ __forceinline__ void zeroer_init(RowZeroer &z, 
 }
 
 // ------------------------------------------------------------------ forward
+// The first 13 values of an entry's packed record (mean2d, A, texture row, opacity), as
+// vector loads from the 16-byte aligned record.
+template <typename S> struct RecHead {
+    S v[16];
+};
+template <typename S> __device__ __forceinline__ RecHead<S> load_rec_head(const S *R);
+template <> __device__ __forceinline__ RecHead<float> load_rec_head<float>(const float *R) {
+    RecHead<float> h;
+    const float4 *q = (const float4 *)R;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float4 t = q[k];
+        h.v[4 * k] = t.x, h.v[4 * k + 1] = t.y, h.v[4 * k + 2] = t.z, h.v[4 * k + 3] = t.w;
+    }
+    return h;
+}
+template <> __device__ __forceinline__ RecHead<double> load_rec_head<double>(const double *R) {
+    RecHead<double> h;
+    const double2 *q = (const double2 *)R;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const double2 t = q[k];
+        h.v[2 * k] = t.x, h.v[2 * k + 1] = t.y;
+    }
+    h.v[14] = h.v[15] = 0;
+    return h;
+}
+
+// Candidate test of one list entry for a wavefront's pixels, whose centres lie in the
+// box [x0, x1] x [y0, y1]: false only if no pixel of the box can keep the pair in the
+// kernel's own arithmetic, so skipping the entry changes no output.  With cr affine in
+// the pixel (cr = crc + u A0 + v A1 around the box centre, |u| <= hx, |v| <= hy), each
+// |cr_k| lies within |crc_k| -/+ (hx |A0_k| + hy |A1_k|), widened by the rounding of
+// the kernel's two float32 FMAs; |s| is then at least |cr.xy|_min / |cr.z|_max.  The
+// pair's sigma = 1/2 min(|s|^2, 2 |d|^2) is bounded below by that and by the distance
+// from the mean to the box.  A texture can hold any alpha, so a textured entry is
+// skipped only when the whole box lies outside the texture range (a_s = 1 there) and
+// op (vis_bias + g e^-sigma) stays below 1/255; the plain kernel needs the second part
+// only.  Margins cover float32 rounding of sigma (1e-3 + 1e-5 relative) and of alpha
+// (1e-3 relative).  Non-positive or non-finite opacities and NaN bounds keep the entry.
+template <typename S, int TEX>
+__device__ __forceinline__ bool fwd_candidate(const RasterArgs<S> &a, int32_t g, double x0, double x1, double y0,
+                                              double y1) {
+    const RecHead<S> R = load_rec_head<S>(a.rec + (int64_t)g * kRec);
+    const double mx = R.v[0], my = R.v[1], op = R.v[kRecOp];
+    const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1), hx = 0.5 * (x1 - x0), hy = 0.5 * (y1 - y0);
+    const double dxc = mx - cx, dyc = my - cy;
+    const double dxm = fabs(dxc) + hx, dym = fabs(dyc) + hy;
+    const double eps = sizeof(S) == 4 ? 4e-7 : 1e-15;
+    double lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double a0 = R.v[kRecA + k], a1 = R.v[kRecA + 3 + k], c0 = R.v[kRecA + 6 + k];
+        const double crc = c0 - dxc * a0 - dyc * a1;
+        const double spread = hx * fabs(a0) + hy * fabs(a1);
+        const double err = eps * (dxm * fabs(a0) + dym * fabs(a1) + fabs(c0));
+        lo[k] = fabs(crc) - spread - err;
+        hi[k] = fabs(crc) + spread + err;
+    }
+    const double ddx = fmax(0.0, fabs(dxc) - hx), ddy = fmax(0.0, fabs(dyc) - hy);
+    double sig = ddx * ddx + ddy * ddy;  // 1/2 of 2 |d|^2
+    bool out_of_range = false;
+    {
+        // hi[2] == 0 means cr.z == 0 over the box (no pixel keeps the pair): the bounds go
+        // to +inf, or NaN where lo is 0 too, which keeps the entry
+        const double lx = fmax(0.0, lo[0]) / hi[2] * (1.0 - 1e-5), ly = fmax(0.0, lo[1]) / hi[2] * (1.0 - 1e-5);
+        sig = fmin(sig, 0.5 * (lx * lx + ly * ly));
+        out_of_range = lx > (double)a.trx * (1.0 + 1e-5) || ly > (double)a.try_ * (1.0 + 1e-5);
+    }
+    sig = sig * (1.0 - 1e-5) - 1e-3;
+    if (!(op > 0.0) || !(op < 1e30) || !(sig == sig)) return true;
+    const double vis_ub = (double)a.vis_bias + fmax(0.0, (double)a.g_weight * exp(-fmax(sig, 0.0)));
+    const bool faint = op * vis_ub < (1.0 / 255.0) * (1.0 - 1e-3);
+    if constexpr (TEX == kTexNone) return !faint;
+    return !(faint && out_of_range);
+}
+
 template <typename S, int K, int TEX>
 __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) {
     const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
@@ -541,94 +619,102 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 #pragma unroll
     for (int k = 0; k < K; ++k) pc[k] = 0;
     bool done = !inside;
-    // live-entry bytes of this wave: bit i of lw is entry lcb + i, written out per 64 entries
+    // live-entry bytes of this wave, one per list entry, written per chunk of 64 entries
     uint8_t *live = a.live ? a.live + (int64_t)(threadIdx.x / kWave) * a.n_isects : nullptr;
-    uint64_t lw = 0;
-    int32_t lcb = start;
+    // the box of this wavefront's pixel centres (full tile width: a wave may span partial rows)
+    const int wv = threadIdx.x / kWave;
+    const int rr0 = (wv * kWave) / a.ts, rr1 = min(a.ts - 1, (wv * kWave + kWave - 1) / a.ts);
+    const double bx0 = tx * a.ts + 0.5, bx1 = tx * a.ts + a.ts - 0.5;
+    const double by0 = ty * a.ts + rr0 + 0.5, by1 = ty * a.ts + rr1 + 0.5;
+    constexpr bool kCullable = TEX != kTexMip2;  // the mip2 sample's validity depends on the footprint
 
-    int32_t idx = start;
-    for (; idx < end; ++idx) {
+    // chunks of 64 entries: each lane tests one entry of the chunk, then the wave walks the
+    // candidates front to back (all entries when culling is off)
+    for (int32_t lo = start; lo < end; lo += kWave) {
         if (__ballot(!done) == 0) break;  // every pixel of this wave is finished
+        const int32_t e = lo + lane;
+        bool cand = e < end;
+        if (kCullable && a.cull && cand) cand = fwd_candidate<S, TEX>(a, ids[e], bx0, bx1, by0, by1);
+        uint64_t todo = __ballot(cand);
+        uint64_t lw = 0;  // bit i: entry lo + i composited by some pixel of this wave
+        while (todo) {
+            if (__ballot(!done) == 0) break;
+            const int bit = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const int32_t idx = __builtin_amdgcn_readfirstlane(lo + bit);
 #pragma unroll
-        for (int zs = 0; zs < kFwdZeroSteps; ++zs) zr.step(lane);
-        const int32_t g = ids[idx];
-        const GaussGeom<S> G = load_geom<S>(a, g);
-        const GaussShade<S, K> H = load_shade<S, K>(a, g);
-        PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
-        S a_s = 1, tex[K];
+            for (int zs = 0; zs < kFwdZeroSteps; ++zs) zr.step(lane);
+            const int32_t g = ids[idx];
+            const GaussGeom<S> G = load_geom<S>(a, g);
+            const GaussShade<S, K> H = load_shade<S, K>(a, g);
+            PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
+            S a_s = 1, tex[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) tex[k] = 0;
-        bool inr = false;
-        if constexpr (TEX == kTexMip2) {
-            const Mip2<S> mm = mip2_setup(p, G, a);
-            p.okz = p.okz && mm.zok;
-            inr = !done && p.okz && mm.valid;
-            if (inr) a_s = mip2_blend<S, K>(mm, a.textures + (int64_t)G.row * a.tex_stride, tex);
-        } else if constexpr (TEX) {
-            inr = !done && in_tex_range(p, a);
-            if (inr) {
-                const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
-                a_s = bilerp_blend<S, K>(bweights(bl), fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl),
-                                         tex);
-            }
-        }
-        const S alpha = pair_alpha(H.op, p.vis, a_s);
-        S contrib = 0;
-        bool counted = false, used = false;
-        if (!done && pair_keep(p, alpha)) {
-            const S nT = T * (S(1) - alpha);
-            if (nT <= S(1e-4)) {
-                done = true;  // this entry is excluded
-            } else {
-                used = true;
-                const S w = alpha * T;
-#pragma unroll
-                for (int k = 0; k < K; ++k) pc[k] += (H.col[k] + tex[k]) * w;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) pn[k] += H.nrm[k] * w;
-                const S depth = H.col[K - 1];
-                distort += S(2) * (w * depth * (S(1) - T) - w * acc_vd);
-                acc_vd += w * depth;
-                if (T > S(0.5)) {
-                    median = depth;
-                    median_id = idx;
-                }
-                cur = idx;
-                T = nT;
-                if (alpha > a.thr) {
-                    contrib = alpha;
-                    counted = true;
+            for (int k = 0; k < K; ++k) tex[k] = 0;
+            bool inr = false;
+            if constexpr (TEX == kTexMip2) {
+                const Mip2<S> mm = mip2_setup(p, G, a);
+                p.okz = p.okz && mm.zok;
+                inr = !done && p.okz && mm.valid;
+                if (inr) a_s = mip2_blend<S, K>(mm, a.textures + (int64_t)G.row * a.tex_stride, tex);
+            } else if constexpr (TEX) {
+                inr = !done && in_tex_range(p, a);
+                if (inr) {
+                    const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
+                    a_s = bilerp_blend<S, K>(bweights(bl),
+                                             fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl), tex);
                 }
             }
-        }
-        // per-Gaussian contribution counters: one wave sum, one atomic pair per wave
-        const uint64_t cm = __ballot(counted);
-        if (cm) {
-            const S sum = wave_sum(contrib);
-            if (lane == 0) {
-                atomic_add(a.csum + G.row, sum);
-                atomic_add(a.ccnt + G.row, S(__popcll(cm)));
+            const S alpha = pair_alpha(H.op, p.vis, a_s);
+            S contrib = 0;
+            bool counted = false, used = false;
+            if (!done && pair_keep(p, alpha)) {
+                const S nT = T * (S(1) - alpha);
+                if (nT <= S(1e-4)) {
+                    done = true;  // this entry is excluded
+                } else {
+                    used = true;
+                    const S w = alpha * T;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) pc[k] += (H.col[k] + tex[k]) * w;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) pn[k] += H.nrm[k] * w;
+                    const S depth = H.col[K - 1];
+                    distort += S(2) * (w * depth * (S(1) - T) - w * acc_vd);
+                    acc_vd += w * depth;
+                    if (T > S(0.5)) {
+                        median = depth;
+                        median_id = idx;
+                    }
+                    cur = idx;
+                    T = nT;
+                    if (alpha > a.thr) {
+                        contrib = alpha;
+                        counted = true;
+                    }
+                }
             }
-        }
-        if constexpr (TEX) {
-            if (a.tex_cnt) {
-                // pairs whose backward adds texture gradient: composited and in range
-                const uint64_t tu = __ballot(used && inr);
-                if (tu && lane == 0) atomicAdd(a.tex_cnt + G.row, (int32_t)__popcll(tu));
+            // per-Gaussian contribution counters: one wave sum, one atomic pair per wave
+            const uint64_t cm = __ballot(counted);
+            if (cm) {
+                const S sum = wave_sum(contrib);
+                if (lane == 0) {
+                    atomic_add(a.csum + G.row, sum);
+                    atomic_add(a.ccnt + G.row, S(__popcll(cm)));
+                }
             }
-        }
-        if (live) {
-            lw |= (uint64_t)(__ballot(used) != 0) << (idx - lcb);
-            if (idx - lcb == kWave - 1) {
-                live[lcb + lane] = (uint8_t)((lw >> lane) & 1);
-                lcb += kWave;
-                lw = 0;
+            if constexpr (TEX) {
+                if (a.tex_cnt) {
+                    // pairs whose backward adds texture gradient: composited and in range
+                    const uint64_t tu = __ballot(used && inr);
+                    if (tu && lane == 0) atomicAdd(a.tex_cnt + G.row, (int32_t)__popcll(tu));
+                }
             }
+            lw |= (uint64_t)(__ballot(used) != 0) << bit;
         }
+        // the backward reads these bytes only up to the wave's last composited entry
+        if (live && e < end) live[e] = (uint8_t)((lw >> lane) & 1);
     }
-    // entries [lcb, idx) not yet written; the backward reads no entry past the wave's
-    // last composited one, which is below idx
-    if (live && lane < idx - lcb) live[lcb + lane] = (uint8_t)((lw >> lane) & 1);
     zr.drain(lane);
     if (inside) {
         a.out_a[pix] = S(1) - T;
@@ -1683,7 +1769,8 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  size_t zero_row_bytes, int zero_rows_select, void *workspace, size_t workspace_size,
                                  tgs_stream_t stream) {
     const int tex_mode = tex_mode_of(textures, dtype);
-    dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2);
+    const int cull = (dtype & TGS_FWD_NO_CULL) ? 0 : 1;
+    dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2 | TGS_FWD_NO_CULL);
     TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_fwd: bfloat16 textures need float32 compute");
     TGS_CHECK_ARG(tex_mode != kTexMip2 || mip_log_of(tex_h, tex_w) >= 0,
                   "rasterize_fwd: the mip2 filter needs square power-of-two textures up to 4096");
@@ -1711,6 +1798,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.median_ids = median_ids; a.csum = (S *)contrib_sum; a.ccnt = (S *)contrib_count;                      \
         a.out_accd = (S *)render_accum_depth;                                                                   \
         a.out_T = (S *)render_trans;                                                                            \
+        a.cull = cull;                                                                                          \
         a.live = live_entries;                                                                                  \
         a.tex_cnt = textures ? tex_counts : nullptr;                                                            \
         return dispatch<S, false>(a, K, tex_mode, st);                                                          \

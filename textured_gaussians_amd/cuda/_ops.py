@@ -77,6 +77,9 @@ def _same(dtype: torch.dtype, **ts: Tensor) -> None:
 
 TEX_BF16 = 0x100  # OR-ed into the rasterizers' dtype: bfloat16 textures
 TEX_MIP2 = 0x200  # OR-ed into the rasterizers' dtype: the mipmapped2 filter over a pyramid
+FWD_NO_CULL = 0x400  # OR-ed into the forward's dtype: visit every list entry (no candidate pre-pass)
+# The forward's candidate pre-pass (same outputs; off only to compare against the full walk)
+FWD_CULL = True
 
 
 def _tex_dtype_flag(textured, textures, means2d):
@@ -481,7 +484,7 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         zd, zm, zb, zs = _zero_rows_args(zero_rows, N, 1)
         with _timed("raster_fwd", means2d):
             check(lib().tgs_rasterize_fwd(
-              dt | tex_flag, C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
+              dt | tex_flag | (0 if FWD_CULL else FWD_NO_CULL), C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
               _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
               _p(masks.view(torch.uint8)) if masks is not None else None, _p(gids), nnz, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
