@@ -46,6 +46,12 @@ enum { TGS_TEX_MIP2 = 0x200 };
  * pixels with a conservative bound (outside the texture range and op * vis below
  * 1/255 everywhere) and walks only the candidates; the outputs are the same. */
 enum { TGS_FWD_NO_CULL = 0x400 };
+/* OR-ed into the dtype of tgs_rasterize_bwd with the texture-gradient records (no
+ * reference counterpart): only the rows whose tex_counts entry is nonzero take records
+ * (and the gather writes exactly those rows in full); the other rows take float atomics
+ * into v_textures, which must be zero there.  tex_counts may then hold zeros for rows
+ * that do have pairs: the caller picks the rows (e.g. those with many pairs). */
+enum { TGS_BWD_RECORD_ROWS = 0x800 };
 enum { TGS_OK = 0, TGS_ERR_ARG = 1, TGS_ERR_LAUNCH = 2, TGS_ERR_UNSUPPORTED = 3 };
 
 /* Gradient rule for the bilinear texture lookup w.r.t. the splat coordinate.

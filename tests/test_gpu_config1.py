@@ -25,16 +25,24 @@ def _stages(dtype):
     return st
 
 
-@pytest.mark.parametrize("dtype,uv,records", [(torch.float32, "none", False), (torch.float32, "double", False),
-                                              (torch.float32, "exact", False), (torch.float64, "none", False),
-                                              (torch.float64, "exact", False), (torch.float32, "none", True),
-                                              (torch.float64, "exact", True)])
-def test_config1_full_frame(dtype, uv, records, monkeypatch):
-    """records: texture gradients through per-pair records and the row gather (with the
-    unlisted-row fill split between forward and backward) instead of float atomics."""
+@pytest.mark.parametrize("dtype,uv,path", [(torch.float32, "none", "default"), (torch.float32, "double", "default"),
+                                           (torch.float32, "exact", "default"), (torch.float64, "none", "default"),
+                                           (torch.float64, "exact", "default"), (torch.float32, "none", "records"),
+                                           (torch.float64, "exact", "records"), (torch.float32, "none", "atomics"),
+                                           (torch.float32, "exact", "hybrid"), (torch.float64, "none", "hybrid")])
+def test_config1_full_frame(dtype, uv, path, monkeypatch):
+    """Texture-gradient paths: "records" - every texture pair through per-pair records and
+    the row gather (the unlisted-row fill split between forward and backward);
+    "atomics" - float atomics only; "hybrid" - rows with at least 16 pairs through
+    records (many rows at this size), the others atomics; "default" - as shipped."""
     from textured_gaussians_amd.cuda import _wrapper
-    monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", records)
-    monkeypatch.setattr(_wrapper, "FWD_FILL_RESIDUES", 0x0F)
+    if path == "records":
+        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
+        monkeypatch.setattr(_wrapper, "FWD_FILL_RESIDUES", 0x0F)
+    elif path == "atomics":
+        monkeypatch.setattr(_wrapper, "HEAVY_ROW_PAIRS", 0)
+    elif path == "hybrid":
+        monkeypatch.setattr(_wrapper, "HEAVY_ROW_PAIRS", 16)
     st = _stages(dtype)
     assert st["flatten_ids"].numel() > 50_000
     s = st["scene"]

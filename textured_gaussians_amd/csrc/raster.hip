@@ -120,6 +120,7 @@ template <typename S> struct RasterArgs {
     int32_t *tex_cnt;
     const int64_t *rec_incl;
     const int32_t *rec_cnt;
+    int32_t rec_rows;  // 1: only rows with rec_cnt > 0 take records, the others float atomics
     int32_t *rec_cur;
     void *recs;
     // backward inputs
@@ -877,10 +878,13 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         uint64_t tl = 0;
         int tpos = 0;
         int32_t rbase = 0;
+        bool use_rec = false;
         if constexpr (TEX) {
             tl = __ballot(valid && inr);
             tpos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(tl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tl, 0u));
-            if (a.recs && tl) {
+            // records for this entry's row, or float atomics (uniform: one row per entry)
+            use_rec = a.recs && (!a.rec_rows || as_const(a.rec_cnt)[G.row] > 0);
+            if (use_rec && tl) {
                 // claim this entry's slots in the row's record segment now; used at the end
                 if (lane == 0) rbase = atomicAdd(a.rec_cur + G.row, (int32_t)__popcll(tl));
             }
@@ -955,7 +959,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
                                 r.v[4 + k] = mm.w[2 * j + 1] * q[k];
                             }
                         }
-                    } else if (a.recs) {
+                    } else if (use_rec) {
                         // lane 0 claimed the slots; read its value whatever lanes are active here
                         const int32_t pos = __builtin_amdgcn_readlane(rbase, 0) + tpos;
                         S *gtb = a.g_tex + (int64_t)G.row * a.tex_stride;
@@ -1033,7 +1037,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
             // lanes add the 8 values of one pixel's texel row (two adjacent RGBA texels,
             // contiguous), so one atomic instruction touches 8 such runs instead of 64
             // scattered values; the n staged pixels take ceil(n / 8) rounds.
-            const int n_tl = a.recs ? 0 : __popcll(tl);
+            const int n_tl = use_rec ? 0 : __popcll(tl);
             if (n_tl) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -1670,7 +1674,7 @@ template <typename S> static bool gather_fits(int64_t tex_h, int64_t tex_w) {
 
 template <typename S>
 static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterArgs<S> &a, const uint8_t *mask,
-                         hipStream_t st) {
+                         int overwrite, hipStream_t st) {
     const int64_t row_bytes = tex_h * tex_w * 4 * (int64_t)sizeof(S);
     const size_t lds = 2 * (size_t)row_bytes;  // two row buffers
     if (lds > 64 * 1024 &&
@@ -1696,7 +1700,7 @@ static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterAr
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, kGatherMaxLds / std::max<size_t>(lds, 1)));
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
     tex_gather_kernel<S><<<(unsigned)blocks, 512, lds, st>>>((int32_t)tex_w, row_bytes / 16, list, cap, count,
-                                                             (const TexRecord<S> *)a.recs, mask ? 0 : 1, a.g_tex);
+                                                             (const TexRecord<S> *)a.recs, overwrite ? 0 : 1, a.g_tex);
     TGS_CHECK_LAUNCH("rasterize_bwd texture gather");
     return TGS_OK;
 }
@@ -1831,7 +1835,8 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const uint8_t *zero_rows_mask, size_t zero_row_bytes, int zero_rows_select,
                                  void *workspace, size_t workspace_size, tgs_stream_t stream) {
     const int tex_mode = tex_mode_of(textures, dtype);
-    dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2);
+    const int rec_rows = (dtype & TGS_BWD_RECORD_ROWS) ? 1 : 0;
+    dtype &= ~(TGS_TEX_BF16 | TGS_TEX_MIP2 | TGS_BWD_RECORD_ROWS);
     TGS_CHECK_ARG(tex_mode != kTexBF16 || dtype == TGS_F32, "rasterize_bwd: bfloat16 textures need float32 compute");
     TGS_CHECK_ARG(tex_mode != kTexMip2 || (mip_log_of(tex_h, tex_w) >= 0 && uv_grad_mode == TGS_UV_NONE &&
                                           v_textures && !tex_records),
@@ -1873,8 +1878,8 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                           (!zero_rows_dst || ((uintptr_t)zero_rows_mask % 4) == 0);                              \
         if (recs) {                                                                                             \
             a.rec_cnt = tex_counts; a.rec_incl = tex_rec_incl; a.rec_cur = tex_rec_cursor;                      \
-            a.recs = tex_records;                                                                               \
-        } else if (textures && tex_records && zero_rows_dst) {                                                  \
+            a.recs = tex_records; a.rec_rows = rec_rows;                                                        \
+        } else if (textures && tex_records && zero_rows_dst && !rec_rows) {                                     \
             /* records requested but the row does not fit the gather's LDS: the rows the */                   \
             /* forward left for the gather are zero-filled here and added to directly    */                   \
             rc = tgs_zero_fill_rows(zero_rows_dst, zero_row_bytes, N, zero_rows_mask, 1, st);                   \
@@ -1895,7 +1900,10 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         }                                                                                                       \
         if (rc) return rc;                                                                                      \
         if (recs && !records_only) {                                                                            \
-            rc = launch_gather<S>(N, tex_h, tex_w, a, zero_rows_dst ? zero_rows_mask : nullptr, st);             \
+            /* the rows with records (given all rows: every row with mask 1) are written in full */           \
+            rc = rec_rows ? launch_gather<S>(N, tex_h, tex_w, a, nullptr, 1, st)                                \
+                          : launch_gather<S>(N, tex_h, tex_w, a, zero_rows_dst ? zero_rows_mask : nullptr,      \
+                                             zero_rows_dst ? 1 : 0, st);                                          \
             if (rc) return rc;                                                                                  \
         }                                                                                                       \
         if (n > 0)                                                                                              \

@@ -78,6 +78,7 @@ def _same(dtype: torch.dtype, **ts: Tensor) -> None:
 TEX_BF16 = 0x100  # OR-ed into the rasterizers' dtype: bfloat16 textures
 TEX_MIP2 = 0x200  # OR-ed into the rasterizers' dtype: the mipmapped2 filter over a pyramid
 FWD_NO_CULL = 0x400  # OR-ed into the forward's dtype: visit every list entry (no candidate pre-pass)
+BWD_RECORD_ROWS = 0x800  # OR-ed into the backward's dtype: only rows with a nonzero count take records
 # The forward's candidate pre-pass (same outputs; off only to compare against the full walk)
 FWD_CULL = True
 
@@ -635,7 +636,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
                 v_textures_out=None, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None,
-                n_gaussians=None, mip_log_res=None, render_trans=None):
+                n_gaussians=None, mip_log_res=None, render_trans=None, record_rows=False):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -660,6 +661,8 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
     records_only = textured and v_textures_out is False
     tex_h, tex_w, mip_flag = _tex_dims(textured, textures, mip_log_res)
     tex_flag |= mip_flag
+    if record_rows:
+        tex_flag |= BWD_RECORD_ROWS
     with torch.cuda.device(dev):
         v_means2d = torch.zeros_like(means2d)
         v_rt = torch.zeros_like(ray_transforms)

@@ -361,6 +361,10 @@ def rasterize_to_pixels_textured_gaussians(
 # atomics (71M records x 16 values) add about 3 ms, and the unlisted-row fill no longer
 # hides behind the atomic-bound backward.
 TEXTURE_GRAD_RECORDS = False
+# Between the two (used when TEXTURE_GRAD_RECORDS is False): the rows with at least
+# HEAVY_ROW_PAIRS texture pairs (few rows, most pairs: large splats) take records and
+# the gather, the others float atomics.  0 disables it.
+HEAVY_ROW_PAIRS = 0
 # With records, the rows absent from the tile lists are zero-filled along the
 # rasterizer walks: the forward takes the rows whose index mod 8 is in this set
 # (bit r), the backward the others; the gather writes the listed rows.
@@ -386,6 +390,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
         # with float atomics and zeroes the other rows.
         ctx.v_tex = None
         ctx.tex_counts = None
+        ctx.record_rows = False
         ctx.adam = None
         zero_rows = None
         if texture_adam is not None:
@@ -413,6 +418,10 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             else:  # the forward zero-fills the listed rows and the backward adds in place
                 zero_rows = (v_tex, rows, 1)
                 ctx.v_tex = (v_tex, rows, 0)
+                if HEAVY_ROW_PAIRS > 0 and _records_fit(textures, means2d.dtype):
+                    # rows with many pairs through records and the gather (it overwrites them)
+                    ctx.tex_counts = _C.tex_counts_buffer(textures)
+                    ctx.record_rows = True
         # the forward records which list entries each wavefront composited; the backward
         # visits only those
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
@@ -422,6 +431,8 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
             g_weight, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids)
         if ctx.tex_counts is not None:
+            if ctx.record_rows:  # only the heavy rows take records
+                ctx.tex_counts = torch.where(ctx.tex_counts >= HEAVY_ROW_PAIRS, ctx.tex_counts, 0)
             # record sizing starts now; the backward reads its total without waiting
             ctx.tex_counts = _C.TexRecordPlan(ctx.tex_counts)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, textures, normals, densify, backgrounds,
@@ -457,7 +468,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
             v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None,
             render_trans=trans, v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live,
-            tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids)
+            tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids, record_rows=ctx.record_rows)
         ctx.live = None
         ctx.tex_counts = None
         if ctx.adam is not None:
