@@ -84,7 +84,8 @@ def test_host_validation_errors_without_gpu():
     rc = L.tgs_rasterize_fwd(0, 1, 10, 5, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 16, 1, 1, null, null,
                              0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
     assert rc != 0 and b"channels" in L.tgs_last_error()
-    rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 12, 2, 2, null, null,
+    # any tile size whose square is at most 1024 (the reference's tile_size^2 threads)
+    rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 33, 1, 1, null, null,
                              0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
     assert rc != 0 and b"tile_size" in L.tgs_last_error()
     # the packed-record workspace is required and sized by tgs_rasterize_workspace_bytes

@@ -380,3 +380,19 @@ def test_forward_candidate_prepass_is_exact(dtype, textured, tile_size, monkeypa
     opac = sc.opacities.double()[None].repeat(C, 1).to(DEV, dtype)
     tex = sc.textures.to(DEV, dtype) if textured else None
     assert_prepass_exact(_fwd_pair(st, colors, opac, tex, monkeypatch, tile_size))
+
+
+@pytest.mark.parametrize("tile_size", [4, 12, 24, 32])
+def test_raster_any_tile_size(tile_size):
+    """Tile sizes other than 8 and 16 (the reference launches tile_size^2 threads per
+    tile, so any size up to 32): forward and every gradient of the whole frame against
+    the oracle in float64 (1e-9 / 1e-8), through the autograd path."""
+    from gpu_util import all_tiles, raster_tile_parity
+    sc = scenes.config1(n=1500, width=70, height=53, tex_res=10, seed=8, sh_degree=None)
+    st = gpu_stages(sc, torch.float64, tile_size=tile_size)
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(9)
+    colors = torch.rand(C, N, 3, generator=g, dtype=torch.float64).to(DEV)
+    opac = sc.opacities.double()[None].repeat(C, 1).to(DEV)
+    s = st["scene"]
+    raster_tile_parity(st, colors, opac, s.textures, s.backgrounds, torch.float64, all_tiles(st), uv="exact")

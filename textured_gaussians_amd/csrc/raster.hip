@@ -104,6 +104,7 @@ template <typename S> struct RasterArgs {
     int32_t cull;                             // forward: skip list entries no pixel of a wave can keep
     S g_weight, vis_bias, thr;
     int32_t W, H, ts, tw, th;
+    int32_t bpt;  // workgroups per tile: a tile's ts * ts pixels, row-major, over bpt workgroups
     const int32_t *offsets, *flatten_ids;
     // forward outputs
     S *out_c, *out_a, *out_n, *out_d, *out_m, *csum, *ccnt;
@@ -579,14 +580,15 @@ __device__ __forceinline__ bool fwd_candidate(const RasterArgs<S> &a, int32_t g,
 template <typename S, int K, int TEX>
 __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) {
     const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
-    const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t tile_lin = blk / a.bpt;
     const int64_t c = tile_lin / tiles_per_cam;
     const int64_t tid = tile_lin % tiles_per_cam;
     const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
-    const int r = threadIdx.x;
+    const int r = (int)(blk % a.bpt) * (int)blockDim.x + (int)threadIdx.x;  // pixel of the tile
     const int lane = r & (kWave - 1);
     const int i = ty * a.ts + r / a.ts, j = tx * a.ts + r % a.ts;
-    const bool inside = i < a.H && j < a.W;
+    const bool inside = r < a.ts * a.ts && i < a.H && j < a.W;
     const int64_t pix = (c * a.H + i) * a.W + j;
     const S px = S(j) + S(0.5), py = S(i) + S(0.5);
     RowZeroer zr;
@@ -621,9 +623,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
     for (int k = 0; k < K; ++k) pc[k] = 0;
     bool done = !inside;
     // live-entry bytes of this wave, one per list entry, written per chunk of 64 entries
-    uint8_t *live = a.live ? a.live + (int64_t)(threadIdx.x / kWave) * a.n_isects : nullptr;
+    uint8_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
     // the box of this wavefront's pixel centres (full tile width: a wave may span partial rows)
-    const int wv = threadIdx.x / kWave;
+    const int wv = r / kWave;
     const int rr0 = (wv * kWave) / a.ts, rr1 = min(a.ts - 1, (wv * kWave + kWave - 1) / a.ts);
     const double bx0 = tx * a.ts + 0.5, bx1 = tx * a.ts + a.ts - 0.5;
     const double by0 = ty * a.ts + rr0 + 0.5, by1 = ty * a.ts + rr1 + 0.5;
@@ -742,10 +744,11 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
     TexRec<S> *stage = (TexRec<S> *)smem + (threadIdx.x / kWave) * (kTexRows * kWave);
 
     const int64_t tiles_per_cam = (int64_t)a.tw * a.th;
-    const int64_t tile_lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t tile_lin = blk / a.bpt;
     const int64_t c = tile_lin / tiles_per_cam;
     const int64_t tid = tile_lin % tiles_per_cam;
-    const int r = threadIdx.x;
+    const int r = (int)(blk % a.bpt) * (int)blockDim.x + (int)threadIdx.x;  // pixel of the tile
     const int lane = r & (kWave - 1);
     RowZeroer zr;
     zeroer_init(zr, a, lane);
@@ -755,7 +758,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
     }
     const int ty = (int)(tid / a.tw), tx = (int)(tid % a.tw);
     const int i = ty * a.ts + r / a.ts, j = tx * a.ts + r % a.ts;
-    const bool inside = i < a.H && j < a.W;
+    const bool inside = r < a.ts * a.ts && i < a.H && j < a.W;
     const int64_t pix = inside ? (c * a.H + i) * a.W + j : (c * a.H + a.H - 1) * a.W + a.W - 1;
     const S px = S(j) + S(0.5), py = S(i) + S(0.5);
 
@@ -808,7 +811,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
     }
     const bool uv_term = TEX && TEX != kTexMip2 && a.uv_mode != TGS_UV_NONE;  // mip2: the reference's rule only
     const int32_t first = min(end - 1, wave_last);
-    const uint8_t *live = a.live ? a.live + (int64_t)(threadIdx.x / kWave) * a.n_isects : nullptr;
+    const uint8_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
     // per-Gaussian gradient fields in reduce-scatter order: dL/dA (9), colour (K),
     // normal (3), mean2d (2), opacity, |mean2d| (2); this lane adds field `slot`
     constexpr int kGradFields = 17 + K;
@@ -1542,9 +1545,13 @@ template <typename S, int K, int TEX> static void launch_bwd(const RasterArgs<S>
 }
 
 template <typename S, bool BWD>
-static int dispatch(const RasterArgs<S> &a, int K, int tex, hipStream_t st) {
-    const unsigned grid = (unsigned)(a.C * a.tw * a.th);
-    const int bs = a.ts * a.ts;
+static int dispatch(const RasterArgs<S> &a0, int K, int tex, hipStream_t st) {
+    // a tile's ts * ts pixels over workgroups of at most 256 threads (whole wavefronts);
+    // every workgroup of a tile walks the tile's whole list for its own pixels
+    RasterArgs<S> a = a0;
+    const int bs = std::min(256, kWave * (int)ceil_div(a.ts * a.ts, kWave));
+    a.bpt = (int32_t)ceil_div(a.ts * a.ts, bs);
+    const unsigned grid = (unsigned)(a.C * a.tw * a.th * a.bpt);
     if (grid == 0) {  // no kernel to fuse the row fill into
         return a.zdst ? tgs_zero_fill_rows(a.zdst, (size_t)a.zrow16 * 16, a.N, a.zmask, a.zsel, st) : TGS_OK;
     }
@@ -1629,7 +1636,9 @@ static int check_common(int64_t C, int64_t N, int32_t K, const void *textures, i
     TGS_CHECK_ARG(C >= 0 && N >= 0 && C * N < (int64_t(1) << 31), "rasterize: bad sizes C=%lld N=%lld",
                   (long long)C, (long long)N);
     TGS_CHECK_ARG(K >= 1 && K <= 4, "rasterize: Unsupported number of channels: %d", K);
-    TGS_CHECK_ARG(ts == 8 || ts == 16, "rasterize: tile_size %d unsupported (8 or 16)", ts);
+    // the reference launches tile_size^2 threads per tile
+    // (rasterize_to_pixels_textured_gaussians_2dgs_fwd.cu:510), so at most 32
+    TGS_CHECK_ARG(ts >= 1 && ts <= 32, "rasterize: tile_size %d unsupported (tile_size^2 must be 1..1024)", ts);
     TGS_CHECK_ARG((int64_t)tw * ts >= W && (int64_t)th * ts >= H, "rasterize: tile grid does not cover the image");
     TGS_CHECK_ARG(n_isects >= 0 && n_isects < (int64_t(1) << 31), "rasterize: bad n_isects");
     TGS_CHECK_ARG(!textures || (tex_h >= 1 && tex_w >= 1 && tex_h * tex_w * 4 * 8 < (int64_t(1) << 31)),
