@@ -289,6 +289,38 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       int zero_rows_select, void *workspace, size_t workspace_size,
                       tgs_stream_t stream);
 
+/* Allocation-free variant of tgs_rasterize_fwd (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and outputs as tgs_rasterize_fwd minus the workspace, which is a
+ * static device array of TGS_RASTERIZE_NOALLOC_SCRATCH_BYTES bytes inside the
+ * library: one copy per GPU, held from the first call on that GPU until the
+ * process ends.  Calls needing more than that
+ * (tgs_rasterize_workspace_bytes(dtype, 1, entries) > the limit; entries = C*N,
+ * or nnz when packed: about 578k float32 or 289k float64 entries) fail with
+ * TGS_ERR_ARG.  Nothing in the scratch needs to start at zero.  Every call on
+ * one GPU uses that GPU's one array: calls that overlap on one GPU (other
+ * streams, other threads) race; order them on one stream.  This entry point
+ * calls no hipMalloc/hipFree; its first call on a GPU looks the array up with
+ * hipGetSymbolAddress. */
+#define TGS_RASTERIZE_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tgs_rasterize_fwd_noalloc(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
+                              const void *ray_transforms, const void *colors, const void *opacities,
+                              const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
+                              double texture_range_x, double texture_range_y,
+                              const void *backgrounds, const uint8_t *masks,
+                              const int64_t *gaussian_ids, int64_t nnz, int32_t width,
+                              int32_t height, int32_t tile_size, int32_t tile_width,
+                              int32_t tile_height, const int32_t *tile_offsets,
+                              const int32_t *flatten_ids, int64_t n_isects,
+                              double gs_contrib_threshold, double g_weight, void *render_colors,
+                              void *render_alphas, void *render_normals, void *render_distort,
+                              void *render_median, int32_t *last_ids, int32_t *median_ids,
+                              void *contrib_sum, void *contrib_count, void *render_accum_depth,
+                              void *render_trans, uint8_t *live_entries, int32_t *tex_counts,
+                              void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
+                              int zero_rows_select, tgs_stream_t stream);
+
 /* Replaces `rasterize_to_pixels_bwd_textured_gaussians` /
  * `rasterize_to_pixels_bwd2_textured_gaussians` (ext.cpp:89-94,
  * bindings.h:540-629; uv_grad_mode TGS_UV_NONE / TGS_UV_DOUBLE) and

@@ -39,6 +39,8 @@
 // The forward and the backward evaluate a pair through the same helpers with FP
 // contraction disabled and explicit FMAs, so both take identical keep/skip
 // decisions.
+#include <mutex>
+
 #include "common.h"
 
 namespace tgs {
@@ -1821,6 +1823,80 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
 #undef TGS_FWD
     set_error("rasterize_fwd: unsupported dtype %d", dtype);
     return TGS_ERR_UNSUPPORTED;
+}
+
+namespace tgs {
+
+// Workspace of tgs_rasterize_fwd_noalloc: a file-scope device array, one copy per
+// GPU, resident from the library's first use on that GPU until the process ends.
+// In host code `scratch` names a host-side stand-in; kernels get the device
+// address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[TGS_RASTERIZE_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device
+
+// The current device's address of `scratch`, looked up on the first call per device.
+static int scratch_address(void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("rasterize_fwd_noalloc: no usable current device (%d)", dev);
+        return TGS_ERR_LAUNCH;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("rasterize_fwd_noalloc: hipGetSymbolAddress of the scratch array failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return TGS_ERR_LAUNCH;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return TGS_OK;
+}
+
+}  // namespace tgs
+
+extern "C" int tgs_rasterize_fwd_noalloc(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,
+                                         const void *ray_transforms, const void *colors, const void *opacities,
+                                         const void *normals, const void *textures, int64_t tex_h, int64_t tex_w,
+                                         double texture_range_x, double texture_range_y, const void *backgrounds,
+                                         const uint8_t *masks, const int64_t *gaussian_ids, int64_t nnz,
+                                         int32_t width, int32_t height, int32_t tile_size, int32_t tile_width,
+                                         int32_t tile_height, const int32_t *tile_offsets, const int32_t *flatten_ids,
+                                         int64_t n_isects, double gs_contrib_threshold, double g_weight,
+                                         void *render_colors, void *render_alphas, void *render_normals,
+                                         void *render_distort, void *render_median, int32_t *last_ids,
+                                         int32_t *median_ids, void *contrib_sum, void *contrib_count,
+                                         void *render_accum_depth, void *render_trans, uint8_t *live_entries,
+                                         int32_t *tex_counts, void *zero_rows_dst, const uint8_t *zero_rows_mask,
+                                         size_t zero_row_bytes, int zero_rows_select, tgs_stream_t stream) {
+    constexpr size_t cap = TGS_RASTERIZE_NOALLOC_SCRATCH_BYTES;
+    const int base = dtype & ~(TGS_TEX_BF16 | TGS_TEX_MIP2 | TGS_FWD_NO_CULL);
+    // sizes tgs_rasterize_fwd rejects are left to its own checks
+    if (C >= 0 && N >= 0 && C < (int64_t(1) << 31) && N < (int64_t(1) << 31) && nnz < (int64_t(1) << 31) &&
+        (base == TGS_F32 || base == TGS_F64)) {
+        const int64_t E = gaussian_ids ? nnz : C * N;
+        const size_t need = E > 0 ? workspace_bytes(base, 1, E) : 0;
+        TGS_CHECK_ARG(need <= cap,
+                      "rasterize_fwd_noalloc: %lld entries need %zu bytes of workspace; the static scratch holds "
+                      "%zu bytes per GPU (%zu bytes per entry of this dtype)",
+                      (long long)E, need, cap, (size_t)(kRec + kAbc) * (base == TGS_F64 ? 8 : 4));
+    }
+    void *ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc) return rc;
+    return tgs_rasterize_fwd(dtype, C, N, K, means2d, ray_transforms, colors, opacities, normals, textures, tex_h,
+                             tex_w, texture_range_x, texture_range_y, backgrounds, masks, gaussian_ids, nnz, width,
+                             height, tile_size, tile_width, tile_height, tile_offsets, flatten_ids, n_isects,
+                             gs_contrib_threshold, g_weight, render_colors, render_alphas, render_normals,
+                             render_distort, render_median, last_ids, median_ids, contrib_sum, contrib_count,
+                             render_accum_depth, render_trans, live_entries, tex_counts, zero_rows_dst,
+                             zero_rows_mask, zero_row_bytes, zero_rows_select, ws, cap, stream);
 }
 
 extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *means2d,

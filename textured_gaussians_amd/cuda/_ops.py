@@ -446,7 +446,9 @@ def _packed_args(means2d, gaussian_ids, textures, n_gaussians):
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
                 g_weight, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None, n_gaussians=None,
-                mip_log_res=None):
+                mip_log_res=None, noalloc=False):
+    """noalloc: call tgs_rasterize_fwd_noalloc, whose workspace is the library's static
+    per-GPU scratch (the outputs are still allocated here)."""
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, tile_offsets=tile_offsets, flatten_ids=flatten_ids)
     if textured:
@@ -481,18 +483,23 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         ccnt = torch.zeros((N,), dtype=fdt, device=dev)
         accd = torch.empty((C, H, W), dtype=fdt, device=dev)
         trans = torch.empty((C, H, W), dtype=fdt, device=dev)
-        ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
+        if noalloc:
+            ws = None
+        else:
+            ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
         zd, zm, zb, zs = _zero_rows_args(zero_rows, N, 1)
+        entry = lib().tgs_rasterize_fwd_noalloc if noalloc else lib().tgs_rasterize_fwd
+        ws_args = () if noalloc else (_p(ws), ws.numel())
         with _timed("raster_fwd", means2d):
-            check(lib().tgs_rasterize_fwd(
+            check(entry(
               dt | tex_flag | (0 if FWD_CULL else FWD_NO_CULL), C, N, K, _p(means2d), _p(ray_transforms), _p(colors), _p(opacities), _p(normals),
               _p(textures) if textured else None, tex_h, tex_w, float(trx), float(try_), _p(backgrounds),
               _p(masks.view(torch.uint8)) if masks is not None else None, _p(gids), nnz, W, H, tile_size, tw_t, th_t,
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
               _p(csum), _p(ccnt), _p(accd), _p(trans), _live_arg(live_entries, flatten_ids, tile_size),
-              _counts_arg(tex_counts, textures.shape[0] if textured else 0), zd, zm, zb, zs, _p(ws),
-              ws.numel(), _stream(means2d)),
+              _counts_arg(tex_counts, textures.shape[0] if textured else 0), zd, zm, zb, zs, *ws_args,
+              _stream(means2d)),
               "rasterize_to_pixels_fwd")
     return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd, trans
 
