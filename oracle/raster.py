@@ -104,13 +104,17 @@ def _mip2(mip: Tensor, n: Tensor, s: Tensor, dsdx: Tensor, dsdy: Tensor, log_res
     u = (s.x + rx) / (2 rx) (res - 1) unclamped; per level l, u_l = (u + 0.5) / 2^l - 0.5
     and texels floor / ceil; invalid when any of them leaves its level.  Sample
     positions and weights carry no gradient (the reference's rule).
-    Returns (sample [P,L,4], valid [P,L])."""
+    Returns (sample [P,L,4], valid [P,L], edge [P,L]); ``edge`` marks pairs within
+    float32 rounding of a place where ``valid`` flips: a level coordinate at 0 or at
+    its level's last texel, or t at an integer (the levels, and so their extents,
+    change there)."""
     res = 1 << log_res
     s, dsdx, dsdy = s.detach(), dsdx.detach(), dsdy.detach()
     kx, ky = (res - 1) / (2 * rx), (res - 1) / (2 * ry)
     duv = torch.maximum(torch.sqrt((dsdx[..., 0] * kx) ** 2 + (dsdx[..., 1] * kx) ** 2),
                         torch.sqrt((dsdy[..., 0] * ky) ** 2 + (dsdy[..., 1] * ky) ** 2))
     t = torch.log2(duv)
+    edge = (t > -1e-4) & (t < 31) & ((t - torch.round(t)).abs() < 1e-4)
     t = torch.where(t < 0, torch.zeros_like(t), torch.where(t > 31, torch.full_like(t, 31.0), t))
     u = (s[..., 0] + rx) / (2 * rx) * (res - 1)
     v = (s[..., 1] + ry) / (2 * ry) * (res - 1)
@@ -126,6 +130,7 @@ def _mip2(mip: Tensor, n: Tensor, s: Tensor, dsdx: Tensor, dsdy: Tensor, log_res
         v0, v1 = torch.floor(vl), torch.ceil(vl)
         hi = torch.floor((res - 1) / sc)  # (res - 1) >> level
         valid &= (u0 >= 0) & (v0 >= 0) & (u1 <= hi) & (v1 <= hi)
+        edge |= ((ul.abs() < 2e-4) | ((ul - hi).abs() < 2e-4) | (vl.abs() < 2e-4) | ((vl - hi).abs() < 2e-4))
         corners.append((lvl, ul - u0, vl - v0, u0, u1, v0, v1))
     wt = t - tl
     nn = n[None, :].expand_as(u)
@@ -140,7 +145,7 @@ def _mip2(mip: Tensor, n: Tensor, s: Tensor, dsdx: Tensor, dsdy: Tensor, log_res
         for (vv, uu, w) in ((v0, u0, (1 - wu) * (1 - wv)), (v0, u1, wu * (1 - wv)), (v1, u0, (1 - wu) * wv),
                             (v1, u1, wu * wv)):
             samp = samp + (w * wl)[..., None] * mip[nn, idx(vv, uu)]
-    return samp, valid
+    return samp, valid, edge
 
 
 def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
@@ -286,7 +291,7 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
             vis = (0.998 - g_weight) + g_weight * ker
             if mip_log_res is not None:
                 assert uv_grad == "none", "the mip2 filter has the reference's rule only"
-                samp, inr = _mip2(textures, n, s, dsdx, dsdy, mip_log_res, rx, ry)
+                samp, inr, mip_edge = _mip2(textures, n, s, dsdx, dsdy, mip_log_res, rx, ry)
             else:
                 samp, inr = _bilinear(textures, n, s, rx, ry, uv_grad)
             a_s = torch.where(inr, samp[..., 3], torch.ones_like(samp[..., 3]))
@@ -328,6 +333,12 @@ def rasterize_to_pixels(means2d: Tensor, ray_transforms: Tensor, colors: Tensor,
             fr |= (Texcl - 0.5).abs() < 2e-6
             if textured and mip_log_res is None:
                 fr |= ((s[..., 0].abs() - rx).abs() < 2e-5) | ((s[..., 1].abs() - ry).abs() < 2e-5)
+            elif textured:
+                # a validity flip moves the pair's alpha between op*vis*a_s and op*vis: it
+                # matters where op*vis reaches 1/255 and no earlier entry stopped the pixel
+                stop_i = stop.to(torch.int32)
+                before = (torch.cumsum(stop_i, 1) - stop_i) > 0
+                fr |= okz & mip_edge & (op * vis >= (1.0 - 1e-6) / 255.0) & ~before
             fr_pix = fr.any(1)
             fragile[c, yy, xx] = fr_pix
             # Gaussians whose gradient a flipped decision can move: the fragile pair's own

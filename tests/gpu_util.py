@@ -30,6 +30,22 @@ def gpu_stages(sc, dtype=torch.float32, tile_size=16):
                 isect_ids=ids, flatten_ids=flat, offsets=offs, tw=tw, th=th, C=C, tile_size=tile_size)
 
 
+def gpu_stages_packed(sc, dtype=torch.float32, tile_size=16):
+    """As gpu_stages, in the packed layout: per-entry tensors [nnz, ...] for the visible
+    (camera, Gaussian) pairs, with their camera_ids and gaussian_ids."""
+    s = sc.to(DEV, dtype)
+    _, cid, gid, radii, means2d, depths, rt, normals = _ops.fully_fused_projection_packed_fwd_2dgs(
+        s.means, s.quats, s.scales, s.viewmats, s.Ks, s.width, s.height, 0.01, 1e10, 0.0)
+    C = s.viewmats.shape[0]
+    tw = math.ceil(s.width / tile_size)
+    th = math.ceil(s.height / tile_size)
+    tpg, ids, flat = _ops.isect_tiles(means2d, radii, depths, cid, gid, C, tile_size, tw, th, True, True)
+    offs = _ops.isect_offset_encode(ids, C, tw, th)
+    return dict(scene=s, radii=radii, means2d=means2d, depths=depths, rt=rt, normals=normals, tpg=tpg,
+                isect_ids=ids, flatten_ids=flat, offsets=offs, tw=tw, th=th, C=C, tile_size=tile_size,
+                camera_ids=cid, gaussian_ids=gid)
+
+
 def rel_err(a, b):
     """max |a-b| / max |b| (0 if both vanish)."""
     a = a.detach().cpu().double()
@@ -155,7 +171,8 @@ def _raise(rec):
     raise AssertionError(rec)
 
 
-def raster_tile_parity(st, colors, opac, textures, bg, dtype, tiles, uv="none", seed=5, backward=True):
+def raster_tile_parity(st, colors, opac, textures, bg, dtype, tiles, uv="none", seed=5, backward=True,
+                       gaussian_ids=None, n_gaussians=None, filtering=None, zero_fragile_cotangents=False):
     """Rasterizer forward over the whole frame on the GPU, then its backward with upstream
     gradients (all five rendered outputs) nonzero only inside ``tiles``, so every
     per-Gaussian and per-texel gradient comes from those tiles and must equal the
@@ -163,25 +180,105 @@ def raster_tile_parity(st, colors, opac, textures, bg, dtype, tiles, uv="none", 
     (camera, Gaussian) entries their lists reference).  Inputs are GPU tensors of the
     kernel dtype (textures may be another float dtype, bfloat16 included; the oracle
     reads them widened to float64/float32).  ``backward=False``: forward parity only.
+    ``gaussian_ids`` [nnz]: the packed layout (per-entry inputs [nnz, ...], entry i
+    samples texture row gaussian_ids[i]; ``n_gaussians`` rows when untextured).
+    ``filtering="mipmapped2"``: the textures are the [N, R, R, 4] base level; both sides
+    build the pyramid and differentiate through it.  The background gradient is
+    checked too when ``bg`` is given.  ``zero_fragile_cotangents``: the upstream gradient
+    is zero at the pixels the float64 oracle marks fragile (for filters whose flips move a
+    pair's alpha by a large step, which can move a pixel's median entry to another
+    Gaussian: a whole term that no bound on rounding covers).
     Returns a dict of statistics; asserts the tolerances above."""
+    from oracle.raster import generate_mipmap as oracle_mipmap
     from oracle.raster import rasterize_to_pixels
     s = st["scene"]
-    C, N = st["means2d"].shape[:2]
+    packed = gaussian_ids is not None
+    C = st["offsets"].shape[0]
+    if packed:
+        E = st["means2d"].shape[0]
+        N = textures.shape[0] if textures is not None else int(n_gaussians)
+        lead = 1
+    else:
+        N = st["means2d"].shape[1]
+        E = C * N
+        lead = 2
     K = colors.shape[-1]
     W, H, ts = s.width, s.height, st["tile_size"]
     textured = textures is not None
     exact = uv == "exact"
+    mip = filtering == "mipmapped2"
+    L_mip = int(textures.shape[1]).bit_length() - 1 if mip else None
+    assert not mip or uv == "none"
     # the op-level forward for the list positions (last_ids, median_ids)
-    fo = (_ops.rasterize_to_pixels_fwd_textured_gaussians_full(
-        st["means2d"], st["rt"], colors, opac, textures, 3.0, 3.0, st["normals"], bg, None, W, H, ts,
-        st["offsets"], st["flatten_ids"], 0.0, 1.0) if textured else
-        _ops.rasterize_to_pixels_fwd_2dgs_full(st["means2d"], st["rt"], colors, opac, st["normals"], bg, None, W, H,
-                                              ts, st["offsets"], st["flatten_ids"], 0.0))
+    tex_fwd = _ops.generate_mipmap_fwd(textures, L_mip) if mip else textures
+    fo = _ops._raster_fwd(textured, st["means2d"], st["rt"], colors, opac, tex_fwd, 3.0, 3.0, st["normals"], bg, None,
+                          W, H, ts, st["offsets"], st["flatten_ids"], 0.0, 1.0, gaussian_ids=gaussian_ids,
+                          n_gaussians=N if packed else None, mip_log_res=L_mip)
     ids = (fo[5], fo[6])
-    fo = None
+    fo = tex_fwd = None
+
+    # the compact problem: the (camera, Gaussian) entries the sampled tiles reference
+    flat = st["flatten_ids"].cpu().numpy()
+    sel_ids = np.unique(np.concatenate([flat[a:b] for *_, a, b in tiles]))
+    u = torch.from_numpy(sel_ids.astype(np.int64))
+    rank = np.full(E, -1, np.int64)
+    rank[sel_ids] = np.arange(sel_ids.size)
+    new_flat = torch.from_numpy(np.concatenate([rank[flat[a:b]] for *_, a, b in tiles]).astype(np.int32))
+    ltiles, o = [], 0
+    for c, ty, tx, a, b in tiles:
+        ltiles.append((0, ty, tx, o, o + (b - a)))
+        o += b - a
+    Gn = u.numel()
+    tex_rows = None
+    # texture row (Gaussian) of each compact entry
+    row_of = gaussian_ids.cpu()[u] if packed else u % N
+    if textured:
+        tex_rows = textures[row_of.to(textures.device)].cpu()
+
+    def oracle(dt, grad=backward):
+        def take(t):
+            tail = t.shape[lead:]
+            return t.detach().cpu().to(dt).reshape(E, *tail)[u].reshape(1, Gn, *tail).clone().requires_grad_(True)
+        L = dict(m2=take(st["means2d"]), rt=take(st["rt"]), col=take(colors),
+                 op=opac.detach().cpu().to(dt).reshape(-1)[u].reshape(1, Gn).clone().requires_grad_(True),
+                 nrm=take(st["normals"]))
+        if textured:
+            L["tex"] = tex_rows.to(dt).requires_grad_(True)
+        if bg is not None:
+            L["bg"] = bg.detach().cpu().to(dt).clone().requires_grad_(True)
+        tex_in = None
+        if textured:
+            tex_in = oracle_mipmap(L["tex"]) if mip else L["tex"]
+        offs_dummy = torch.zeros(1, st["th"], st["tw"], dtype=torch.int32)
+        loss, outs = 0.0, []
+        for (c, ty, tx, a, b), lt in zip(tiles, ltiles):  # one tile at a time: each has its own camera
+            r = rasterize_to_pixels(L["m2"], L["rt"], L["col"], L["op"], L["nrm"], W, H, ts, offs_dummy, new_flat,
+                                    textures=tex_in, backgrounds=None if bg is None else L["bg"][c:c + 1],
+                                    uv_grad=uv, distort_grad="exact" if exact else "reference", tiles=[lt],
+                                    mip_log_res=L_mip)
+            ys, xs = slice(ty * ts, (ty + 1) * ts), slice(tx * ts, (tx + 1) * ts)
+            outs.append((c, ys, xs, r))
+            if grad:
+                for k, v in cot.items():
+                    loss = loss + (r[k][0, ys, xs] * v[c, ys, xs].to(dt)).sum()
+        if grad:
+            loss.backward()
+        return L, outs
+
     sel = torch.zeros(C, H, W, 1, dtype=torch.float64)
     for c, ty, tx, _, _ in tiles:
         sel[c, ty * ts:(ty + 1) * ts, tx * ts:(tx + 1) * ts] = 1.0
+    n_zeroed = 0
+    if zero_fragile_cotangents:
+        # no upstream gradient at pixels with a pair within float32 rounding of a flip (their
+        # forward is still checked at the fragile bound)
+        with torch.no_grad():
+            _, pre = oracle(torch.float64, grad=False)
+        for (c, ys, xs, r) in pre:
+            fr = r["fragile"][0, ys, xs]
+            n_zeroed += int(fr.sum())
+            sel[c, ys, xs, 0] *= (~fr).double()
+        pre = None
     g = torch.Generator().manual_seed(seed)
     cot = {"render_colors": torch.randn(C, H, W, K, generator=g, dtype=torch.float64) * sel,
            "render_alphas": torch.randn(C, H, W, 1, generator=g, dtype=torch.float64) * sel,
@@ -197,24 +294,28 @@ def raster_tile_parity(st, colors, opac, textures, bg, dtype, tiles, uv="none", 
               "op": opac.detach().clone().requires_grad_(backward),
               "nrm": st["normals"].detach().clone().requires_grad_(backward)}
     dens = torch.zeros_like(st["means2d"])
+    bg_leaf = None if bg is None else bg.detach().clone().requires_grad_(backward)
     if textured:
         tex_leaf = textures.detach().requires_grad_(backward and textures.is_floating_point()
                                                     and textures.dtype != torch.bfloat16)
-        filt = {"none": "bilinear", "double": "bilinear_bwd2", "exact": "bilinear_exact"}[uv]
+        filt = "mipmapped2" if mip else {"none": "bilinear", "double": "bilinear_bwd2", "exact": "bilinear_exact"}[uv]
         outs_gpu = Wr.rasterize_to_pixels_textured_gaussians(
             leaves["m2"], leaves["rt"], leaves["col"], leaves["op"], tex_leaf, 3.0, 3.0, leaves["nrm"], dens, W, H,
-            ts, st["offsets"], st["flatten_ids"], backgrounds=bg, filtering=filt)
+            ts, st["offsets"], st["flatten_ids"], backgrounds=bg_leaf, filtering=filt, packed=packed,
+            gaussian_ids=gaussian_ids)
     else:
         outs_gpu = Wr.rasterize_to_pixels_2dgs(
             leaves["m2"], leaves["rt"], leaves["col"], leaves["op"], leaves["nrm"], dens, W, H, ts, st["offsets"],
-            st["flatten_ids"], backgrounds=bg, grad_mode="exact" if exact else "reference")
+            st["flatten_ids"], backgrounds=bg_leaf, grad_mode="exact" if exact else "reference", packed=packed,
+            gaussian_ids=gaussian_ids, n_gaussians=N if packed else None)
     fo = [o.detach() for o in outs_gpu[:5]] + list(ids)
     assert float(fo[1].min()) >= 0.0 and float(fo[1].max()) <= 1.0
-    v_m2 = v_rt = v_col = v_op = v_tex = v_nrm = None
+    v_m2 = v_rt = v_col = v_op = v_tex = v_nrm = v_bg = None
     if backward:
         names5 = ["render_colors", "render_alphas", "render_normals", "render_distort", "render_median"]
         torch.autograd.backward(list(outs_gpu[:5]), [cot[k].to(DEV, dtype) for k in names5])
         v_m2, v_rt, v_col, v_op, v_nrm = (leaves[k].grad for k in ("m2", "rt", "col", "op", "nrm"))
+        v_bg = None if bg_leaf is None else bg_leaf.grad
         if textured:
             v_tex = tex_leaf.grad
             tex_leaf.grad = None
@@ -222,52 +323,12 @@ def raster_tile_parity(st, colors, opac, textures, bg, dtype, tiles, uv="none", 
     if textured:
         textures = textures.detach()
 
-    # the compact problem: the (camera, Gaussian) entries the sampled tiles reference
-    flat = st["flatten_ids"].cpu().numpy()
-    sel_ids = np.unique(np.concatenate([flat[a:b] for *_, a, b in tiles]))
-    u = torch.from_numpy(sel_ids.astype(np.int64))
-    rank = np.full(C * N, -1, np.int64)
-    rank[sel_ids] = np.arange(sel_ids.size)
-    new_flat = torch.from_numpy(np.concatenate([rank[flat[a:b]] for *_, a, b in tiles]).astype(np.int32))
-    ltiles, o = [], 0
-    for c, ty, tx, a, b in tiles:
-        ltiles.append((0, ty, tx, o, o + (b - a)))
-        o += b - a
-    Gn = u.numel()
-    tex_rows = None
-    if textured:
-        tex_rows = textures[(u % N).to(textures.device)].cpu()
-
-    def oracle(dt):
-        def take(t):
-            return t.detach().cpu().to(dt).reshape(C * N, *t.shape[2:])[u].reshape(1, Gn, *t.shape[2:]).clone() \
-                .requires_grad_(True)
-        L = dict(m2=take(st["means2d"]), rt=take(st["rt"]), col=take(colors),
-                 op=opac.detach().cpu().to(dt).reshape(-1)[u].reshape(1, Gn).clone().requires_grad_(True),
-                 nrm=take(st["normals"]))
-        if textured:
-            L["tex"] = tex_rows.to(dt).requires_grad_(True)
-        offs_dummy = torch.zeros(1, st["th"], st["tw"], dtype=torch.int32)
-        loss, outs = 0.0, []
-        for (c, ty, tx, a, b), lt in zip(tiles, ltiles):  # one tile at a time: each has its own camera
-            r = rasterize_to_pixels(L["m2"], L["rt"], L["col"], L["op"], L["nrm"], W, H, ts, offs_dummy, new_flat,
-                                    textures=L.get("tex"),
-                                    backgrounds=None if bg is None else bg[c:c + 1].cpu().to(dt),
-                                    uv_grad=uv, distort_grad="exact" if exact else "reference", tiles=[lt])
-            ys, xs = slice(ty * ts, (ty + 1) * ts), slice(tx * ts, (tx + 1) * ts)
-            outs.append((c, ys, xs, r))
-            for k, v in cot.items():
-                loss = loss + (r[k][0, ys, xs] * v[c, ys, xs].to(dt)).sum()
-        if backward:
-            loss.backward()
-        return L, outs
-
     with torch.set_grad_enabled(backward):
         L64, outs64 = oracle(torch.float64)
         L32, outs32 = oracle(torch.float32) if (dtype == torch.float32 and backward) else (None, None)
 
     # forward
-    stats = {"tiles": len(tiles), "gaussians": Gn}
+    stats = {"tiles": len(tiles), "gaussians": Gn, "cotangent_zeroed_pixels": n_zeroed}
     failures = []
     names = ["render_colors", "render_alphas", "render_normals", "render_distort", "render_median"]
     n_frag, n_pix, worst, frag_over = 0, 0, {}, 0
@@ -321,16 +382,16 @@ def raster_tile_parity(st, colors, opac, textures, bg, dtype, tiles, uv="none", 
         g32 = (lambda k: L32[k].grad[0]) if L32 is not None else (lambda k: None)
         ud = u.to(DEV)
         stats["norm_only_rows"] = float(norm_rows.float().mean())
-        for nm, got, key in (("colors", v_col.reshape(C * N, K), "col"), ("opacities", v_op.reshape(C * N, 1), "op"),
-                             ("normals", v_nrm.reshape(C * N, 3), "nrm"), ("means2d", v_m2.reshape(C * N, 2), "m2"),
-                             ("ray_transforms", v_rt.reshape(C * N, 9), "rt")):
+        for nm, got, key in (("colors", v_col.reshape(E, K), "col"), ("opacities", v_op.reshape(E, 1), "op"),
+                             ("normals", v_nrm.reshape(E, 3), "nrm"), ("means2d", v_m2.reshape(E, 2), "m2"),
+                             ("ray_transforms", v_rt.reshape(E, 9), "rt")):
             ref = L64[key].grad[0].reshape(Gn, -1)
             r32 = None if L32 is None else L32[key].grad[0].reshape(Gn, -1)
             stats[nm] = check_grad(nm, got[ud], ref, r32, fr_rows, dtype, info, failures, norm_rows)
         if textured:
             # a Gaussian seen by several sampled cameras has one texture row on the GPU but one
             # copy per (camera, Gaussian) in the compact problem: sum the copies
-            rows, inv = torch.unique(u % N, return_inverse=True)
+            rows, inv = torch.unique(row_of, return_inverse=True)
 
             def tex_sum(t):
                 return torch.zeros(rows.numel(), *t.shape[1:], dtype=t.dtype).index_add_(0, inv, t)
@@ -350,19 +411,28 @@ def raster_tile_parity(st, colors, opac, textures, bg, dtype, tiles, uv="none", 
                     z = max(z, float(blk[m].abs().max()))
             if z != 0.0:
                 failures.append({"tensor": "textures", "unlisted_rows_max": z})
+        if bg is not None:
+            # background: one row per camera, the sum over the sampled tiles' pixels of
+            # T_final * v_colors; cameras with a fragile pixel may use the loose bound
+            cam_frag = torch.zeros(C, dtype=torch.bool)
+            for (c, ys, xs, r) in outs64:
+                cam_frag[c] |= bool(r["fragile"][0, ys, xs].any())
+            stats["backgrounds"] = check_grad("backgrounds", v_bg, L64["bg"].grad,
+                                              None if L32 is None else L32["bg"].grad, cam_frag, dtype, None, failures)
         # per-entry gradients of unreferenced entries: exactly zero
-        others = torch.ones(C * N, dtype=torch.bool)
+        others = torch.ones(E, dtype=torch.bool)
         others[u] = False
         od = others.to(DEV)
         for nm, t in (("colors", v_col), ("opacities", v_op), ("means2d", v_m2), ("ray_transforms", v_rt)):
-            tt = t.reshape(C * N, -1)[od]
+            tt = t.reshape(E, -1)[od]
             if tt.numel() and float(tt.abs().max()) != 0.0:
                 failures.append({"tensor": nm, "unreferenced_max": float(tt.abs().max())})
     # release the GPU outputs before a failure's traceback can keep them (tens of GB at size)
-    fo = v_m2 = v_rt = v_col = v_op = v_tex = v_nrm = tex_leaf = None  # noqa: F841
+    fo = v_m2 = v_rt = v_col = v_op = v_tex = v_nrm = v_bg = tex_leaf = bg_leaf = None  # noqa: F841
     textures = colors = opac = tex_rows = L64 = L32 = outs64 = outs32 = st = None  # noqa: F841
     if failures:
-        _report({"case": f"C={C} N={N} {W}x{H} K={K} tex={textured} {dtype} uv={uv}", "stats": stats,
+        _report({"case": f"C={C} N={N} E={E} {W}x{H} K={K} tex={textured} {dtype} uv={uv} packed={packed} "
+                         f"filter={filtering}", "stats": stats,
                  "failures": failures[:40]})
     assert not failures, (stats, failures[:6])
     return stats

@@ -1,6 +1,8 @@
 """The ``mipmapped2`` filter on the GPU against the oracle: the pyramid (bit-exact in
 the same arithmetic), its adjoint, and the trilinear rasterizer forward and backward
-(the reference's gradient rule) in float64; the filter through the public API."""
+(the reference's gradient rule) in float64; the whole frame of the CPU-runnable workload
+through the pyramid and the rasterizer's autograd in float32 and float64; the filter
+through the public API."""
 import pytest
 import torch
 
@@ -10,7 +12,7 @@ from oracle.raster import rasterize_to_pixels
 from textured_gaussians_amd import scenes
 from textured_gaussians_amd.cuda import _ops
 
-from gpu_util import DEV, cpu64, gpu_stages, rel_err
+from gpu_util import DEV, all_tiles, cpu64, gpu_stages, raster_tile_parity, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -107,3 +109,21 @@ def test_mipmapped2_through_the_api():
         m["opacities"], mip, 4, 3.0, 3.0, m["normals"], sc.backgrounds, None, sc.width, sc.height, 16,
         m["isect_offsets"], m["flatten_ids"], 0.0, 1.0)
     assert rel_err(fwd[0], out[0]) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_mip2_whole_frame_matches_oracle(dtype):
+    """N = 10,000, 16x16 textures (5 levels), 256x256, every tile: the pyramid built on
+    both sides from the base textures, the texture gradient taken back through it.
+    Tolerances are the ones stated in gpu_util; fragile pairs include those within
+    rounding of a level's edge or of an integer level of detail.  There a flip moves the
+    pair's alpha between op*vis*a_s and op*vis, which can move the pixel's median entry to
+    another Gaussian, so those pixels (under 1%) get no upstream gradient."""
+    sc = scenes.config1()
+    st = gpu_stages(sc, dtype)
+    s = st["scene"]
+    colors = torch.clamp_min(0.28209479177387814 * s.colors[None, :, 0, :] + 0.5, 0.0).contiguous()
+    stats = raster_tile_parity(st, colors, s.opacities[None].contiguous(), s.textures, s.backgrounds, dtype,
+                               all_tiles(st), filtering="mipmapped2", zero_fragile_cotangents=True)
+    assert stats["cotangent_zeroed_pixels"] < 0.01 * 256 * 256
+    print("mip2", dtype, stats)

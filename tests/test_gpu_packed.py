@@ -1,9 +1,10 @@
-"""Packed layout (the reference's ``packed=True``) against the grid layout, whose every
-stage is pinned against the oracle elsewhere: the packed projection is the grid's visible
-entries in (camera, Gaussian) order, the packed tile lists are the grid's under the entry
--> (camera, Gaussian) map (bit-exact), and renders and gradients through the public API
-agree (float64: to rounding of the atomics' order).  Unlike the reference, each packed
-entry samples its own Gaussian's texture (Q3)."""
+"""Packed layout (the reference's ``packed=True``): the packed projection is the grid's
+visible entries in (camera, Gaussian) order, the packed tile lists are the grid's under
+the entry -> (camera, Gaussian) map (bit-exact), renders and gradients through the public
+API agree with the grid layout's (float64: to rounding of the atomics' order), and the
+packed rasterizer matches the oracle directly on every tile, entry by entry, with entry i
+sampling texture row gaussian_ids[i].  Unlike the reference, which indexes the textures
+with the entry index there, each packed entry samples its own Gaussian's texture."""
 import pytest
 import torch
 
@@ -11,7 +12,7 @@ import textured_gaussians_amd as tg
 from textured_gaussians_amd import scenes
 from textured_gaussians_amd.cuda import _ops
 
-from gpu_util import DEV, rel_err
+from gpu_util import DEV, all_tiles, gpu_stages_packed, raster_tile_parity, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -137,3 +138,28 @@ def test_packed_viewmat_gradient_matches_grid():
         out[0].sum().backward()
         res.append(vm.grad)
     assert float(res[0].abs().max()) > 0 and rel_err(res[1], res[0]) < 1e-10
+
+
+@pytest.mark.parametrize("dtype,textured,uv", [(torch.float64, True, "none"), (torch.float64, True, "exact"),
+                                               (torch.float32, True, "none"), (torch.float32, True, "double"),
+                                               (torch.float64, False, "none"), (torch.float32, False, "exact")])
+def test_packed_rasterizer_matches_oracle_on_every_tile(dtype, textured, uv):
+    """The packed rasterizer against the oracle over every tile of three cameras: the
+    oracle's compact problem holds the packed entries the tile lists reference, each with
+    its Gaussian's texture row; a Gaussian seen by several cameras has one texture row,
+    whose gradient sums its entries'.  Tolerances are the ones stated in gpu_util."""
+    sc = scenes.make_scene(3000, 96, 80, 50.0, CAMS, 10, logscale_mu=-3.2, logscale_sigma=0.4, sh_degree=None,
+                           seed=7)
+    st = gpu_stages_packed(sc, dtype)
+    gids = st["gaussian_ids"]
+    nnz, N, C = gids.numel(), sc.means.shape[0], len(CAMS)
+    # entries of one Gaussian in several cameras, and entry indices that are not Gaussian ids
+    assert nnz > 1.5 * N and bool((gids != torch.arange(nnz, device=DEV) % N).any())
+    g = torch.Generator().manual_seed(3)
+    colors = torch.rand(nnz, 3, generator=g, dtype=dtype).to(DEV)
+    opac = st["scene"].opacities[gids].contiguous()
+    bg = torch.rand(C, 3, generator=g, dtype=dtype).to(DEV)
+    tex = st["scene"].textures if textured else None
+    stats = raster_tile_parity(st, colors, opac, tex, bg, dtype, all_tiles(st), uv=uv, gaussian_ids=gids,
+                               n_gaussians=N)
+    print("packed", dtype, textured, uv, stats)

@@ -396,3 +396,29 @@ def test_raster_any_tile_size(tile_size):
     opac = sc.opacities.double()[None].repeat(C, 1).to(DEV)
     s = st["scene"]
     raster_tile_parity(st, colors, opac, s.textures, s.backgrounds, torch.float64, all_tiles(st), uv="exact")
+
+
+@pytest.mark.parametrize("uv", ["none", "exact"])
+def test_saturated_frame_f32(uv):
+    """A frame where most pixels composite down to the 1e-4 transmittance stop (large,
+    nearly opaque splats with opaque textures): the float32 backward, which walks T back
+    by division from the forward's final T, against the oracle on every tile.  Tolerances
+    are the ones stated in gpu_util."""
+    from gpu_util import all_tiles, raster_tile_parity
+    sc = scenes.config1(n=6000, width=128, height=128, tex_res=12, seed=8, sh_degree=None, logscale_mu=-2.4)
+    sc.opacities = 0.75 + 0.24 * sc.opacities
+    sc.textures[..., 3] = 0.7 + 0.3 * sc.textures[..., 3]
+    st = gpu_stages(sc, torch.float32)
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(2)
+    colors = torch.rand(C, N, 3, generator=g).to(DEV)
+    opac = st["scene"].opacities[None].contiguous()
+    s = st["scene"]
+    # most pixels end at the stop: final T below 1e-3
+    fo = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(
+        st["means2d"], st["rt"], colors, opac, s.textures, 3.0, 3.0, st["normals"], s.backgrounds, None, s.width,
+        s.height, 16, st["offsets"], st["flatten_ids"], 0.0, 1.0)
+    assert float((fo[1] > 0.999).float().mean()) > 0.6
+    fo = None
+    stats = raster_tile_parity(st, colors, opac, s.textures, s.backgrounds, torch.float32, all_tiles(st), uv=uv)
+    print("saturated", uv, stats)

@@ -68,7 +68,7 @@ def test_mip2_at_integer_lod_is_bilinear_on_that_level():
         dsdx = torch.stack([d, torch.zeros_like(d)], -1)
         dsdy = torch.zeros_like(dsdx)
         n = torch.tensor([0, 1])
-        got, ok = _mip2(mip, n, s, dsdx, dsdy, L, rx, rx)
+        got, ok, _ = _mip2(mip, n, s, dsdx, dsdy, L, rx, rx)
         assert bool(ok.all())
         lev = mip[:, mip_base(L, lvl):mip_base(L, lvl) + r * r].reshape(2, r, r, 4)
         want, inr = _bilinear(lev, n, torch.stack([_s_for_u(ul, r, rx), _s_for_u(vl, r, rx)], -1), rx, rx, "none")
@@ -86,7 +86,7 @@ def test_mip2_interpolates_linearly_between_levels():
     def at(lod):
         d = torch.full((1, 1), 2.0 ** lod * 2 * rx / (res - 1), dtype=torch.float64)
         dsdx = torch.stack([torch.zeros_like(d), d], -1)  # |dsdx| counts, whatever the axis
-        return _mip2(mip, n, s, dsdx, torch.zeros_like(dsdx), L, rx, rx)
+        return _mip2(mip, n, s, dsdx, torch.zeros_like(dsdx), L, rx, rx)[:2]
 
     s1, ok1 = at(1.0)
     s2, ok2 = at(2.0)
@@ -104,8 +104,40 @@ def test_mip2_validity_shrinks_with_the_level():
     u = torch.tensor([[0.2]], dtype=torch.float64)  # near the left edge
     s = torch.stack([_s_for_u(u, res, rx), _s_for_u(u + 5, res, rx)], -1)
     small = torch.zeros(1, 1, 2, dtype=torch.float64)
-    _, ok0 = _mip2(mip, n, s, small, small, L, rx, rx)
+    _, ok0, _ = _mip2(mip, n, s, small, small, L, rx, rx)
     d = torch.full((1, 1), 4.0 * 2 * rx / (res - 1), dtype=torch.float64)
-    _, ok2 = _mip2(mip, n, s, torch.stack([d, d * 0], -1), small, L, rx, rx)
+    _, ok2, _ = _mip2(mip, n, s, torch.stack([d, d * 0], -1), small, L, rx, rx)
     # level 0: u = 0.2 has texels 0 and 1; level 2: (0.2 + 0.5) / 4 - 0.5 < 0 leaves the level
     assert bool(ok0.all()) and not bool(ok2.any())
+
+
+def test_mip2_edge_marks_where_validity_flips():
+    """The edge flag is set exactly where a small move of u or of the level of detail
+    changes validity: u at a level's first texel, and t at an integer."""
+    L, rx = 4, 3.0
+    mip = generate_mipmap(_tex(n=1, L=L, seed=6))
+    res = 1 << L
+    n = torch.tensor([0])
+    small = torch.zeros(1, 1, 2, dtype=torch.float64)
+
+    def run(u, lod=None):
+        uu = torch.tensor([[u]], dtype=torch.float64)
+        s = torch.stack([_s_for_u(uu, res, rx), _s_for_u(uu + 5, res, rx)], -1)
+        if lod is None:
+            return _mip2(mip, n, s, small, small, L, rx, rx)
+        d = torch.full((1, 1), 2.0 ** lod * 2 * rx / (res - 1), dtype=torch.float64)
+        return _mip2(mip, n, s, torch.stack([d, d * 0], -1), small, L, rx, rx)
+    # level 0, u just inside / outside the first texel: validity flips, both flagged
+    _, ok_in, e_in = run(1e-6)
+    _, ok_out, e_out = run(-1e-6)
+    assert bool(ok_in.all()) and not bool(ok_out.any()) and bool(e_in.all()) and bool(e_out.all())
+    # well inside: no flag
+    _, ok, e = run(3.3)
+    assert bool(ok.all()) and not bool(e.any())
+    # u = 1.2 is valid at level 1 ((1.2 + 0.5) / 2 - 0.5 = 0.35) but not at level 2
+    # (-0.075); t just below / above 1 flips it (level 2 enters the blend)
+    _, ok_lo, e_lo = run(1.2, 1.0 - 1e-6)
+    _, ok_hi, e_hi = run(1.2, 1.0 + 1e-6)
+    assert bool(ok_lo.all()) and not bool(ok_hi.any()) and bool(e_lo.all()) and bool(e_hi.all())
+    _, ok_mid, e_mid = run(1.2, 0.5)
+    assert bool(ok_mid.all()) and not bool(e_mid.any())
