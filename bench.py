@@ -10,9 +10,13 @@ projection (gradients for means, quats, scales, opacities, SH coefficients and
 textures; ``.grad`` is reset to None before each step, as an optimizer's
 zero_grad does).  Inputs are resident in HBM before the timed region.
 
-Prints ONE JSON line (rank 0).  ``roofline`` is for the dominant kernel (the
-rasterizer backward), ``cpu_baseline`` times the CPU oracle on a bounded tile
-sample and extrapolates.
+Prints ONE JSON line (rank 0).  ``roofline`` is the rasterizer pair: the
+algorithmic bytes of forward and backward over their summed kernel time, as
+launched by the product path (which zero-fills the 40 GB texture gradient along
+the two rasterizer walks); the same pair with the fill done by one dense launch
+beforehand is reported beside it (``raster_fill_free``), with the fill bytes as
+their own figure.  ``cpu_baseline`` times the CPU oracle on a bounded tile sample
+(median of 3 runs after a warm-up) and extrapolates.
 """
 import argparse
 import json
@@ -39,11 +43,16 @@ def parse():
     ap.add_argument("--n", type=int, default=1_000_000, help="Gaussians (default: the headline 1M)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-tiles", type=int, default=60, help="tiles in the CPU oracle sample")
+    ap.add_argument("--cpu-tiles", type=int, default=40, help="tiles in the CPU oracle sample")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU runs (after one warm-up); median")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-texture leg")
     ap.add_argument("--no-config5", action="store_true", help="skip the full-training-iteration leg")
     ap.add_argument("--config5-only", action="store_true", help="run only the full-training-iteration leg")
     ap.add_argument("--no-layouts", action="store_true", help="skip the mipmapped2 and packed-layout legs")
+    ap.add_argument("--fwd-no-cull", action="store_true",
+                    help="forward without the candidate pre-pass (every list entry visited; same outputs)")
+    ap.add_argument("--grad-paths", action="store_true",
+                    help="also time the texture-gradient variants (records + gather, hybrid) on the headline scene")
     return ap.parse_args()
 
 
@@ -118,62 +127,97 @@ def algorithmic_bytes(out, params, sc, textured, K=3):
     return dict(I=I, P=P, V=V, U=U, tiles=tiles, bytes_fwd=bf, bytes_bwd=bb)
 
 
-def copy_bandwidth(dev):
-    """Measured device-to-device copy rate (read + write bytes / time)."""
+def stream_rates(dev):
+    """Measured streaming rates on this device, GB/s of bytes moved (read + written):
+    a device-to-device copy, and c = a + b (two reads, one write) over 4 GiB arrays."""
     n = 1 << 30
     a = torch.empty(n, dtype=torch.float32, device=dev)
     b = torch.empty_like(a)
-    for _ in range(2):
-        b.copy_(a)
-    torch.cuda.synchronize()
+    c = torch.empty_like(a)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(5):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / 5
-    del a, b
-    return 2 * n * 4 / (ms * 1e-3) / 1e9
+    out = {}
+    for name, fn, nbytes in (("copy", lambda: b.copy_(a), 2 * n * 4),
+                             ("add", lambda: torch.add(a, b, out=c), 3 * n * 4)):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(5):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = round(nbytes / (e0.elapsed_time(e1) / 5 * 1e-3) / 1e9, 1)
+    del a, b, c
+    return out
 
 
-def cpu_baseline(sc, meta_gpu, n_tiles_sample):
-    """The CPU oracle (float32 torch on the host) on a bounded sample: full
-    projection + tile intersection + SH of all Gaussians, and the rasterizer
-    forward+backward on every k-th tile, extrapolated to all tiles by the
-    per-tile work sum(I_tile * 256)."""
+def host_cpus():
+    """CPUs this process may use: its affinity set, capped by a cgroup CPU quota."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def host_info():
+    try:
+        model = [l for l in open("/proc/cpuinfo") if l.startswith("model name")][0].split(":")[1].strip()
+    except (OSError, IndexError):
+        model = platform.processor()
+    try:
+        kb = int([l for l in open("/proc/meminfo") if l.startswith("MemTotal")][0].split()[1])
+        ram = round(kb / 1024 ** 2, 1)
+    except (OSError, IndexError, ValueError):
+        ram = None
+    return model, ram
+
+
+def cpu_baseline(sc, meta_gpu, n_tiles_sample, runs=3):
+    """The CPU oracle (float32 torch on the host, every CPU this process may use) on a
+    bounded sample of one iteration: projection + SH forward/backward and tile
+    intersection of all Gaussians, and the rasterizer forward+backward on every k-th
+    tile (on the GPU's projected inputs), extrapolated to all tiles by the per-tile
+    work sum(I_tile * 256).  One warm-up, then the median of ``runs`` timed runs."""
     import numpy as np
     from oracle import isect as O_isect
     from oracle.projection import project_2dgs
     from oracle.raster import rasterize_to_pixels
     from oracle.sh import spherical_harmonics
 
-    threads = min(os.cpu_count() or 1, 16)  # the host-core share of one GPU
+    threads = host_cpus()
     torch.set_num_threads(threads)
     f = torch.float32
     cpu = lambda t: t.detach().cpu().to(f) if t.is_floating_point() else t.detach().cpu()  # noqa: E731
     means, quats, scales = cpu(sc.means), cpu(sc.quats), cpu(sc.scales)
     viewmats, Ks = cpu(sc.viewmats), cpu(sc.Ks)
-    sh = cpu(sc.colors)
+    sh, opac = cpu(sc.colors), cpu(sc.opacities)
     W, H = sc.width, sc.height
-    # (1) projection + SH, forward and backward, all Gaussians
-    t0 = time.perf_counter()
-    m = means.clone().requires_grad_(True)
-    q = quats.clone().requires_grad_(True)
-    s = scales.clone().requires_grad_(True)
-    shc = sh.clone().requires_grad_(True)
-    radii, m2, dep, rt, nrm = project_2dgs(m, q, s, viewmats, Ks, W, H)
-    dirs = m[None] - torch.inverse(viewmats)[:, None, :3, 3]
-    col = torch.clamp_min(spherical_harmonics(3, dirs, shc[None], radii > 0) + 0.5, 0.0)
-    ((m2.sum() + rt.sum() + nrm.sum() + col.sum())).backward()
-    t_proj = time.perf_counter() - t0
-    # (2) tile intersection + sort + offsets (numpy), all Gaussians
-    t0 = time.perf_counter()
     tw, th = math.ceil(W / 16), math.ceil(H / 16)
-    _, ids, flat = O_isect.isect_tiles(m2.detach().numpy(), radii.numpy(), dep.detach().numpy(), 16, tw, th)
-    offs = O_isect.isect_offset_encode(ids, 1, tw, th)
-    t_isect = time.perf_counter() - t0
-    # (3) rasterizer fwd+bwd on a tile sample, on the GPU's projected inputs
+
+    def front():
+        # (1) projection + SH, forward and backward, all Gaussians
+        t0 = time.perf_counter()
+        m = means.clone().requires_grad_(True)
+        q = quats.clone().requires_grad_(True)
+        s = scales.clone().requires_grad_(True)
+        shc = sh.clone().requires_grad_(True)
+        radii, m2, dep, rt, nrm = project_2dgs(m, q, s, viewmats, Ks, W, H)
+        dirs = m[None] - torch.inverse(viewmats)[:, None, :3, 3]
+        col = torch.clamp_min(spherical_harmonics(3, dirs, shc[None], radii > 0) + 0.5, 0.0)
+        ((m2.sum() + rt.sum() + nrm.sum() + col.sum())).backward()
+        t1 = time.perf_counter()
+        # (2) tile intersection + sort + offsets (numpy), all Gaussians
+        _, ids, flat = O_isect.isect_tiles(m2.detach().numpy(), radii.numpy(), dep.detach().numpy(), 16, tw, th)
+        offs = O_isect.isect_offset_encode(ids, 1, tw, th)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, col.detach(), ids, flat, offs
+
+    t_proj, t_isect, col, ids, flat, offs = front()
+    # (3) the rasterizer sample: every k-th tile, compact sub-problem of the Gaussians it references
     offs_flat = offs.reshape(-1).astype(np.int64)
     I = ids.size
     ends = np.append(offs_flat[1:], I)
@@ -181,45 +225,78 @@ def cpu_baseline(sc, meta_gpu, n_tiles_sample):
     n_t = offs_flat.size
     stride = max(1, n_t // n_tiles_sample)
     sample = list(range(stride // 2, n_t, stride))[:n_tiles_sample]
-    # compact sub-problem: only the Gaussians the sampled tiles reference
     sel = np.concatenate([flat[offs_flat[t]:ends[t]] for t in sample]) if sample else np.zeros(0, np.int32)
     uniq, inv = np.unique(sel, return_inverse=True)
-    Nloc = max(1, uniq.size)
     ut = torch.from_numpy(uniq.astype(np.int64))
-    tex = sc.textures[ut.to(sc.textures.device)].detach().cpu().to(f).requires_grad_(True)
-    mm2 = cpu(meta_gpu["means2d"])[0, ut][None].requires_grad_(True)
-    rrt = cpu(meta_gpu["ray_transforms"])[0, ut][None].requires_grad_(True)
-    nn_ = cpu(meta_gpu["normals"])[0, ut][None].requires_grad_(True)
-    cc = col.detach()[0, ut][None].clone().requires_grad_(True)
-    oo = torch.sigmoid(torch.randn(1, Nloc)).requires_grad_(True)
-    # each sampled tile gets its [start, end) in the compact list
+    tex0 = sc.textures[ut.to(sc.textures.device)].detach().cpu().to(f)
+    m2_0 = cpu(meta_gpu["means2d"])[0, ut][None]
+    rt_0 = cpu(meta_gpu["ray_transforms"])[0, ut][None]
+    nrm_0 = cpu(meta_gpu["normals"])[0, ut][None]
     new_flat = torch.from_numpy(inv.astype(np.int32))
     tiles_list, o = [], 0
     for t in sample:
         tiles_list.append((0, t // tw, t % tw, o, o + int(per_tile[t])))
         o += int(per_tile[t])
     loc_offs = torch.zeros(1, th, tw, dtype=torch.int32)
-    t0 = time.perf_counter()
-    outr = rasterize_to_pixels(mm2, rrt, cc, oo, nn_, W, H, 16, loc_offs, new_flat, textures=tex,
-                               backgrounds=cpu(sc.backgrounds), uv_grad="none", distort_grad="reference",
-                               tiles=tiles_list)
-    tgt = cpu(sc.target)
-    loss = (outr["render_colors"] - tgt).abs().sum() / (H * W * 3)
-    loss.backward()
-    t_sample = time.perf_counter() - t0
+    tgt, bg = cpu(sc.target), cpu(sc.backgrounds)
+
+    def raster():
+        t0 = time.perf_counter()
+        leaves = [x.clone().requires_grad_(True) for x in (m2_0, rt_0, col[0, ut][None], opac[ut][None], nrm_0,
+                                                            tex0)]
+        outr = rasterize_to_pixels(leaves[0], leaves[1], leaves[2], leaves[3], leaves[4], W, H, 16, loc_offs,
+                                   new_flat, textures=leaves[5], backgrounds=bg, uv_grad="none",
+                                   distort_grad="reference", tiles=tiles_list)
+        loss = (outr["render_colors"] - tgt).abs().sum() / (H * W * 3)
+        loss.backward()
+        return time.perf_counter() - t0
+
+    raster()  # warm-up (the front stages warmed up above)
     work_all = float((per_tile * 256).sum())
     work_smp = float((per_tile[sample] * 256).sum())
-    t_raster = t_sample * (work_all / max(work_smp, 1.0))
-    total = t_proj + t_isect + t_raster
-    try:
-        model = [l for l in open("/proc/cpuinfo") if l.startswith("model name")][0].split(":")[1].strip()
-    except Exception:
-        model = platform.processor()
-    return dict(value=round(total * 1e3, 1), unit="ms/iter", cores=torch.get_num_threads(), kind="port",
-                sample=(f"extrapolated: full projection+SH fwd/bwd ({t_proj:.1f}s) and tile intersection "
-                        f"({t_isect:.1f}s) of all {means.shape[0]} Gaussians, rasterizer fwd+bwd on {len(sample)} "
-                        f"of {n_t} tiles "
-                        f"({t_sample:.1f}s) scaled by sum(I_tile*256); fp32 torch oracle; cpu: {model}"))
+    totals, parts = [], []
+    for _ in range(runs):
+        tp, ti = front()[:2]
+        ts = raster()
+        tr = ts * (work_all / max(work_smp, 1.0))
+        totals.append(tp + ti + tr)
+        parts.append((tp, ti, ts))
+    k = int(np.argsort(totals)[len(totals) // 2])
+    tp, ti, ts = parts[k]
+    model, ram = host_info()
+    return dict(value=round(totals[k] * 1e3, 1), unit="ms/iter", cores=torch.get_num_threads(), kind="port",
+                runs=runs, runs_ms=[round(t * 1e3, 1) for t in totals], cpu_model=model, ram_GB=ram,
+                os_cpu_count=os.cpu_count(),
+                sample=(f"extrapolated, median of {runs} runs after 1 warm-up: full projection+SH fwd/bwd "
+                        f"({tp:.2f}s) and tile intersection ({ti:.2f}s) of all {means.shape[0]} Gaussians, "
+                        f"rasterizer fwd+bwd on {len(sample)} of {n_t} tiles ({ts:.1f}s) scaled by "
+                        f"sum(I_tile*256) ({work_all / max(work_smp, 1.0):.0f}x); fp32 torch oracle, scene opacities"))
+
+
+def grad_paths_leg(tg, sc, params, steps, warmup):
+    """The headline iteration under each texture-gradient variant of the autograd layer
+    (all give the same gradients): float atomics into the row-filled gradient (the
+    product path), the same with one dense fill, per-pair records summed per row by the
+    gather kernel (the forward zero-filling every unlisted row), and records for the rows
+    with at least 64 pairs only.  The records variants run the rasterizer backward
+    without texture-gradient atomics; the gather runs after it, inside the backward op."""
+    from textured_gaussians_amd.cuda import _wrapper
+    variants = [("atomics_fused_fill", {}), ("atomics_dense_fill", {"FUSED_FILL": False}),
+                ("records_gather", {"TEXTURE_GRAD_RECORDS": True, "FWD_FILL_RESIDUES": 0xFF}),
+                ("records_rows_ge64", {"HEAVY_ROW_PAIRS": 64})]
+    res = {}
+    for name, cfg in variants:
+        saved = {k: getattr(_wrapper, k) for k in cfg}
+        try:
+            for k, v in cfg.items():
+                setattr(_wrapper, k, v)
+            ms, kt, out = timed_steps(tg, sc, params, steps, warmup, True)
+            del out
+        finally:
+            for k, v in saved.items():
+                setattr(_wrapper, k, v)
+        res[name] = {"ms_per_iter": round(ms, 3), "kernels_ms": {k: round(v, 3) for k, v in kt.items()}}
+    return res
 
 
 DECODER_FLOP = 2 * (32 * 128 + 128 * 128 + 128 * 1024)  # per latent
@@ -392,6 +469,9 @@ def main():
     lib()
     dev = torch.device("cuda:0")
     torch.manual_seed(0)
+    if args.fwd_no_cull:
+        from textured_gaussians_amd.cuda import _ops
+        _ops.FWD_CULL = False
     if args.config5_only:
         print("[bench] full training iteration leg only", file=sys.stderr, flush=True)
         print(json.dumps({"config5": config5_leg(tg, args.steps, args.warmup, dev)}), flush=True)
@@ -404,24 +484,44 @@ def main():
     ab = algorithmic_bytes(out, params, sc, True)
     t_bwd = kt.get("raster_bwd", float("nan"))
     t_fwd = kt.get("raster_fwd", float("nan"))
+    B = ab["bytes_fwd"] + ab["bytes_bwd"]
+    achieved_pair = B / ((t_fwd + t_bwd) * 1e-3) / 1e9
     achieved_bwd = ab["bytes_bwd"] / (t_bwd * 1e-3) / 1e9
-    achieved_fr = (ab["bytes_fwd"] + ab["bytes_bwd"]) / ((t_fwd + t_bwd) * 1e-3) / 1e9
+    meta_ref = {k: out[-1][k].detach() for k in ("means2d", "ray_transforms", "normals")}
     del out
+    # the same iteration with the texture gradient zero-filled by one dense launch before
+    # the forward: the rasterizer kernels without the fill, and the fill alone
+    from textured_gaussians_amd.cuda import _wrapper
+    print(f"[bench] textured {ms:.2f} ms/iter; timing the dense-fill variant", file=sys.stderr, flush=True)
+    _wrapper.FUSED_FILL = False
+    try:
+        fms, fkt, fout = timed_steps(tg, sc, params, max(3, args.steps // 2), 1, True)
+    finally:
+        _wrapper.FUSED_FILL = True
+    del fout
+    ff_fwd, ff_bwd = fkt.get("raster_fwd", float("nan")), fkt.get("raster_bwd", float("nan"))
+    t_fill = fkt.get("grad_fill", float("nan"))
+    fill_bytes = params["textures"].numel() * 4
+    achieved_ff = B / ((ff_fwd + ff_bwd) * 1e-3) / 1e9
+    fill_gbs = fill_bytes / (t_fill * 1e-3) / 1e9
     plain = None
-    print(f"[bench] textured {ms:.2f} ms/iter; timing plain 2DGS", file=sys.stderr, flush=True)
+    print("[bench] timing plain 2DGS", file=sys.stderr, flush=True)
     if not args.no_plain:
         pms, pkt, pout = timed_steps(tg, sc, params, max(3, args.steps // 2), 1, False)
         pab = algorithmic_bytes(pout, params, sc, False)
         pt = pkt.get("raster_fwd", float("nan")) + pkt.get("raster_bwd", float("nan"))
         p_ach = (pab["bytes_fwd"] + pab["bytes_bwd"]) / (pt * 1e-3) / 1e9
         plain = dict(ms_per_iter=round(pms, 3), raster_fwd_ms=round(pkt.get("raster_fwd", 0), 3),
-                     raster_bwd_ms=round(pkt.get("raster_bwd", 0), 3), raster_hbm_frac=round(p_ach / HBM_PEAK_GBS, 4))
+                     raster_bwd_ms=round(pkt.get("raster_bwd", 0), 3), raster_hbm_frac=round(p_ach / HBM_PEAK_GBS, 4),
+                     algorithmic_bytes=pab["bytes_fwd"] + pab["bytes_bwd"])
         del pout
-    copy_gbs = copy_bandwidth(dev)
+    streams = stream_rates(dev)
+    streams["zero_fill"] = round(fill_gbs, 1)
+    practical = max(streams.values())
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "raster_bwd_traffic_config2.json")
+    tf = os.path.join(ROOT, "profiles", "raster_traffic_config2.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+        traffic = json.load(open(tf)).get("pair_bytes_per_launch")
     line = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/iter", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
@@ -431,24 +531,30 @@ def main():
                    "n_isects": ab["I"], "visible": ab["V"], "texels_touched": ab["U"]},
         "mpix_per_s": round(ab["P"] / (ms * 1e-3) / 1e6, 2),
         "raster_ms": {"fwd": round(t_fwd, 3), "bwd": round(t_bwd, 3)},
-        "raster_hbm_frac_fwd_bwd": round(achieved_fr / HBM_PEAK_GBS, 4),
-        # the same against the device-to-device copy measured on this box
-        "raster_frac_of_copy_fwd_bwd": round(achieved_fr / copy_gbs, 4),
+        "raster_hbm_frac_fwd_bwd": round(achieved_pair / HBM_PEAK_GBS, 4),
         "plain_2dgs": plain,
-        "copy_bandwidth_GBs": round(copy_gbs, 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved_bwd, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_bwd / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "frac_of_copy": round(achieved_bwd / copy_gbs, 4),
-                     "kernel": "raster_bwd_kernel<float,3,1>", "algorithmic_bytes": ab["bytes_bwd"]},
+        # the rasterizer pair without the fused fill, and the fill as one dense launch
+        "raster_fill_free": {"ms_per_iter": round(fms, 3), "fwd_ms": round(ff_fwd, 3), "bwd_ms": round(ff_bwd, 3),
+                             "achieved_GBs": round(achieved_ff, 1), "frac": round(achieved_ff / HBM_PEAK_GBS, 4),
+                             "frac_of_stream_peak": round(achieved_ff / practical, 4)},
+        "grad_fill": {"bytes": fill_bytes, "ms": round(t_fill, 3), "GBs": round(fill_gbs, 1),
+                      "note": "dense zero-fill of the 40 GB texture gradient (end-to-end bytes, outside B_f+B_b)"},
+        "stream_GBs": streams,
+        "roofline": {"bound": "hbm", "achieved": round(achieved_pair, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_pair / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac_of_stream_peak": round(achieved_pair / practical, 4), "stream_peak_GBs": practical,
+                     "kernel": "raster_fwd_kernel<float,3,1> + raster_bwd_kernel<float,3,1>",
+                     "algorithmic_bytes": B, "bytes_fwd": ab["bytes_fwd"], "bytes_bwd": ab["bytes_bwd"],
+                     "time_ms": round(t_fwd + t_bwd, 3)},
+        "raster_bwd_only": {"achieved": round(achieved_bwd, 1), "frac": round(achieved_bwd / HBM_PEAK_GBS, 4)},
     }
+    if args.grad_paths:
+        print("[bench] texture-gradient variants", file=sys.stderr, flush=True)
+        line["grad_paths"] = grad_paths_leg(tg, sc, params, max(3, args.steps // 2), 2)
     if not args.no_cpu_baseline:
         print("[bench] CPU oracle baseline (bounded sample)", file=sys.stderr, flush=True)
-        with torch.no_grad():
-            radii, m2, dep, rt, nrm = tg.fully_fused_projection_2dgs(sc.means, sc.quats, sc.scales, sc.viewmats,
-                                                                     sc.Ks, sc.width, sc.height)
-        meta = dict(means2d=m2, ray_transforms=rt, normals=nrm)
         try:
-            line["cpu_baseline"] = cpu_baseline(sc, meta, args.cpu_tiles)
+            line["cpu_baseline"] = cpu_baseline(sc, meta_ref, args.cpu_tiles, args.cpu_runs)
         except MemoryError as e:  # pragma: no cover
             line["cpu_baseline"] = {"error": str(e)}
     del sc, params  # the other legs build their own scenes

@@ -264,6 +264,12 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * of tgs_rasterize_bwd, the rows with mask 0 are split between forward and
  * backward (disjoint residues) and the gather writes the rows with mask 1.  */
 size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
+/* Measurement hook (no reference counterpart): while set, tgs_rasterize_fwd
+ * (which = 0) or tgs_rasterize_bwd (which = 1) records `begin` and `end`
+ * (hipEvent_t) on its stream right before and right after the rasterizer
+ * kernel, so that the pair times that kernel alone.  NULL, NULL clears it.
+ * One setting per process, not per thread. */
+int tgs_set_raster_events(int which, void *begin, void *end);
 /* Bytes of one texture-gradient record (see tgs_rasterize_bwd). */
 size_t tgs_tex_record_bytes(int dtype);
 /* inclusive[i] = sum_{j <= i} counts[j] (int64); temp of tgs_isect_count_temp_bytes(n)

@@ -29,12 +29,15 @@ def _stages(dtype):
                                            (torch.float32, "exact", "default"), (torch.float64, "none", "default"),
                                            (torch.float64, "exact", "default"), (torch.float32, "none", "records"),
                                            (torch.float64, "exact", "records"), (torch.float32, "none", "atomics"),
-                                           (torch.float32, "exact", "hybrid"), (torch.float64, "none", "hybrid")])
+                                           (torch.float32, "exact", "hybrid"), (torch.float64, "none", "hybrid"),
+                                           (torch.float32, "none", "dense_fill")])
 def test_config1_full_frame(dtype, uv, path, monkeypatch):
     """Texture-gradient paths: "records" - every texture pair through per-pair records and
     the row gather (the unlisted-row fill split between forward and backward);
     "atomics" - float atomics only; "hybrid" - rows with at least 16 pairs through
-    records (many rows at this size), the others atomics; "default" - as shipped."""
+    records (many rows at this size), the others atomics; "dense_fill" - float atomics
+    with the gradient zero-filled by one launch before the forward instead of along the
+    rasterizer walks; "default" - as shipped."""
     from textured_gaussians_amd.cuda import _wrapper
     if path == "records":
         monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
@@ -43,6 +46,8 @@ def test_config1_full_frame(dtype, uv, path, monkeypatch):
         monkeypatch.setattr(_wrapper, "HEAVY_ROW_PAIRS", 0)
     elif path == "hybrid":
         monkeypatch.setattr(_wrapper, "HEAVY_ROW_PAIRS", 16)
+    elif path == "dense_fill":
+        monkeypatch.setattr(_wrapper, "FUSED_FILL", False)
     st = _stages(dtype)
     assert st["flatten_ids"].numel() > 50_000
     s = st["scene"]

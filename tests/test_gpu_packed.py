@@ -163,3 +163,25 @@ def test_packed_rasterizer_matches_oracle_on_every_tile(dtype, textured, uv):
     stats = raster_tile_parity(st, colors, opac, tex, bg, dtype, all_tiles(st), uv=uv, gaussian_ids=gids,
                                n_gaussians=N)
     print("packed", dtype, textured, uv, stats)
+
+
+def test_packed_gaussian_ids_out_of_range_are_refused():
+    """An entry naming a Gaussian past the texture rows (or a negative one) is refused on
+    the host before any kernel runs."""
+    sc = scenes.make_scene(400, 64, 48, 50.0, CAMS, 8, logscale_mu=-3.0, logscale_sigma=0.4, sh_degree=None, seed=1)
+    st = gpu_stages_packed(sc, torch.float32)
+    gids = st["gaussian_ids"]
+    nnz, N = gids.numel(), sc.means.shape[0]
+    colors = torch.rand(nnz, 3, device=DEV)
+    opac = st["scene"].opacities[gids].contiguous()
+    for bad in (N, -1):
+        ids = gids.clone()
+        ids[nnz // 2] = bad
+        with pytest.raises(RuntimeError, match="gaussian_ids must lie in"):
+            _ops._raster_fwd(True, st["means2d"], st["rt"], colors, opac, st["scene"].textures, 3.0, 3.0,
+                             st["normals"], None, None, 64, 48, 16, st["offsets"], st["flatten_ids"], 0.0, 1.0,
+                             gaussian_ids=ids)
+    # in range: runs
+    _ops._raster_fwd(True, st["means2d"], st["rt"], colors, opac, st["scene"].textures, 3.0, 3.0, st["normals"],
+                     None, None, 64, 48, 16, st["offsets"], st["flatten_ids"], 0.0, 1.0, gaussian_ids=gids)
+    torch.cuda.synchronize()

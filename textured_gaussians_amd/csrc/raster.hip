@@ -1529,9 +1529,19 @@ static void launch_pack(const RasterArgs<S> &a, hipStream_t st) {
     pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.N, a.gids, a.means2d, a.rt, a.opac, a.colors,
                                                                     a.normals, (S *)a.rec);
 }
+// Measurement hook (tgs_set_raster_events): events recorded on the launch stream right
+// before and after the rasterizer kernel itself, [0] forward, [1] backward.
+static hipEvent_t g_raster_ev[2][2];
+
+static void record_raster_event(int which, int end, hipStream_t st) {
+    if (g_raster_ev[which][end]) (void)hipEventRecord(g_raster_ev[which][end], st);
+}
+
 template <typename S, int K, int TEX> static void launch_fwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
     launch_pack<S, K>(a, st);
+    record_raster_event(0, 0, st);
     raster_fwd_kernel<S, K, TEX><<<grid, bs, 0, st>>>(a);
+    record_raster_event(0, 1, st);
 }
 template <typename S, int K, int TEX> static void launch_bwd(const RasterArgs<S> &a, unsigned grid, int bs, hipStream_t st) {
     launch_pack<S, K>(a, st);
@@ -1543,7 +1553,9 @@ template <typename S, int K, int TEX> static void launch_bwd(const RasterArgs<S>
             raised = hipFuncSetAttribute((const void *)raster_bwd_kernel<S, K, TEX>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage) == hipSuccess;
     }
+    record_raster_event(1, 0, st);
     raster_bwd_kernel<S, K, TEX><<<grid, bs, stage, st>>>(a);
+    record_raster_event(1, 1, st);
 }
 
 template <typename S, bool BWD>
@@ -1726,6 +1738,14 @@ static size_t workspace_bytes(int dtype, int64_t C, int64_t N) {
 using namespace tgs;
 
 extern "C" size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N) { return workspace_bytes(dtype, C, N); }
+
+extern "C" int tgs_set_raster_events(int which, void *begin, void *end) {
+    TGS_CHECK_ARG(which == 0 || which == 1, "set_raster_events: which must be 0 (forward) or 1 (backward)");
+    TGS_CHECK_ARG((begin == nullptr) == (end == nullptr), "set_raster_events: give both events or neither");
+    g_raster_ev[which][0] = (hipEvent_t)begin;
+    g_raster_ev[which][1] = (hipEvent_t)end;
+    return TGS_OK;
+}
 
 extern "C" size_t tgs_tex_record_bytes(int dtype) {
     return dtype == TGS_F64 ? sizeof(TexRecord<double>) : sizeof(TexRecord<float>);

@@ -36,20 +36,45 @@ def set_kernel_timer(store: Optional[dict]) -> None:
     _TIMER = store
 
 
-def _timed(name, t: Tensor):
-    class _Ctx:
-        def __enter__(self):
-            if _TIMER is not None:
-                st = torch.cuda.current_stream(t.device)
-                self.e0 = torch.cuda.Event(enable_timing=True)
-                self.e1 = torch.cuda.Event(enable_timing=True)
-                self.e0.record(st)
+_RASTER_EVENTS = {"raster_fwd": 0, "raster_bwd": 1}
 
-        def __exit__(self, *exc):
-            if _TIMER is not None:
-                self.e1.record(torch.cuda.current_stream(t.device))
-                _TIMER.setdefault(name, []).append((self.e0, self.e1))
-    return _Ctx()
+
+class _Timed:
+    """HIP events on an op's stream: around the rasterizer kernel alone for
+    "raster_fwd" / "raster_bwd" (recorded by the library, tgs_set_raster_events),
+    around the whole op otherwise.  Holds the device only: a reference to the op's
+    tensor would outlive the op (and a gradient buffer with an extra reference is
+    copied by autograd instead of handed over)."""
+    __slots__ = ("name", "device", "e0", "e1")
+
+    def __init__(self, name, device):
+        self.name, self.device = name, device
+
+    def __enter__(self):
+        if _TIMER is not None:
+            st = torch.cuda.current_stream(self.device)
+            self.e0 = torch.cuda.Event(enable_timing=True)
+            self.e1 = torch.cuda.Event(enable_timing=True)
+            self.e0.record(st)
+            which = _RASTER_EVENTS.get(self.name)
+            if which is not None:
+                # recorded once so that they exist; the library records them again
+                self.e1.record(st)
+                check(lib().tgs_set_raster_events(which, self.e0.cuda_event, self.e1.cuda_event),
+                      "set_raster_events")
+
+    def __exit__(self, *exc):
+        if _TIMER is not None:
+            which = _RASTER_EVENTS.get(self.name)
+            if which is None:
+                self.e1.record(torch.cuda.current_stream(self.device))
+            else:
+                check(lib().tgs_set_raster_events(which, None, None), "set_raster_events")
+            _TIMER.setdefault(self.name, []).append((self.e0, self.e1))
+
+
+def _timed(name, t: Tensor):
+    return _Timed(name, t.device)
 
 
 def _stream(t: Tensor):
@@ -436,6 +461,12 @@ def _packed_args(means2d, gaussian_ids, textures, n_gaussians):
         N = int(n_gaussians)
     else:
         N = int(gaussian_ids.max().item()) + 1 if gaussian_ids.numel() else 1
+    if gaussian_ids.numel():
+        # an id outside [0, N) would address texture rows and counters past their ends
+        lo, hi = torch.stack(torch.aminmax(gaussian_ids)).tolist()
+        if lo < 0 or hi >= N:
+            raise RuntimeError(f"gaussian_ids must lie in [0, {N}) (one per texture row / Gaussian); "
+                               f"they span [{lo}, {hi}]")
     if gaussian_ids.numel() == 0:
         # no entries: the C ABI tells the packed layout by a non-null gaussian_ids, and an
         # empty tensor has none; the one placeholder id is never read (nnz = 0)
@@ -884,7 +915,7 @@ def zero_fill_(t: Tensor) -> Tensor:
     the current stream (no reference counterpart; used for the texture
     gradient, which the backward accumulates into)."""
     _check_input(t, "t")
-    with torch.cuda.device(t.device):
+    with torch.cuda.device(t.device), _timed("grad_fill", t):
         check(lib().tgs_zero_fill(_p(t), t.numel() * t.element_size(), _stream(t)), "zero_fill")
     return t
 

@@ -369,6 +369,10 @@ HEAVY_ROW_PAIRS = 0
 # rasterizer walks: the forward takes the rows whose index mod 8 is in this set
 # (bit r), the backward the others; the gather writes the listed rows.
 FWD_FILL_RESIDUES = 0xFF
+# Float-atomic path: the texture gradient's zero-fill rides along the rasterizer walks
+# (True), or is one dense fill launch before the forward (False; same result, used to
+# time the rasterizer kernels without the fill).
+FUSED_FILL = True
 
 
 def _records_fit(textures: Tensor, dtype: torch.dtype) -> bool:
@@ -415,6 +419,9 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
                 bwd_res = 0xFF & ~fwd_res
                 # the backward's share; mask value 2 (no row) when the forward took every residue
                 ctx.v_tex = (v_tex, rows, (bwd_res << 8) if bwd_res else 2)
+            elif not FUSED_FILL:  # one dense fill; the backward adds in place
+                _C.zero_fill_(v_tex)
+                ctx.v_tex = (v_tex, None, None)
             else:  # the forward zero-fills the listed rows and the backward adds in place
                 zero_rows = (v_tex, rows, 1)
                 ctx.v_tex = (v_tex, rows, 0)
@@ -458,7 +465,7 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             v_tex_buf = False  # records only
         elif ctx.v_tex is not None:
             v_tex_buf = ctx.v_tex[0]
-            zero_rows = ctx.v_tex
+            zero_rows = ctx.v_tex if ctx.v_tex[1] is not None else None
             ctx.v_tex = None
         (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, v_normals,
          v_densify) = _make_lazy_cuda_func(ctx.bwd_fn)(
