@@ -74,8 +74,9 @@ int tgs_abi_version(void);
 /* Zero-fills nbytes bytes at dst (16-byte aligned, nbytes a multiple of 4) in
  * one launch of non-temporal stores.  No reference counterpart: the reference
  * allocates the texture gradient with torch::zeros inside its backward
- * (rasterize_to_pixels_textured_gaussians_2dgs_bwd.cu); here the autograd
- * layer fills it on a side stream while the forward runs.                */
+ * (rasterize_to_pixels_textured_gaussians_2dgs_bwd.cu).  The autograd layer
+ * normally fuses the fill into the rasterizer kernels (zero_rows_* below);
+ * this dense fill is its alternative (same result, one launch).           */
 int tgs_zero_fill(void *dst, size_t nbytes, tgs_stream_t stream);
 
 /* Row-selective variant for the texture gradient [N, row_bytes]: mask[N] (uint8)
