@@ -19,6 +19,8 @@ Modules
                 the reference's ``bilinear`` / ``bilinear_bwd2`` gradient rules).
 ``render``      the full pipeline (projection -> isect -> SH -> raster).
 ``adam``        the reference's selective Adam update (no bias correction).
+``ssim``        the trainer's SSIM (torchmetrics' published algorithm), checker of the
+                fused SSIM kernels.
 
 Parity pins (what checks each function, besides the GPU comparison)
 -------------------------------------------------------------------

@@ -1,5 +1,6 @@
-"""CPU checks of the training-step helpers around the hot path: the SSIM term of the
-trainer's loss and the float32-master split of the bfloat16 texture state.
+"""CPU checks of the training-step helpers around the hot path: the oracle's SSIM
+(the checker of the fused SSIM kernels) and the float32-master split of the bfloat16
+texture state; the product's SSIM raises on CPU tensors.
 
 SSIM (torchmetrics, not vendored: parity with it is unpinned) is pinned against a
 brute-force float64 restatement of its published algorithm (a direct 2-D window sum
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import ssim as O
 from textured_gaussians_amd import losses
 from textured_gaussians_amd.texture_adam import join_master, split_master
 
@@ -36,7 +38,7 @@ def _ssim_bruteforce(x, y, size=11, sigma=1.5, c1=1e-4, c2=9e-4):
 
 
 def test_gaussian_window_is_normalised_and_symmetric():
-    g = losses.gaussian_window(11, 1.5, torch.float64)
+    g = O.gaussian_window(11, 1.5, torch.float64)
     assert g.shape == (11,) and abs(float(g.sum()) - 1) < 1e-15
     assert torch.equal(g, g.flip(0)) and int(g.argmax()) == 5
     assert abs(float(g[5] / g[6]) - math.exp(0.5 / 1.5 ** 2)) < 1e-12
@@ -46,33 +48,32 @@ def test_ssim_matches_bruteforce_window_sum():
     gen = torch.Generator().manual_seed(0)
     x = torch.rand(2, 3, 17, 21, generator=gen, dtype=torch.float64)
     y = (x + 0.2 * torch.randn(2, 3, 17, 21, generator=gen, dtype=torch.float64)).clamp(0, 1)
-    got = float(losses.ssim(x, y))
+    got = float(O.ssim(x, y))
     want = np.mean([_ssim_bruteforce(x[b].numpy(), y[b].numpy()) for b in range(2)])
     assert abs(got - want) < 1e-12, (got, want)
-    assert abs(float(losses.ssim(y, x)) - got) < 1e-14  # symmetric
+    assert abs(float(O.ssim(y, x)) - got) < 1e-14  # symmetric
 
 
 def test_ssim_fixed_points():
     gen = torch.Generator().manual_seed(1)
     x = torch.rand(1, 3, 16, 16, generator=gen, dtype=torch.float64)
-    assert abs(float(losses.ssim(x, x)) - 1.0) < 1e-12
-    assert float(losses.ssim(x, 1 - x)) < 0.5
+    assert abs(float(O.ssim(x, x)) - 1.0) < 1e-12
+    assert float(O.ssim(x, 1 - x)) < 0.5
 
 
 def test_ssim_gradient_is_the_derivative():
     gen = torch.Generator().manual_seed(2)
     x = torch.rand(1, 2, 12, 13, generator=gen, dtype=torch.float64, requires_grad=True)
     y = torch.rand(1, 2, 12, 13, generator=gen, dtype=torch.float64)
-    assert torch.autograd.gradcheck(lambda a: losses.ssim(a, y), (x,), eps=1e-6, atol=1e-8)
+    assert torch.autograd.gradcheck(lambda a: O.ssim(a, y), (x,), eps=1e-6, atol=1e-8)
 
 
-def test_l1_dssim_loss_formula():
-    gen = torch.Generator().manual_seed(3)
-    r = torch.rand(1, 14, 15, 3, generator=gen, dtype=torch.float64)
-    t = torch.rand(1, 14, 15, 3, generator=gen, dtype=torch.float64)
-    s = _ssim_bruteforce(t[0].permute(2, 0, 1).numpy(), r[0].permute(2, 0, 1).numpy())
-    want = 0.8 * float((r - t).abs().mean()) + 0.2 * (1 - s)
-    assert abs(float(losses.l1_dssim_loss(r, t)) - want) < 1e-12
+def test_product_ssim_has_no_cpu_path():
+    x = torch.rand(1, 12, 12, 3)
+    with pytest.raises(RuntimeError, match="GPU tensors"):
+        losses.ssim_hwc(x, x)
+    with pytest.raises(RuntimeError, match="GPU tensors"):
+        losses.l1_dssim_loss(x, x)
 
 
 def test_master_split_round_trips_bit_exactly():

@@ -2,49 +2,19 @@
 (examples/simple_trainer_textured_gaussians.py:1424-1429, lambda = ssim_lambda = 0.2).
 
 The trainer's SSIM is torchmetrics' ``StructuralSimilarityIndexMeasure(data_range=1.0)``
-([TR]:430), a third-party module that is not vendored.  Its published algorithm,
-restated here: an 11-tap Gaussian window (sigma 1.5, normalised) applied to x, y,
-x^2, y^2, xy per channel; c1 = (0.01 R)^2, c2 = (0.03 R)^2;
-SSIM = (2 mu_x mu_y + c1)(2 s_xy + c2) / ((mu_x^2 + mu_y^2 + c1)(s_x^2 + s_y^2 + c2));
-the map is averaged over the windows that lie inside the image (torchmetrics pads by
-reflection and then crops the padded border away, which leaves exactly those), then
-over images.  Variances are clamped at 0.
+([TR]:430): an 11-tap Gaussian window (sigma 1.5) over x, y, x^2, y^2, xy per channel,
+c1 = (0.01 R)^2, c2 = (0.03 R)^2, the map averaged over the windows inside the image,
+variances clamped at 0.  Here it runs as the fused gfx950 kernels (``tgs_ssim_fwd`` /
+``tgs_ssim_bwd``); there is no other implementation in the package.
 """
 import torch
-import torch.nn.functional as F
 from torch import Tensor
 
 
-def gaussian_window(size: int = 11, sigma: float = 1.5, dtype=torch.float32, device=None) -> Tensor:
-    d = torch.arange((1 - size) / 2, (1 + size) / 2, 1.0, dtype=dtype, device=device)
-    g = torch.exp(-0.5 * (d / sigma) ** 2)
-    return g / g.sum()
-
-
-def ssim(preds: Tensor, target: Tensor, data_range: float = 1.0, kernel_size: int = 11, sigma: float = 1.5,
-         k1: float = 0.01, k2: float = 0.03) -> Tensor:
-    """Mean SSIM of [B, C, H, W] images (H, W >= kernel_size)."""
-    assert preds.shape == target.shape and preds.dim() == 4, (preds.shape, target.shape)
-    B, C, H, W = preds.shape
-    assert H >= kernel_size and W >= kernel_size, "image smaller than the SSIM window"
-    g = gaussian_window(kernel_size, sigma, preds.dtype, preds.device)
-    kv = g.view(1, 1, -1, 1).expand(C, 1, kernel_size, 1).contiguous()
-    kh = g.view(1, 1, 1, -1).expand(C, 1, 1, kernel_size).contiguous()
-    x = torch.cat([preds, target, preds * preds, target * target, preds * target])  # [5B, C, H, W]
-    y = F.conv2d(F.conv2d(x, kv, groups=C), kh, groups=C)  # separable window, valid positions only
-    mu_p, mu_t, e_pp, e_tt, e_pt = y.split(B)
-    c1, c2 = (k1 * data_range) ** 2, (k2 * data_range) ** 2
-    mu_pp, mu_tt, mu_pt = mu_p * mu_p, mu_t * mu_t, mu_p * mu_t
-    s_pp = (e_pp - mu_pp).clamp_min(0.0)
-    s_tt = (e_tt - mu_tt).clamp_min(0.0)
-    s_pt = e_pt - mu_pt
-    ssim_map = ((2 * mu_pt + c1) * (2 * s_pt + c2)) / ((mu_pp + mu_tt + c1) * (s_pp + s_tt + c2))
-    return ssim_map.reshape(B, -1).mean(-1).mean()
-
-
 class _FusedSSIM(torch.autograd.Function):
-    """SSIM of channels-last [B, H, W, C] images by the gfx950 kernels (the same
-    formula as ``ssim``); differentiable in the first argument."""
+    """SSIM of channels-last [B, H, W, C] images by the gfx950 kernels.  The kernels
+    differentiate in the first image; SSIM is symmetric, so the second image's gradient
+    is the first-image gradient of SSIM(y, x)."""
 
     @staticmethod
     def forward(ctx, x, y):
@@ -57,15 +27,25 @@ class _FusedSSIM(torch.autograd.Function):
     def backward(ctx, g):
         from .cuda import _ops
         x, y, work = ctx.saved_tensors
-        return _ops.ssim_bwd(x, y, work) * g, None
+        gx = _ops.ssim_bwd(x, y, work) * g if ctx.needs_input_grad[0] else None
+        gy = None
+        if ctx.needs_input_grad[1]:
+            _, work_yx = _ops.ssim_fwd(y, x)
+            gy = _ops.ssim_bwd(y, x, work_yx) * g
+        return gx, gy
 
 
 def ssim_hwc(x: Tensor, y: Tensor) -> Tensor:
-    """Mean SSIM of channels-last [B, H, W, C] images; the fused kernels on the GPU
-    (gradient with respect to ``x`` only), ``ssim`` elsewhere."""
-    if x.is_cuda and x.dtype in (torch.float32, torch.float64) and not y.requires_grad and x.shape[-1] <= 4:
-        return _FusedSSIM.apply(x.contiguous(), y.contiguous())
-    return ssim(x.permute(0, 3, 1, 2), y.permute(0, 3, 1, 2))
+    """Mean SSIM of channels-last [B, H, W, C] images (C <= 4, float32 or float64, on
+    the GPU), differentiable in both images."""
+    if not (x.is_cuda and y.is_cuda):
+        raise RuntimeError("ssim_hwc: the fused SSIM kernels need GPU tensors")
+    if x.dtype not in (torch.float32, torch.float64) or y.dtype != x.dtype:
+        raise RuntimeError(f"ssim_hwc: float32 or float64 images of one dtype expected, got {x.dtype}, {y.dtype}")
+    if x.shape != y.shape or x.dim() != 4 or not 1 <= x.shape[-1] <= 4:
+        raise RuntimeError(f"ssim_hwc: [B, H, W, C] images with C <= 4 expected, got {tuple(x.shape)}, "
+                           f"{tuple(y.shape)}")
+    return _FusedSSIM.apply(x.contiguous(), y.contiguous())
 
 
 def l1_dssim_loss(render: Tensor, target: Tensor, ssim_lambda: float = 0.2) -> Tensor:
