@@ -12,7 +12,7 @@ from oracle.raster import rasterize_to_pixels
 from textured_gaussians_amd import scenes
 from textured_gaussians_amd.cuda import _ops
 
-from gpu_util import DEV, all_tiles, cpu64, gpu_stages, raster_tile_parity, rel_err
+from gpu_util import DEV, all_tiles, cpu64, gpu_stages, gpu_stages_packed, raster_tile_parity, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -127,3 +127,41 @@ def test_mip2_whole_frame_matches_oracle(dtype):
                                all_tiles(st), filtering="mipmapped2", zero_fragile_cotangents=True)
     assert stats["cotangent_zeroed_pixels"] < 0.01 * 256 * 256
     print("mip2", dtype, stats)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_mip2_packed_matches_oracle_on_every_tile(dtype):
+    """mipmapped2 in the packed layout: two cameras, entry i sampling the pyramid of
+    Gaussian gaussian_ids[i]; every tile against the oracle (same rules as above; a
+    texture row takes the fragility flags of all its cameras' entries, so more cameras
+    put more rows under the norm-only rule)."""
+    cams = [(3.5, 0.3, 0.4), (-3.0, 1.5, 0.8)]
+    sc = scenes.make_scene(2500, 96, 80, 50.0, cams, 16, logscale_mu=-3.0, logscale_sigma=0.4, sh_degree=None,
+                           seed=12)
+    st = gpu_stages_packed(sc, dtype)
+    gids = st["gaussian_ids"]
+    nnz, C = gids.numel(), len(cams)
+    assert nnz > sc.means.shape[0]
+    g = torch.Generator().manual_seed(4)
+    colors = torch.rand(nnz, 3, generator=g, dtype=dtype).to(DEV)
+    opac = st["scene"].opacities[gids].contiguous()
+    bg = torch.rand(C, 3, generator=g, dtype=dtype).to(DEV)
+    stats = raster_tile_parity(st, colors, opac, st["scene"].textures, bg, dtype, all_tiles(st),
+                               filtering="mipmapped2", gaussian_ids=gids, zero_fragile_cotangents=True)
+    print("mip2 packed", dtype, stats)
+
+
+def test_mipmapped2_packed_through_the_api_equals_grid():
+    sc = scenes.make_scene(800, 64, 48, 50.0, [(3.5, 0.3, 0.4), (-3.0, 1.5, 0.8)], 16, logscale_mu=-3.0,
+                           logscale_sigma=0.4, sh_degree=1, seed=5).to(DEV, torch.float64)
+    res = []
+    for packed in (False, True):
+        tex = sc.textures.clone().requires_grad_(True)
+        m = sc.means.clone().requires_grad_(True)
+        out = tg.rasterization_textured_gaussians(m, sc.quats, sc.scales, sc.opacities, sc.colors, tex, sc.viewmats,
+                                                  sc.Ks, sc.width, sc.height, sh_degree=1,
+                                                  backgrounds=sc.backgrounds, filtering="mipmapped2", packed=packed)
+        (out[0] * sc.target).sum().backward()
+        res.append((out[0].detach(), tex.grad, m.grad))
+    assert rel_err(res[1][0], res[0][0]) < 1e-12
+    assert rel_err(res[1][1], res[0][1]) < 1e-10 and rel_err(res[1][2], res[0][2]) < 1e-10

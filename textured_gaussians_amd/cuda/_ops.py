@@ -839,13 +839,14 @@ def rasterize_to_pixels_fwd_mip2_textured_gaussians_full(means2d, ray_transforms
                                                          log_texture_res, texture_range_x, texture_range_y, normals,
                                                          backgrounds, masks, image_width, image_height, tile_size,
                                                          tile_offsets, flatten_ids, gs_contrib_threshold, g_weight,
-                                                         live_entries=None, zero_rows=None):
+                                                         live_entries=None, zero_rows=None, gaussian_ids=None):
     """As above plus the distortion accumulator and the final transmittance (11 outputs,
-    as ``rasterize_to_pixels_fwd_textured_gaussians_full``)."""
+    as ``rasterize_to_pixels_fwd_textured_gaussians_full``); ``gaussian_ids``: the packed
+    layout."""
     return _raster_fwd(True, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
                        texture_range_y, normals, backgrounds, masks, image_width, image_height, tile_size,
                        tile_offsets, flatten_ids, gs_contrib_threshold, g_weight, live_entries=live_entries,
-                       zero_rows=zero_rows, mip_log_res=log_texture_res)
+                       zero_rows=zero_rows, mip_log_res=log_texture_res, gaussian_ids=gaussian_ids)
 
 
 def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, colors, opacities, textures,
@@ -855,7 +856,7 @@ def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, col
                                                     render_alphas, last_ids, median_ids, v_render_colors,
                                                     v_render_alphas, v_render_normals, v_render_distort,
                                                     v_render_median, absgrad, live_entries=None, v_textures_out=None,
-                                                    zero_rows=None, render_trans=None):
+                                                    zero_rows=None, render_trans=None, gaussian_ids=None):
     """The ``mipmapped2`` backward (ext.cpp:139-140): the reference's rule (no
     d(sample)/d(position) term); v_textures has the pyramid's shape."""
     return _raster_bwd(True, UV_NONE, means2d, ray_transforms, colors, opacities, textures, texture_range_x,
@@ -863,7 +864,8 @@ def rasterize_to_pixels_bwd_mip2_textured_gaussians(means2d, ray_transforms, col
                        tile_offsets, flatten_ids, g_weight, render_colors, render_alphas, last_ids, median_ids,
                        v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
                        absgrad, live_entries=live_entries, v_textures_out=v_textures_out, zero_rows=zero_rows,
-                       mip_log_res=log_texture_res, render_trans=render_trans)
+                       mip_log_res=log_texture_res, render_trans=render_trans,
+                       gaussian_ids=gaussian_ids)
 
 
 # ------------------------------------------------------------------- SSIM

@@ -325,8 +325,8 @@ def rasterize_to_pixels_textured_gaussians(
     assert tile_width * tile_size >= image_width, f"Assert Failed: {tile_width} * {tile_size} >= {image_width}"
     if filtering == "mipmapped2":
         # [W]:1944-1989: square power-of-two textures, pyramid built (and differentiated) here
-        if packed or texture_adam is not None:
-            raise NotImplementedError("mipmapped2: packed layout / texture_adam not supported")
+        if texture_adam is not None:
+            raise NotImplementedError("mipmapped2: texture_adam not supported")
         assert textures.size(1) == textures.size(2)
         log_res = textures.size(1).bit_length() - 1
         assert textures.size(1) == 1 << log_res
@@ -335,7 +335,7 @@ def rasterize_to_pixels_textured_gaussians(
             means2d.contiguous(), ray_transforms.contiguous(), colors.contiguous(), opacities.contiguous(), mip,
             log_res, texture_range_x, texture_range_y, normals.contiguous(), densify.contiguous(), backgrounds, masks,
             image_width, image_height, tile_size, isect_offsets.contiguous(), flatten_ids.contiguous(), absgrad,
-            distloss, gs_contrib_threshold, g_weight)
+            distloss, gs_contrib_threshold, g_weight, gaussian_ids)
         render_colors = out[0]
         if padded > 0:
             render_colors = render_colors[..., :-padded]
@@ -519,23 +519,24 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means2d, ray_transforms, colors, opacities, mip_textures, log_texture_res, texture_range_x,
                 texture_range_y, normals, densify, backgrounds, masks, width, height, tile_size, isect_offsets,
-                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight):
+                flatten_ids, absgrad, distloss, gs_contrib_threshold, g_weight, gaussian_ids):
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
         # the pyramid's gradient, zero-filled along the rasterizer walks (listed rows by the
         # forward, the others by the backward), as for the bilinear filter
         ctx.v_tex = None
         if ctx.needs_input_grad[4] and mip_textures.is_cuda:
             v_tex = torch.empty(mip_textures.shape, dtype=means2d.dtype, device=mip_textures.device)
-            ctx.v_tex = (v_tex, _C.mark_rows(flatten_ids, mip_textures.shape[0]))
+            listed = flatten_ids if gaussian_ids is None else gaussian_ids[flatten_ids.long()].int()
+            ctx.v_tex = (v_tex, _C.mark_rows(listed, mip_textures.shape[0]))
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count, _, trans) = _make_lazy_cuda_func(
             "rasterize_to_pixels_fwd_mip2_textured_gaussians_full")(
             means2d, ray_transforms, colors, opacities, mip_textures, log_texture_res, texture_range_x,
             texture_range_y, normals, backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids,
-            gs_contrib_threshold, g_weight, live_entries=ctx.live, zero_rows=ctx.v_tex)
+            gs_contrib_threshold, g_weight, live_entries=ctx.live, zero_rows=ctx.v_tex, gaussian_ids=gaussian_ids)
         ctx.save_for_backward(means2d, ray_transforms, colors, opacities, mip_textures, normals, densify, backgrounds,
                               masks, isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids,
-                              trans)
+                              trans, gaussian_ids)
         ctx.log_texture_res = log_texture_res
         ctx.texture_range = (texture_range_x, texture_range_y)
         ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
@@ -548,7 +549,8 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
     def backward(ctx, v_render_colors, v_render_alphas, v_render_normals, v_render_distort, v_render_median,
                  v_gs_contrib_sum, v_gs_contrib_count):
         (means2d, ray_transforms, colors, opacities, mip_textures, normals, densify, backgrounds, masks,
-         isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids, trans) = ctx.saved_tensors
+         isect_offsets, flatten_ids, render_colors, render_alphas, last_ids, median_ids, trans,
+         gaussian_ids) = ctx.saved_tensors
         (v_means2d_abs, v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, v_normals,
          v_densify) = _make_lazy_cuda_func("rasterize_to_pixels_bwd_mip2_textured_gaussians")(
             means2d, ray_transforms, colors, opacities, mip_textures, ctx.log_texture_res, ctx.texture_range[0],
@@ -556,7 +558,8 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
             isect_offsets, flatten_ids, ctx.g_weight, render_colors, render_alphas, last_ids, median_ids,
             v_render_colors.contiguous(), v_render_alphas.contiguous(), v_render_normals.contiguous(),
             v_render_distort.contiguous(), v_render_median.contiguous(), ctx.absgrad, live_entries=ctx.live,
-            v_textures_out=ctx.v_tex[0] if ctx.v_tex is not None else None, zero_rows=ctx.v_tex, render_trans=trans)
+            v_textures_out=ctx.v_tex[0] if ctx.v_tex is not None else None, zero_rows=ctx.v_tex, render_trans=trans,
+            gaussian_ids=gaussian_ids)
         ctx.live = None
         ctx.v_tex = None
         if ctx.absgrad:
@@ -565,7 +568,7 @@ class _RasterizeToPixelsMip2TexturedGaussians(torch.autograd.Function):
         if ctx.needs_input_grad[10]:
             v_backgrounds = (v_render_colors * (1.0 - render_alphas)).sum(dim=(1, 2))
         return (v_means2d, v_ray_transforms, v_colors, v_opacities, v_textures, None, None, None, v_normals,
-                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None)
+                v_densify, v_backgrounds, None, None, None, None, None, None, None, None, None, None, None)
 
 
 def rasterize_to_pixels_2dgs(means2d: Tensor, ray_transforms: Tensor, colors: Tensor, opacities: Tensor,
