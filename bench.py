@@ -100,16 +100,32 @@ def timed_steps(tg, sc, params, steps, warmup, textured, **kw):
     return dt * 1e3, kt, out
 
 
-def algorithmic_bytes(out, params, sc, textured, K=3):
+def texels_read(tg, sc, params):
+    """U: distinct (Gaussian, texel) pairs the forward reads, counted with a byte map the
+    forward marks (measurement hook of the library), on one extra iteration."""
+    from textured_gaussians_amd.cuda import _ops
+    tex = params["textures"]
+    marks = torch.zeros(tex.shape[:3], dtype=torch.uint8, device=tex.device)
+    _ops.set_texel_marks(marks)
+    try:
+        step(tg, sc, params, True)
+        torch.cuda.synchronize()
+    finally:
+        _ops.set_texel_marks(None)
+    U = int(marks.count_nonzero())
+    del marks
+    return U
+
+
+def algorithmic_bytes(out, params, sc, textured, K=3, U=None):
     """Per-launch algorithmic HBM bytes of the rasterizer forward and backward.
 
     I = intersections, R = 76 B per intersection record (4 id + 8 mean2d + 36
     ray transform + 4 opacity + 12 colour + 12 normal), P = pixels,
-    U = distinct (Gaussian, texel) pairs touched (16 B each; counted as the
-    texels with a nonzero gradient, a lower bound of the texels read),
-    V = visible (camera, Gaussian) pairs.
+    U = distinct (Gaussian, texel) pairs read (16 B each; `texels_read`),
+    U_g = texels with a nonzero gradient, V = visible (camera, Gaussian) pairs.
       forward : I*R + 16U + 4*C*tiles + P*(4K+32)
-      backward: I*R + 16U + P*(4K+36) + 32U + 2V*(68+4K)
+      backward: I*R + 16U + P*(4K+36) + 32U_g + 2V*(68+4K)
     """
     meta = out[-1]
     I = meta["flatten_ids"].numel()
@@ -117,14 +133,15 @@ def algorithmic_bytes(out, params, sc, textured, K=3):
     P = C * sc.width * sc.height
     tiles = meta["tile_width"] * meta["tile_height"]
     V = int((meta["radii"] > 0).sum())
-    U = 0
+    Ug = 0
     if textured:
         g = params["textures"].grad
-        U = int((g != 0).any(-1).sum())  # chunked would save memory; the grad is already resident
+        Ug = int((g != 0).any(-1).sum())  # the gradient is already resident
+    U = Ug if U is None else U
     R = 76
     bf = I * R + 16 * U + 4 * C * tiles + P * (4 * K + 32)
-    bb = I * R + 16 * U + P * (4 * K + 36) + 32 * U + 2 * V * (68 + 4 * K)
-    return dict(I=I, P=P, V=V, U=U, tiles=tiles, bytes_fwd=bf, bytes_bwd=bb)
+    bb = I * R + 16 * U + P * (4 * K + 36) + 32 * Ug + 2 * V * (68 + 4 * K)
+    return dict(I=I, P=P, V=V, U=U, U_g=Ug, tiles=tiles, bytes_fwd=bf, bytes_bwd=bb)
 
 
 def stream_rates(dev):
@@ -481,7 +498,7 @@ def main():
     print("[bench] inputs resident; timing textured steps", file=sys.stderr, flush=True)
 
     ms, kt, out = timed_steps(tg, sc, params, args.steps, args.warmup, True)
-    ab = algorithmic_bytes(out, params, sc, True)
+    ab = algorithmic_bytes(out, params, sc, True, U=texels_read(tg, sc, params))
     t_bwd = kt.get("raster_bwd", float("nan"))
     t_fwd = kt.get("raster_fwd", float("nan"))
     B = ab["bytes_fwd"] + ab["bytes_bwd"]
@@ -528,7 +545,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene of the headline workload)",
         "config": {"workload": "textured-2DGS fwd+bwd, N=1M, 50x50 RGBA f32 textures, SH3, 1297x840, 1 camera",
                    "n_gaussians": args.n, "width": sc.width, "height": sc.height, "tile": 16,
-                   "n_isects": ab["I"], "visible": ab["V"], "texels_touched": ab["U"]},
+                   "n_isects": ab["I"], "visible": ab["V"], "texels_read": ab["U"], "texels_with_gradient": ab["U_g"]},
         "mpix_per_s": round(ab["P"] / (ms * 1e-3) / 1e6, 2),
         "raster_ms": {"fwd": round(t_fwd, 3), "bwd": round(t_bwd, 3)},
         "raster_hbm_frac_fwd_bwd": round(achieved_pair / HBM_PEAK_GBS, 4),

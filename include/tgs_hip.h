@@ -271,6 +271,12 @@ size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
  * kernel, so that the pair times that kernel alone.  NULL, NULL clears it.
  * One setting per process, not per thread. */
 int tgs_set_raster_events(int which, void *begin, void *end);
+/* Measurement hook (no reference counterpart): while set, tgs_rasterize_fwd with
+ * float textures (no bf16, no mip2) stores 1 into marks[row * tex_h * tex_w + texel]
+ * for every texel it reads (uint8[n_texels], n_texels >= N * tex_h * tex_w; caller
+ * zero-fills); the number of nonzero bytes is the count of distinct texels read.
+ * NULL, 0 clears it.  One setting per process. */
+int tgs_set_texel_marks(uint8_t *marks, int64_t n_texels);
 /* Bytes of one texture-gradient record (see tgs_rasterize_bwd). */
 size_t tgs_tex_record_bytes(int dtype);
 /* inclusive[i] = sum_{j <= i} counts[j] (int64); temp of tgs_isect_count_temp_bytes(n)

@@ -111,6 +111,12 @@ def test_host_validation_errors_without_gpu():
     assert rc == 1 and b"null operand" in L.tgs_last_error()
     rc = L.tgs_sigmoid_bwd_colsum(10, 6, null, null, null, 1, null, null)
     assert rc == 1 and b"multiple of 4" in L.tgs_last_error()
+    # the measurement hook: forward (0) or backward (1), both events or neither
+    rc = L.tgs_set_raster_events(2, null, null)
+    assert rc == 1 and b"which" in L.tgs_last_error()
+    rc = L.tgs_set_raster_events(0, 1234, null)
+    assert rc == 1 and b"both events" in L.tgs_last_error()
+    assert L.tgs_set_raster_events(1, null, null) == 0
 
 
 def test_ops_reject_cpu_and_noncontiguous_tensors():

@@ -422,3 +422,41 @@ def test_saturated_frame_f32(uv):
     fo = None
     stats = raster_tile_parity(st, colors, opac, s.textures, s.backgrounds, torch.float32, all_tiles(st), uv=uv)
     print("saturated", uv, stats)
+
+
+def test_texel_marks_cover_the_gradient_and_change_nothing():
+    """The measurement hook that counts distinct texels read (tgs_set_texel_marks): the
+    forward's outputs are bit-identical with it set, every texel with a nonzero gradient
+    is marked, and only rows in the tile lists are."""
+    from textured_gaussians_amd.cuda import _wrapper
+    sc = scenes.config1(n=3000, width=96, height=80, tex_res=12, seed=4, sh_degree=None)
+    st = gpu_stages(sc, torch.float32)
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(1)
+    colors = torch.rand(C, N, 3, generator=g).to(DEV)
+    opac = st["scene"].opacities[None].contiguous()
+    tex = st["scene"].textures
+    s = st["scene"]
+
+    def render(marks):
+        _ops.set_texel_marks(marks)
+        try:
+            t = tex.detach().clone().requires_grad_(True)
+            out = _wrapper.rasterize_to_pixels_textured_gaussians(
+                st["means2d"], st["rt"], colors, opac, t, 3.0, 3.0, st["normals"], torch.zeros_like(st["means2d"]),
+                s.width, s.height, 16, st["offsets"], st["flatten_ids"], backgrounds=s.backgrounds)
+        finally:
+            _ops.set_texel_marks(None)
+        (out[0] * torch.rand(out[0].shape, generator=torch.Generator(device=DEV).manual_seed(2), device=DEV)).sum() \
+            .backward()
+        return [o.detach() for o in out[:5]], t.grad
+    marks = torch.zeros(tex.shape[:3], dtype=torch.uint8, device=DEV)
+    a, ga = render(None)
+    b, gb = render(marks)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+    touched = (gb != 0).any(-1)
+    assert bool(touched.any()) and not bool((touched & (marks == 0)).any())
+    listed = _ops.mark_rows(st["flatten_ids"], N).bool()
+    assert not bool(marks[~listed].any())
+    assert int(marks.count_nonzero()) >= int(touched.count_nonzero())

@@ -116,6 +116,9 @@ template <typename S> struct RasterArgs {
     // optional: per wave of a tile, one byte per list entry, nonzero where some pixel of
     // the wave composited the entry ([waves per tile][n_isects]; forward writes, backward reads)
     uint8_t *live;
+    // measurement only (tgs_set_texel_marks): one byte per texel of the textures,
+    // set to 1 by the forward for every texel it reads (native float textures)
+    uint8_t *tex_seen;
     // optional texture-gradient records: the forward counts per texture row the
     // (pixel, entry) pairs that will add texture gradient (tex_cnt); the backward
     // claims slots of each row's segment (rec_incl: inclusive prefix sum of the
@@ -668,6 +671,14 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
                     const Bilerp<S> bl = bilerp_setup<S>(p.sx, p.sy, a);
                     a_s = bilerp_blend<S, K>(bweights(bl),
                                              fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl), tex);
+                    if (TEX == kTexNative && a.tex_seen) {
+                        uint8_t *m = a.tex_seen + (int64_t)G.row * (a.tex_stride / 4);
+                        constexpr uint32_t tb = 4 * sizeof(S);
+                        m[bl.o00 / tb] = 1;
+                        m[(bl.o00 + bl.du) / tb] = 1;
+                        m[(bl.o00 + bl.dv) / tb] = 1;
+                        m[(bl.o00 + bl.du + bl.dv) / tb] = 1;
+                    }
                 }
             }
             const S alpha = pair_alpha(H.op, p.vis, a_s);
@@ -1529,6 +1540,10 @@ static void launch_pack(const RasterArgs<S> &a, hipStream_t st) {
     pack_kernel<S, K><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, a.N, a.gids, a.means2d, a.rt, a.opac, a.colors,
                                                                     a.normals, (S *)a.rec);
 }
+// Measurement hook (tgs_set_texel_marks): the byte map the forward marks texel reads in.
+static uint8_t *g_texel_marks;
+static int64_t g_texel_marks_n;
+
 // Measurement hook (tgs_set_raster_events): events recorded on the launch stream right
 // before and after the rasterizer kernel itself, [0] forward, [1] backward.
 static hipEvent_t g_raster_ev[2][2];
@@ -1747,6 +1762,14 @@ extern "C" int tgs_set_raster_events(int which, void *begin, void *end) {
     return TGS_OK;
 }
 
+extern "C" int tgs_set_texel_marks(uint8_t *marks, int64_t n_texels) {
+    TGS_CHECK_ARG((marks == nullptr) == (n_texels == 0) && n_texels >= 0,
+                  "set_texel_marks: give the map and its size, or NULL and 0");
+    g_texel_marks = marks;
+    g_texel_marks_n = n_texels;
+    return TGS_OK;
+}
+
 extern "C" size_t tgs_tex_record_bytes(int dtype) {
     return dtype == TGS_F64 ? sizeof(TexRecord<double>) : sizeof(TexRecord<float>);
 }
@@ -1836,6 +1859,8 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.cull = cull;                                                                                          \
         a.live = live_entries;                                                                                  \
         a.tex_cnt = textures ? tex_counts : nullptr;                                                            \
+        a.tex_seen = (tex_mode == kTexNative && g_texel_marks && g_texel_marks_n >= N * tex_h * tex_w)          \
+                         ? g_texel_marks : nullptr;                                                             \
         return dispatch<S, false>(a, K, tex_mode, st);                                                          \
     }
     if (dtype == TGS_F32) TGS_FWD(float)

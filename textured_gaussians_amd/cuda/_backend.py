@@ -62,6 +62,7 @@ _PROTOS = {
                                           c_dbl, c_dbl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                           c_size, c_int, vp]),
     "tgs_set_raster_events": (c_int, [c_int, vp, vp]),
+    "tgs_set_texel_marks": (c_int, [vp, c_i64]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
     "tgs_tex_record_bytes": (c_size, [c_int]),
     "tgs_mipmap_texels": (c_i64, [c_i32]),

@@ -912,6 +912,19 @@ def selective_adam_update(param, param_grad, exp_avg, exp_avg_sq, tiles_touched,
                                               float(b2), float(eps), _stream(param)), "selective_adam_update")
 
 
+def set_texel_marks(marks: Optional[Tensor]) -> None:
+    """Measurement only: while set (a zero-filled uint8 device tensor of at least
+    N * tex_h * tex_w bytes), the float-texture forward marks every texel it reads;
+    None clears it."""
+    if marks is None:
+        check(lib().tgs_set_texel_marks(None, 0), "set_texel_marks")
+        return
+    _check_input(marks, "marks")
+    if marks.dtype != torch.uint8:
+        raise RuntimeError("marks must be uint8")
+    check(lib().tgs_set_texel_marks(_p(marks), marks.numel()), "set_texel_marks")
+
+
 def zero_fill_(t: Tensor) -> Tensor:
     """Zero-fills a contiguous device tensor in place with one native launch on
     the current stream (no reference counterpart; used for the texture
