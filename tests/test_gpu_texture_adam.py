@@ -202,3 +202,28 @@ def test_training_step_config5_shape():
     assert texopt.steps == 3
     assert float((state.params["means"].detach() - m0).abs().max()) > 0
     assert float((texopt.master() - t0).abs().max()) > 0
+
+
+def test_texture_adam_in_the_packed_layout_matches_the_grid():
+    """texture_adam= with packed=True over three cameras: each entry's records go to its
+    Gaussian's row, so one step moves the textures, their moments and the geometry
+    gradients as the grid layout does (summation order aside)."""
+    cams = [(3.5, 0.3, 0.4), (-3.0, 1.5, 0.8), (0.5, -3.6, -0.6)]
+    sc = scenes.make_scene(1500, 72, 56, 50.0, cams, 8, logscale_mu=-3.0, logscale_sigma=0.4, sh_degree=1,
+                           seed=8).to(DEV)
+    res = []
+    for packed in (False, True):
+        opt = BF16TextureAdam.from_textures(sc.textures.float(), lr=1e-2)
+        means = sc.means.clone().requires_grad_(True)
+        out = tg.rasterization_textured_gaussians(means, sc.quats, sc.scales, sc.opacities, sc.colors, opt.textures,
+                                                  sc.viewmats, sc.Ks, sc.width, sc.height, sh_degree=1,
+                                                  backgrounds=sc.backgrounds, texture_adam=opt, packed=packed)
+        (out[0] - sc.target).abs().mean().backward()
+        assert opt.steps == 1
+        res.append((opt.master(), opt.exp_avg.float(), opt.exp_avg_sq.float(), means.grad))
+    (p0, m0, v0, g0), (p1, m1, v1, g1) = res
+    assert float((p0 - sc.textures.float()).abs().max()) > 0  # the step moved texels
+    # master values agree to float32 summation order; moments to their bfloat16 rounding
+    assert rel_err(p1, p0) < 1e-6
+    assert rel_err(m1, m0) < 1e-2 and rel_err(v1, v0) < 1e-2
+    assert rel_err(g1, g0) < 1e-5
