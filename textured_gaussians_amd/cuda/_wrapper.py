@@ -68,7 +68,9 @@ def spherical_harmonics_views(degrees_to_use: int, dirs: Tensor, coeffs: Tensor,
             assert masks.shape == dirs.shape[:-1], masks.shape
             masks = masks.contiguous()
         return _SphericalHarmonicsViews.apply(degrees_to_use, dirs.contiguous(), coeffs.contiguous(), masks)
-    return spherical_harmonics(degrees_to_use, dirs, coeffs.expand(C, *([-1] * coeffs.dim())), masks)
+    # one view: a leading axis by view (an expand's backward is a reduction over it)
+    views = coeffs[None] if C == 1 else coeffs.expand(C, *([-1] * coeffs.dim()))
+    return spherical_harmonics(degrees_to_use, dirs, views, masks)
 
 
 class _SphericalHarmonicsViews(torch.autograd.Function):

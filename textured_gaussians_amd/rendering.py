@@ -96,7 +96,8 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
         opacities = opacities.index_select(0, gaussian_ids)
     else:
         radii, means2d, depths, ray_transforms, normals = proj
-        opacities = opacities.repeat(C, 1)
+        # one camera: a view (repeat's backward would be a reduction over the camera axis)
+        opacities = opacities[None] if C == 1 else opacities.repeat(C, 1)
         camera_ids, gaussian_ids = None, None
     densify = torch.zeros_like(means2d, dtype=means.dtype, requires_grad=True, device=means.device)
     # per-view colours before the tile intersection: they need only the projection, and the
@@ -105,7 +106,12 @@ def _prepare(means, quats, scales, opacities, colors, viewmats, Ks, width, heigh
     if sh_degree is not None and not packed:
         # grid layout: one coefficient set seen from C views (the reference expands the
         # coefficients to [C, N, K, 3] first; the views kernel loops over them instead)
-        dirs = means[None, :, :] - camtoworlds[:, None, :3, 3]
+        if camtoworlds.shape[0] == 1:
+            # one camera: the gradient to the means is the directions' gradient itself (a
+            # broadcast over a camera axis would cost a reduction kernel in the backward)
+            dirs = (means - camtoworlds[0, :3, 3])[None]
+        else:
+            dirs = means[None, :, :] - camtoworlds[:, None, :3, 3]
         colors = spherical_harmonics_views(sh_degree, dirs, colors, masks=radii > 0)
         colors = torch.clamp_min(colors + 0.5, 0.0)
     else:
