@@ -109,14 +109,16 @@ def all_tiles(st):
                                       for tx in range(st["tw"])])
 
 
-REPORT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out",
-                      "parity_report.jsonl")
+# a JSON-lines file that parity failures are appended to, for reading offline (off unless set)
+REPORT = os.environ.get("TGS_PARITY_REPORT")
 
 
 def _report(rec):
-    """Append a failure record (one JSON line) for offline reading; best effort."""
+    """Append a failure record (one JSON line) to REPORT, when set; best effort."""
+    if not REPORT:
+        return
     try:
-        os.makedirs(os.path.dirname(REPORT), exist_ok=True)
+        os.makedirs(os.path.dirname(os.path.abspath(REPORT)), exist_ok=True)
         with open(REPORT, "a") as f:
             f.write(json.dumps(rec, default=float) + "\n")
     except OSError:
