@@ -292,15 +292,18 @@ def cpu_baseline(sc, meta_gpu, n_tiles_sample, runs=3):
 
 def grad_paths_leg(tg, sc, params, steps, warmup):
     """The headline iteration under each texture-gradient variant of the autograd layer
-    (all give the same gradients): float atomics into the row-filled gradient (the
-    product path), the same with one dense fill, per-pair records summed per row by the
-    gather kernel (the forward zero-filling every unlisted row), and records for the rows
-    with at least 64 pairs only.  The records variants run the rasterizer backward
-    without texture-gradient atomics; the gather runs after it, inside the backward op."""
+    (all give the same gradients): per-pair records summed per row by the gather kernel
+    (the product path: the backward walk zero-fills the unlisted rows), the same with the
+    forward walk filling them, float atomics into the row-filled gradient, the same with
+    one dense fill, and records for the rows with at least 64 pairs only (atomics for the
+    rest).  The records variants run the rasterizer backward without texture-gradient
+    atomics; the gather runs after it, inside the backward op."""
     from textured_gaussians_amd.cuda import _wrapper
-    variants = [("atomics_fused_fill", {}), ("atomics_dense_fill", {"FUSED_FILL": False}),
-                ("records_gather", {"TEXTURE_GRAD_RECORDS": True, "FWD_FILL_RESIDUES": 0xFF}),
-                ("records_rows_ge64", {"HEAVY_ROW_PAIRS": 64})]
+    variants = [("records_gather", {}),
+                ("records_forward_fills_unlisted", {"TEXTURE_GRAD_RECORDS": True, "FWD_FILL_RESIDUES": 0xFF}),
+                ("atomics_fused_fill", {"TEXTURE_GRAD_RECORDS": False}),
+                ("atomics_dense_fill", {"TEXTURE_GRAD_RECORDS": False, "FUSED_FILL": False}),
+                ("records_rows_ge64", {"TEXTURE_GRAD_RECORDS": False, "HEAVY_ROW_PAIRS": 64})]
     res = {}
     for name, cfg in variants:
         saved = {k: getattr(_wrapper, k) for k in cfg}
@@ -499,16 +502,23 @@ def main():
 
     ms, kt, out = timed_steps(tg, sc, params, args.steps, args.warmup, True)
     ab = algorithmic_bytes(out, params, sc, True, U=texels_read(tg, sc, params))
-    t_bwd = kt.get("raster_bwd", float("nan"))
+    # the backward's texture gradient is complete after the gather of its records (when
+    # the records path runs): the gather's time counts with the backward kernel's
+    t_gat = kt.get("tex_gather", 0.0)
+    t_bwd = kt.get("raster_bwd", float("nan")) + t_gat
     t_fwd = kt.get("raster_fwd", float("nan"))
     B = ab["bytes_fwd"] + ab["bytes_bwd"]
     achieved_pair = B / ((t_fwd + t_bwd) * 1e-3) / 1e9
     achieved_bwd = ab["bytes_bwd"] / (t_bwd * 1e-3) / 1e9
+    from textured_gaussians_amd.cuda import _wrapper
+    records = _wrapper._use_records(params["textures"], torch.float32)
+    listed_rows = int(torch.unique(out[-1]["flatten_ids"].long() % args.n).numel())
     meta_ref = {k: out[-1][k].detach() for k in ("means2d", "ray_transforms", "normals")}
     del out
-    # the same iteration with the texture gradient zero-filled by one dense launch before
-    # the forward: the rasterizer kernels without the fill, and the fill alone
-    from textured_gaussians_amd.cuda import _wrapper
+    # the same iteration with the texture gradient's zero-fill as one launch before the
+    # forward (the unlisted rows on the records path, whose gather writes the listed
+    # ones; all of it with float atomics): the rasterizer kernels without the fill, and
+    # the fill alone
     print(f"[bench] textured {ms:.2f} ms/iter; timing the dense-fill variant", file=sys.stderr, flush=True)
     _wrapper.FUSED_FILL = False
     try:
@@ -516,9 +526,11 @@ def main():
     finally:
         _wrapper.FUSED_FILL = True
     del fout
-    ff_fwd, ff_bwd = fkt.get("raster_fwd", float("nan")), fkt.get("raster_bwd", float("nan"))
+    ff_fwd = fkt.get("raster_fwd", float("nan"))
+    ff_bwd = fkt.get("raster_bwd", float("nan")) + fkt.get("tex_gather", 0.0)
     t_fill = fkt.get("grad_fill", float("nan"))
-    fill_bytes = params["textures"].numel() * 4
+    row_bytes = params["textures"][0].numel() * 4
+    fill_bytes = (args.n - listed_rows if records else args.n) * row_bytes
     achieved_ff = B / ((ff_fwd + ff_bwd) * 1e-3) / 1e9
     fill_gbs = fill_bytes / (t_fill * 1e-3) / 1e9
     plain = None
@@ -547,7 +559,9 @@ def main():
                    "n_gaussians": args.n, "width": sc.width, "height": sc.height, "tile": 16,
                    "n_isects": ab["I"], "visible": ab["V"], "texels_read": ab["U"], "texels_with_gradient": ab["U_g"]},
         "mpix_per_s": round(ab["P"] / (ms * 1e-3) / 1e6, 2),
-        "raster_ms": {"fwd": round(t_fwd, 3), "bwd": round(t_bwd, 3)},
+        "raster_ms": {"fwd": round(t_fwd, 3), "bwd": round(t_bwd - t_gat, 3), "gather": round(t_gat, 3)},
+        "texture_gradient": ("per-pair records + row gather (float64 LDS sums)" if records
+                             else "float atomics into the row-filled gradient"),
         "raster_hbm_frac_fwd_bwd": round(achieved_pair / HBM_PEAK_GBS, 4),
         "plain_2dgs": plain,
         # the rasterizer pair without the fused fill, and the fill as one dense launch
@@ -555,12 +569,15 @@ def main():
                              "achieved_GBs": round(achieved_ff, 1), "frac": round(achieved_ff / HBM_PEAK_GBS, 4),
                              "frac_of_stream_peak": round(achieved_ff / practical, 4)},
         "grad_fill": {"bytes": fill_bytes, "ms": round(t_fill, 3), "GBs": round(fill_gbs, 1),
-                      "note": "dense zero-fill of the 40 GB texture gradient (end-to-end bytes, outside B_f+B_b)"},
+                      "note": ("zero-fill of the texture-gradient rows absent from the tile lists (the gather writes "
+                               "the listed rows whole); end-to-end bytes, outside B_f+B_b" if records else
+                               "dense zero-fill of the 40 GB texture gradient (end-to-end bytes, outside B_f+B_b)")},
         "stream_GBs": streams,
         "roofline": {"bound": "hbm", "achieved": round(achieved_pair, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_pair / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "frac_of_stream_peak": round(achieved_pair / practical, 4), "stream_peak_GBs": practical,
-                     "kernel": "raster_fwd_kernel<float,3,1> + raster_bwd_kernel<float,3,1>",
+                     "kernel": ("raster_fwd_kernel<float,3,1> + raster_bwd_kernel<float,3,1>"
+                                + (" + tex_gather2_kernel" if records else "")),
                      "algorithmic_bytes": B, "bytes_fwd": ab["bytes_fwd"], "bytes_bwd": ab["bytes_bwd"],
                      "time_ms": round(t_fwd + t_bwd, 3)},
         "raster_bwd_only": {"achieved": round(achieved_bwd, 1), "frac": round(achieved_bwd / HBM_PEAK_GBS, 4)},

@@ -268,8 +268,10 @@ size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
 /* Measurement hook (no reference counterpart): while set, tgs_rasterize_fwd
  * (which = 0) or tgs_rasterize_bwd (which = 1) records `begin` and `end`
  * (hipEvent_t) on its stream right before and right after the rasterizer
- * kernel, so that the pair times that kernel alone.  NULL, NULL clears it.
- * One setting per process, not per thread. */
+ * kernel, so that the pair times that kernel alone; which = 2: around the
+ * texture-gradient gather that tgs_rasterize_bwd runs after its kernel when it
+ * writes records (row list and gather kernels).  NULL, NULL clears it.  One
+ * setting per process, not per thread. */
 int tgs_set_raster_events(int which, void *begin, void *end);
 /* Measurement hook (no reference counterpart): while set, tgs_rasterize_fwd with
  * float textures (no bf16, no mip2) stores 1 into marks[row * tex_h * tex_w + texel]
@@ -279,6 +281,12 @@ int tgs_set_raster_events(int which, void *begin, void *end);
 int tgs_set_texel_marks(uint8_t *marks, int64_t n_texels);
 /* Bytes of one texture-gradient record (see tgs_rasterize_bwd). */
 size_t tgs_tex_record_bytes(int dtype);
+/* 1 if tgs_rasterize_bwd sums texture-gradient records of this compute dtype and
+ * texture size with the streaming row gather (float32 compute, rows whose float64
+ * sums fit twice in one CU's LDS: up to about 50x50 texels), 0 if the records take
+ * the slower fallback gather.  No reference counterpart: callers choose between
+ * records and float atomics with it. */
+int tgs_texture_gather_fast(int dtype, int64_t tex_h, int64_t tex_w);
 /* inclusive[i] = sum_{j <= i} counts[j] (int64); temp of tgs_isect_count_temp_bytes(n)
  * bytes.  No reference counterpart (sizes the texture-gradient records). */
 int tgs_prefix_sum_i32(int64_t n, const int32_t *counts, int64_t *inclusive, void *temp,
@@ -349,9 +357,9 @@ int tgs_rasterize_fwd_noalloc(int dtype, int64_t C, int64_t N, int32_t K, const 
  * tex_counts from the forward of the same inputs, tex_rec_incl int64[N] its
  * inclusive prefix sum (tgs_prefix_sum_i32), tex_rec_cursor int32[N] zeroed,
  * tex_records of max(1, tex_rec_incl[N-1]) * tgs_tex_record_bytes(dtype) bytes.  The
- * backward then writes one record per pair instead of adding to v_textures in
- * L2, and a gather kernel sums each texture row's records in LDS and writes the
- * row once: rows with zero_rows_mask == 1 are written in full (so they need no
+ * backward then writes one record per pair instead of adding to v_textures with
+ * float atomics, and a gather kernel sums each texture row's records in LDS (in
+ * float64 where tgs_texture_gather_fast says so) and writes the row once: rows with zero_rows_mask == 1 are written in full (so they need no
  * zero-fill beforehand; the rows with mask 0 that zero_rows_select names are
  * zero-filled by this call), or, with zero_rows_dst NULL, the sums are added
  * to v_textures.  Rows larger than 64 KB fall back to direct accumulation.  With v_textures NULL (and

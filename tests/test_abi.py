@@ -111,9 +111,15 @@ def test_host_validation_errors_without_gpu():
     assert rc == 1 and b"null operand" in L.tgs_last_error()
     rc = L.tgs_sigmoid_bwd_colsum(10, 6, null, null, null, 1, null, null)
     assert rc == 1 and b"multiple of 4" in L.tgs_last_error()
-    # the measurement hook: forward (0) or backward (1), both events or neither
-    rc = L.tgs_set_raster_events(2, null, null)
+    # the measurement hook: forward (0), backward (1) or texture gather (2), both events or neither
+    rc = L.tgs_set_raster_events(3, null, null)
     assert rc == 1 and b"which" in L.tgs_last_error()
+    assert L.tgs_set_raster_events(2, null, null) == 0
+    # the streaming gather: float32 compute (bf16 textures too), rows whose float64 sums fit
+    # twice in LDS (50x50 yes, 64x64 no); never for float64
+    assert L.tgs_texture_gather_fast(0, 50, 50) == 1 and L.tgs_texture_gather_fast(0 | 0x100, 32, 32) == 1
+    assert L.tgs_texture_gather_fast(0, 64, 64) == 0 and L.tgs_texture_gather_fast(1, 16, 16) == 0
+    assert L.tgs_texture_gather_fast(0, 0, 16) == 0
     rc = L.tgs_set_raster_events(0, 1234, null)
     assert rc == 1 and b"both events" in L.tgs_last_error()
     assert L.tgs_set_raster_events(1, null, null) == 0

@@ -30,23 +30,32 @@ def _stages(dtype):
                                            (torch.float64, "exact", "default"), (torch.float32, "none", "records"),
                                            (torch.float64, "exact", "records"), (torch.float32, "none", "atomics"),
                                            (torch.float32, "exact", "hybrid"), (torch.float64, "none", "hybrid"),
-                                           (torch.float32, "none", "dense_fill")])
+                                           (torch.float32, "none", "dense_fill"),
+                                           (torch.float32, "none", "records_dense_fill")])
 def test_config1_full_frame(dtype, uv, path, monkeypatch):
     """Texture-gradient paths: "records" - every texture pair through per-pair records and
-    the row gather (the unlisted-row fill split between forward and backward);
-    "atomics" - float atomics only; "hybrid" - rows with at least 16 pairs through
-    records (many rows at this size), the others atomics; "dense_fill" - float atomics
-    with the gradient zero-filled by one launch before the forward instead of along the
-    rasterizer walks; "default" - as shipped."""
+    the row gather, with the unlisted-row fill split between forward and backward;
+    "records_dense_fill" - records, the unlisted rows zero-filled by one launch before the
+    forward; "atomics" - float atomics only; "hybrid" - rows with at least 16 pairs
+    through records (many rows at this size), the others atomics; "dense_fill" - float
+    atomics with the gradient zero-filled by one launch before the forward instead of
+    along the rasterizer walks; "default" - as shipped (records with the streaming gather
+    in float32, float atomics in float64)."""
     from textured_gaussians_amd.cuda import _wrapper
     if path == "records":
         monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
         monkeypatch.setattr(_wrapper, "FWD_FILL_RESIDUES", 0x0F)
+    elif path == "records_dense_fill":
+        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
+        monkeypatch.setattr(_wrapper, "FUSED_FILL", False)
     elif path == "atomics":
+        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", False)
         monkeypatch.setattr(_wrapper, "HEAVY_ROW_PAIRS", 0)
     elif path == "hybrid":
+        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", False)
         monkeypatch.setattr(_wrapper, "HEAVY_ROW_PAIRS", 16)
     elif path == "dense_fill":
+        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", False)
         monkeypatch.setattr(_wrapper, "FUSED_FILL", False)
     st = _stages(dtype)
     assert st["flatten_ids"].numel() > 50_000

@@ -1371,6 +1371,257 @@ __global__ __launch_bounds__(512, 4) void tex_gather_kernel(int32_t tex_w, int64
     }
 }
 
+// ------------------------------------- texture-gradient gather, adders and writers
+// One workgroup is AD adder waves and WR writer waves sharing two LDS row buffers.
+// Step s: the adders sum the records of list entry b + s G into buffer s & 1 with LDS
+// atomics, while the writers stream the row of entry b + (s - 1) G out of the other
+// buffer, clearing it behind them; one barrier per step.
+//  * The buffers hold float64 sums: on this hardware an LDS float32 atomic add costs
+//    about 3 clocks per active lane (~190 clocks per wave instruction, whatever the
+//    addresses), a float64 one ~28 clocks per wave instruction.  Rows are rounded to
+//    float32 once, as they are written.
+//  * Each buffer has a bitmask of the texels its row's records touch (set by the
+//    adders with LDS integer atomics).  A writer reads and clears only marked texels
+//    and stores zeros elsewhere: a typical row touches a sixth of its texels, and
+//    reading and clearing whole 80 KB float64 rows made LDS traffic the writers' bound.
+//  * Streaming the rows out is the bound (every listed row is written once, 40 KB at
+//    50x50 float32).  A record's 16 contributions take ~50 vector instructions, so the
+//    adders split each row: records come in chunks of 64, one per lane, and adder a
+//    takes chunks a, a + AD, a + 2 AD, ...  Each adder loads its first kG2Pre chunks of
+//    row s + 2 in step s, into the third of three register sets that rotate over the
+//    steps, so those loads have two steps to arrive; further chunks (rows of more than
+//    64 AD kG2Pre records) load during the step, four at a time.  List entries are
+//    read 16 steps ahead in register blocks (4 dwords per entry, one per lane).
+//  * The writers issue no global loads, so nothing they do waits for their stores
+//    (vmcnt counts loads and stores together); adder 0 hands them the row through LDS.
+//    All loads are vector loads: the step barrier waits on lgkmcnt, which scalar loads
+//    would share with the LDS traffic.
+//  * LDS layout: texel t's four sums at doubles 4t .. 4t + 3 with the channel index
+//    XOR-ed by g2_swz(t), so the atomics of one channel over random texels reach every
+//    bank (unswizzled, channel c of every texel falls in the same few banks).  A writer
+//    lane reads a texel as two 16-byte pieces and undoes the permutation in registers.
+constexpr int kG2Adders = 4;
+constexpr int kG2Writers = 6;
+constexpr int kG2Pre = 2;     // record chunks per adder and row loaded two steps ahead
+constexpr int kG2Block = 16;  // list entries per register block
+
+__device__ __forceinline__ uint32_t g2_swz(uint32_t t) { return ((t >> 2) ^ (t >> 4)) & 3u; }
+
+// LDS bytes of the gather for rows of `texels` texels: two float64 row buffers, their
+// texel bitmasks, the row of each buffer, and one zero texel (what writer lanes whose
+// texel is untouched read: one broadcast read instead of a branch)
+__host__ __device__ constexpr size_t g2_mask_words(int64_t texels) { return (size_t)(texels + 31) / 32; }
+__host__ __device__ constexpr size_t g2_pad_offset(int64_t texels) {
+    return (2 * (size_t)texels * 32 + 2 * 4 * g2_mask_words(texels) + 16 + 15) / 16 * 16;
+}
+__host__ __device__ constexpr size_t g2_lds_bytes(int64_t texels) { return g2_pad_offset(texels) + 32; }
+
+struct G2Set {
+    TexRecord<float> r[kG2Pre];
+};
+
+struct G2Entry {
+    int32_t row, n;
+    int64_t start;
+};
+
+// Block m of list entries: entry j is list[b + (16 m + j) G]; lane l holds dword l % 4
+// (row, n, start lo, start hi) of entry l / 4.  Past the end: row -1, no records.
+__device__ __forceinline__ int32_t g2_block_load(const GatherEntry *L, int64_t total, int64_t b, int64_t G,
+                                                 int64_t m, int lane) {
+    const int64_t k = b + (m * kG2Block + lane / 4) * G;
+    const int32_t none = (lane & 3) == 0 ? -1 : 0;
+    return k < total ? ((const int32_t *)(L + k))[lane & 3] : none;
+}
+
+__device__ __forceinline__ G2Entry g2_entry(int32_t blk, int j) {
+    G2Entry e;
+    e.row = __builtin_amdgcn_readlane(blk, 4 * j);
+    e.n = __builtin_amdgcn_readlane(blk, 4 * j + 1);
+    e.start = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane(blk, 4 * j + 3) << 32) |
+                        (uint64_t)(uint32_t)__builtin_amdgcn_readlane(blk, 4 * j + 2));
+    return e;
+}
+
+// this lane's record of chunk c of entry e, or the row's first (record 0 for an empty
+// row) where it has none: every lane loads, so the loads in flight are counted exactly
+__device__ __forceinline__ TexRecord<float> g2_rec(const TexRecord<float> *recs, const G2Entry &e, int32_t c,
+                                                   int lane) {
+    const int32_t q = c * kWave + lane;
+    return recs[q < e.n ? e.start + q : (e.n > 0 ? e.start : 0)];
+}
+
+// one record's 16 contributions (4 texels x RGBA) into the float64 row, and its texels'
+// bits in the row's mask
+__device__ __forceinline__ void g2_add1(double *acc, uint32_t *mask, const TexRecord<float> &rec, bool ok,
+                                        int32_t tex_w) {
+    if (!ok) return;
+    uint32_t t[4];
+    t[0] = rec.o00 / 16u;
+    t[1] = t[0] + (rec.flags & 1u);
+    t[2] = t[0] + ((rec.flags & 2u) ? (uint32_t)tex_w : 0u);
+    t[3] = t[2] + (rec.flags & 1u);
+    Bilerp<float> bl;
+    bl.wu = rec.wu;
+    bl.wv = rec.wv;
+    const BWeights<float> bw = bweights(bl);
+    const float w[4] = {bw.w00, bw.w10, bw.w01, bw.w11};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t sw = g2_swz(t[k]);
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float v = w[k] * rec.q[c];  // the float32 product of the float atomic path
+            if (v != 0.0f) {
+                atomicAdd(acc + 4 * t[k] + (c ^ sw), (double)v);
+                any = true;
+            }
+        }
+        if (any) atomicOr(mask + (t[k] >> 5), 1u << (t[k] & 31u));
+    }
+}
+
+template <int AD, int WR>
+__global__ __launch_bounds__((AD + WR) * kWave) void tex_gather2_kernel(int32_t tex_w, int32_t texels,
+                                                                        const GatherEntry *__restrict__ list,
+                                                                        int64_t cap, const uint32_t *__restrict__ count,
+                                                                        const TexRecord<float> *__restrict__ recs,
+                                                                        float *__restrict__ g_tex) {
+    typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+    typedef double f64x2_t __attribute__((ext_vector_type(2)));
+    constexpr int kThreads = (AD + WR) * kWave;
+    extern __shared__ __align__(16) char smem[];
+    double *const acc = (double *)smem;
+    f64x2_t *const acc2 = (f64x2_t *)smem;
+    const int32_t mw = (int32_t)g2_mask_words(texels);
+    uint32_t *const masks = (uint32_t *)(smem + 2 * (size_t)texels * 32);
+    int32_t *const slot = (int32_t *)(masks + 2 * mw);
+    const int32_t pad = (int32_t)(g2_pad_offset(texels) / 16);  // f64x2 index of the zero texel
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;  // adders 0 .. AD - 1, then the writers
+    const bool writer = wave >= AD;
+    const int64_t total = min((int64_t)count[0], cap);
+    const int64_t G = gridDim.x, b = blockIdx.x;
+    // entries b + s G, s < n_add, are added in steps s; the writers drain entry s - 1 in step s
+    const int64_t n_add = b < total ? (total - b + G - 1) / G : 0;
+    if (n_add == 0) return;  // uniform over the workgroup
+    const int64_t n_steps = n_add + 1;
+    for (int32_t i = threadIdx.x; i < 4 * texels; i += kThreads) acc2[i] = f64x2_t{0.0, 0.0};
+    for (int32_t i = threadIdx.x; i < 2 * mw; i += kThreads) masks[i] = 0u;
+    if (threadIdx.x < 2) acc2[pad + threadIdx.x] = f64x2_t{0.0, 0.0};
+
+    // adders: entry blocks of steps [16 m, 16 m + 16) and the next; three record sets
+    // (rows s, s + 1, s + 2)
+    G2Set x0, x1, x2;
+    int32_t blk = 0, blk_next = 0;
+    if (!writer) {
+        blk = g2_block_load(list, total, b, G, 0, lane);
+        blk_next = g2_block_load(list, total, b, G, 1, lane);
+        const G2Entry e0 = g2_entry(blk, 0), e1 = g2_entry(blk, 1);
+#pragma unroll
+        for (int i = 0; i < kG2Pre; ++i) {
+            x0.r[i] = g2_rec(recs, e0, wave + AD * i, lane);
+            x1.r[i] = g2_rec(recs, e1, wave + AD * i, lane);
+        }
+    }
+    lds_barrier();
+    // step s: add row s from `cur`, load row s + 2 into `nxt2` (the set row s - 1 used)
+    auto step = [&](int64_t s, G2Set &cur, G2Set &nxt2) -> bool {
+        const int buf = (int)(s & 1);
+        if (!writer) {
+            const int j = (int)(s % kG2Block);
+            if (j == 0 && s > 0) {
+                blk = blk_next;
+                blk_next = g2_block_load(list, total, b, G, s / kG2Block + 1, lane);
+            }
+            const G2Entry e = g2_entry(blk, j);
+            const int j2 = j + 2;
+            const G2Entry e2 = j2 < kG2Block ? g2_entry(blk, j2) : g2_entry(blk_next, j2 - kG2Block);
+#pragma unroll
+            for (int i = 0; i < kG2Pre; ++i) nxt2.r[i] = g2_rec(recs, e2, wave + AD * i, lane);
+            double *ab = acc + (size_t)buf * texels * 4;
+            uint32_t *mb = masks + buf * mw;
+            if (e.n > 0) {
+#pragma unroll
+                for (int i = 0; i < kG2Pre; ++i)
+                    g2_add1(ab, mb, cur.r[i], (wave + AD * i) * kWave + lane < e.n, tex_w);
+                // chunks past the prefetched ones, four per round of loads
+                for (int32_t c = wave + AD * kG2Pre; c * kWave < e.n; c += 4 * AD) {
+                    TexRecord<float> r4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) r4[u] = g2_rec(recs, e, c + u * AD, lane);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        g2_add1(ab, mb, r4[u], (c + u * AD) * kWave + lane < e.n, tex_w);
+                }
+            }
+            if (wave == 0 && lane == 0) slot[buf] = e.row;
+        } else if (s >= 1) {
+            const int32_t row = slot[buf ^ 1];
+            if (row >= 0) {
+                const int32_t src = (buf ^ 1) * texels * 2;  // f64x2 index of the buffer
+                uint32_t *mb = masks + (buf ^ 1) * mw;
+                u32x4_t *dst = (u32x4_t *)(g_tex + (int64_t)row * texels * 4);
+                constexpr int kStride = WR * kWave, kB = 8;
+                const int wl = (wave - AD) * kWave + lane;  // lane among the writers
+                for (int32_t base = wl; base < texels; base += kB * kStride) {
+                    // a batch of kB texels per lane: mask words, then sums (a marked texel's
+                    // pair of 16-byte pieces, else the zero texel), then stores, so each phase
+                    // issues all its LDS accesses before waiting on any
+                    uint32_t word[kB];
+#pragma unroll
+                    for (int i = 0; i < kB; ++i) {
+                        const int32_t t = min(base + i * kStride, texels - 1);
+                        word[i] = mb[t >> 5];
+                    }
+                    // the 32 texels of a mask word lie in one 64-texel block, which one writer
+                    // wave reads in one batch: the word is cleared once that wave has read it
+                    // (the LDS accesses of a wave complete in order)
+#pragma unroll
+                    for (int i = 0; i < kB; ++i) {
+                        const int32_t t = base + i * kStride;
+                        if (t < texels && (t & 31) == 0) mb[t >> 5] = 0u;
+                    }
+                    int32_t at[kB];
+                    f64x2_t lo[kB], hi[kB];
+#pragma unroll
+                    for (int i = 0; i < kB; ++i) {
+                        const int32_t t = base + i * kStride;
+                        const bool hit = t < texels && ((word[i] >> (t & 31)) & 1u);
+                        at[i] = hit ? src + 2 * t : pad;
+                        lo[i] = acc2[at[i]];
+                        hi[i] = acc2[at[i] + 1];
+                    }
+#pragma unroll
+                    for (int i = 0; i < kB; ++i) {
+                        if (at[i] != pad) {
+                            acc2[at[i]] = f64x2_t{0.0, 0.0};
+                            acc2[at[i] + 1] = f64x2_t{0.0, 0.0};
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < kB; ++i) {
+                        const int32_t t = base + i * kStride;
+                        u32x4_t v = u32x4_t{__float_as_uint((float)lo[i].x), __float_as_uint((float)lo[i].y),
+                                            __float_as_uint((float)hi[i].x), __float_as_uint((float)hi[i].y)};
+                        const uint32_t sw = g2_swz((uint32_t)t);
+                        // position p holds channel p ^ sw
+                        if (sw & 1u) v = u32x4_t{v.y, v.x, v.w, v.z};
+                        if (sw & 2u) v = u32x4_t{v.z, v.w, v.x, v.y};
+                        if (t < texels) __builtin_nontemporal_store(v, dst + t);
+                    }
+                }
+            }
+        }
+        lds_barrier();
+        return s + 1 >= n_steps;
+    };
+    for (int64_t s = 0;; s += 3) {
+        if (step(s, x0, x2) || step(s + 1, x1, x0) || step(s + 2, x2, x1)) break;
+    }
+}
+
 // ------------------------------------------------------- row-wise texture Adam
 // Selective Adam (adam.cu:16-47 semantics: no bias correction, rows whose
 // visibility flag is clear are left alone) applied to bfloat16 textures straight
@@ -1545,8 +1796,9 @@ static uint8_t *g_texel_marks;
 static int64_t g_texel_marks_n;
 
 // Measurement hook (tgs_set_raster_events): events recorded on the launch stream right
-// before and after the rasterizer kernel itself, [0] forward, [1] backward.
-static hipEvent_t g_raster_ev[2][2];
+// before and after the rasterizer kernel itself, [0] forward, [1] backward, and around
+// the texture-gradient gather of the backward's records, [2].
+static hipEvent_t g_raster_ev[3][2];
 
 static void record_raster_event(int which, int end, hipStream_t st) {
     if (g_raster_ev[which][end]) (void)hipEventRecord(g_raster_ev[which][end], st);
@@ -1725,6 +1977,7 @@ static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterAr
     // reader, has run (16 bytes per row; with the mask of tgs_mark_rows the rows to write
     // are at most the entries, 80 bytes each or more)
     if (a.E == 0) return TGS_OK;  // no entries: no listed rows, no records
+    record_raster_event(2, 0, st);
     uint32_t *count = (uint32_t *)a.rec;
     GatherEntry *list = (GatherEntry *)((char *)a.rec + 16);
     const int64_t cap = ((int64_t)rec_bytes<S>(1, a.E) - 16) / (int64_t)sizeof(GatherEntry);
@@ -1734,12 +1987,30 @@ static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterAr
                                                                        cap, count);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const size_t lds2 = g2_lds_bytes(tex_h * tex_w);
+    if (std::is_same<S, float>::value && overwrite && lds2 <= kGatherMaxLds) {
+        // adder/writer workgroups, one or more per CU
+        if (lds2 > 64 * 1024 && hipFuncSetAttribute((const void *)tex_gather2_kernel<kG2Adders, kG2Writers>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2) != hipSuccess) {
+            set_error("rasterize_bwd: cannot reserve %zu bytes of LDS for the texture gather", lds2);
+            return TGS_ERR_LAUNCH;
+        }
+        const int per_cu = (int)std::min<size_t>(8, kGatherMaxLds / lds2);
+        const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
+        tex_gather2_kernel<kG2Adders, kG2Writers><<<(unsigned)blocks, (kG2Adders + kG2Writers) * kWave, lds2, st>>>(
+            (int32_t)tex_w, (int32_t)(tex_h * tex_w), list, cap, count, (const TexRecord<float> *)a.recs,
+            (float *)a.g_tex);
+        TGS_CHECK_LAUNCH("rasterize_bwd texture gather");
+        record_raster_event(2, 1, st);
+        return TGS_OK;
+    }
     // as many 512-thread workgroups per CU as their buffers allow (at most 2: 16 waves)
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, kGatherMaxLds / std::max<size_t>(lds, 1)));
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(N, (int64_t)cus * per_cu));
     tex_gather_kernel<S><<<(unsigned)blocks, 512, lds, st>>>((int32_t)tex_w, row_bytes / 16, list, cap, count,
                                                              (const TexRecord<S> *)a.recs, overwrite ? 0 : 1, a.g_tex);
     TGS_CHECK_LAUNCH("rasterize_bwd texture gather");
+    record_raster_event(2, 1, st);
     return TGS_OK;
 }
 
@@ -1755,7 +2026,8 @@ using namespace tgs;
 extern "C" size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N) { return workspace_bytes(dtype, C, N); }
 
 extern "C" int tgs_set_raster_events(int which, void *begin, void *end) {
-    TGS_CHECK_ARG(which == 0 || which == 1, "set_raster_events: which must be 0 (forward) or 1 (backward)");
+    TGS_CHECK_ARG(which >= 0 && which <= 2,
+                  "set_raster_events: which must be 0 (forward), 1 (backward) or 2 (texture gather)");
     TGS_CHECK_ARG((begin == nullptr) == (end == nullptr), "set_raster_events: give both events or neither");
     g_raster_ev[which][0] = (hipEvent_t)begin;
     g_raster_ev[which][1] = (hipEvent_t)end;
@@ -1772,6 +2044,13 @@ extern "C" int tgs_set_texel_marks(uint8_t *marks, int64_t n_texels) {
 
 extern "C" size_t tgs_tex_record_bytes(int dtype) {
     return dtype == TGS_F64 ? sizeof(TexRecord<double>) : sizeof(TexRecord<float>);
+}
+
+extern "C" int tgs_texture_gather_fast(int dtype, int64_t tex_h, int64_t tex_w) {
+    return (dtype & ~TGS_TEX_BF16) == TGS_F32 && tex_h > 0 && tex_w > 0 && tex_h * tex_w <= (int64_t)1 << 20 &&
+                   g2_lds_bytes(tex_h * tex_w) <= kGatherMaxLds
+               ? 1
+               : 0;
 }
 
 extern "C" int tgs_texture_adam_rows(int64_t N, int32_t tex_h, int32_t tex_w, const int32_t *tex_counts,

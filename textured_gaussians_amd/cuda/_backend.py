@@ -65,6 +65,7 @@ _PROTOS = {
     "tgs_set_texel_marks": (c_int, [vp, c_i64]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
     "tgs_tex_record_bytes": (c_size, [c_int]),
+    "tgs_texture_gather_fast": (c_int, [c_int, c_i64, c_i64]),
     "tgs_mipmap_texels": (c_i64, [c_i32]),
     "tgs_sigmoid_bwd_colsum": (c_int, [c_i64, c_i64, vp, vp, vp, c_int, vp, vp]),
     "tgs_ssim_work_bytes": (c_size, [c_int, c_i32, c_i32, c_i32, c_i32]),

@@ -36,16 +36,19 @@ def set_kernel_timer(store: Optional[dict]) -> None:
     _TIMER = store
 
 
-_RASTER_EVENTS = {"raster_fwd": 0, "raster_bwd": 1}
+# library event slots (tgs_set_raster_events) recorded inside an op: the rasterizer
+# kernel itself, and for the backward also the texture-gradient gather
+_RASTER_EVENTS = {"raster_fwd": (("raster_fwd", 0),), "raster_bwd": (("raster_bwd", 1), ("tex_gather", 2))}
 
 
 class _Timed:
     """HIP events on an op's stream: around the rasterizer kernel alone for
-    "raster_fwd" / "raster_bwd" (recorded by the library, tgs_set_raster_events),
-    around the whole op otherwise.  Holds the device only: a reference to the op's
-    tensor would outlive the op (and a gradient buffer with an extra reference is
-    copied by autograd instead of handed over)."""
-    __slots__ = ("name", "device", "e0", "e1")
+    "raster_fwd" / "raster_bwd", and around the backward's texture gather as
+    "tex_gather" (recorded by the library, tgs_set_raster_events), around the whole
+    op otherwise.  Holds the device only: a reference to the op's tensor would outlive
+    the op (and a gradient buffer with an extra reference is copied by autograd
+    instead of handed over)."""
+    __slots__ = ("name", "device", "ev")
 
     def __init__(self, name, device):
         self.name, self.device = name, device
@@ -53,24 +56,27 @@ class _Timed:
     def __enter__(self):
         if _TIMER is not None:
             st = torch.cuda.current_stream(self.device)
-            self.e0 = torch.cuda.Event(enable_timing=True)
-            self.e1 = torch.cuda.Event(enable_timing=True)
-            self.e0.record(st)
-            which = _RASTER_EVENTS.get(self.name)
-            if which is not None:
-                # recorded once so that they exist; the library records them again
-                self.e1.record(st)
-                check(lib().tgs_set_raster_events(which, self.e0.cuda_event, self.e1.cuda_event),
-                      "set_raster_events")
+            slots = _RASTER_EVENTS.get(self.name, ((self.name, None),))
+            self.ev = []
+            for key, which in slots:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                if which is not None:
+                    # recorded once so that they exist; the library records them again
+                    # (a gather that does not run leaves them at ~0 ms)
+                    e1.record(st)
+                    check(lib().tgs_set_raster_events(which, e0.cuda_event, e1.cuda_event), "set_raster_events")
+                self.ev.append((key, which, e0, e1))
 
     def __exit__(self, *exc):
         if _TIMER is not None:
-            which = _RASTER_EVENTS.get(self.name)
-            if which is None:
-                self.e1.record(torch.cuda.current_stream(self.device))
-            else:
-                check(lib().tgs_set_raster_events(which, None, None), "set_raster_events")
-            _TIMER.setdefault(self.name, []).append((self.e0, self.e1))
+            for key, which, e0, e1 in self.ev:
+                if which is None:
+                    e1.record(torch.cuda.current_stream(self.device))
+                else:
+                    check(lib().tgs_set_raster_events(which, None, None), "set_raster_events")
+                _TIMER.setdefault(key, []).append((e0, e1))
 
 
 def _timed(name, t: Tensor):
@@ -953,7 +959,7 @@ def zero_fill_rows_(t: Tensor, mask: Tensor, select: int) -> Tensor:
     row_bytes = t[0].numel() * t.element_size() if t.shape[0] > 0 else 0
     if t.shape[0] == 0:
         return t
-    with torch.cuda.device(t.device):
+    with torch.cuda.device(t.device), _timed("grad_fill", t):
         check(lib().tgs_zero_fill_rows(_p(t), row_bytes, t.shape[0], _p(mask), int(select), _stream(t)),
               "zero_fill_rows")
     return t

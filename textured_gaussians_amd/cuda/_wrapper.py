@@ -21,7 +21,9 @@ from typing import Callable, Optional, Tuple
 import torch
 from torch import Tensor
 
+from ._backend import lib
 from ._ops import _C
+
 
 def _make_lazy_cuda_func(name: str) -> Callable:
     def call(*args, **kwargs):
@@ -356,30 +358,44 @@ def rasterize_to_pixels_textured_gaussians(
     return (render_colors,) + tuple(out[1:])
 
 
-# Texture gradients through per-pair records and a per-row LDS gather (True), or
-# float atomics into a v_textures whose listed rows the forward zero-fills (False).
-# Measured on the 1M-Gaussian scene (one MI355X): atomics 19.1-21.0 ms/step; records
-# 20.8-22.3 ms/step.  The gather streams the listed rows at 4.6 TB/s but its LDS float
-# atomics (71M records x 16 values) add about 3 ms, and the unlisted-row fill no longer
-# hides behind the atomic-bound backward.
-TEXTURE_GRAD_RECORDS = False
+# Texture gradients through per-pair records and a per-row gather (True), float atomics
+# into a v_textures whose listed rows the forward zero-fills (False), or whichever is
+# faster for the textures at hand (None): records where the library's streaming gather
+# takes them (float32 compute, rows up to about 50x50 texels), atomics elsewhere.
+# Measured on the 1M-Gaussian scene at 50x50 (one MI355X, same box): atomics 20.7-20.8
+# ms/iter, records 19.0 ms/iter.  The backward's float atomics into the 40 GB gradient
+# are performed at the memory side (1.8e8 requests, ~5 ms of its 10 ms); the records
+# path writes 32-byte records instead and the gather sums each listed row in LDS and
+# streams it out once.
+TEXTURE_GRAD_RECORDS = None
 # Between the two (used when TEXTURE_GRAD_RECORDS is False): the rows with at least
 # HEAVY_ROW_PAIRS texture pairs (few rows, most pairs: large splats) take records and
 # the gather, the others float atomics.  0 disables it.
 HEAVY_ROW_PAIRS = 0
 # With records, the rows absent from the tile lists are zero-filled along the
 # rasterizer walks: the forward takes the rows whose index mod 8 is in this set
-# (bit r), the backward the others; the gather writes the listed rows.
-FWD_FILL_RESIDUES = 0xFF
-# Float-atomic path: the texture gradient's zero-fill rides along the rasterizer walks
-# (True), or is one dense fill launch before the forward (False; same result, used to
-# time the rasterizer kernels without the fill).
+# (bit r), the backward the others; the gather writes the listed rows.  Both walks
+# are bound by memory traffic, so the split changes little; the forward alone is
+# fastest without the fill (19.0 vs 19.4 ms/iter with all of it in the forward).
+FWD_FILL_RESIDUES = 0x00
+# The texture gradient's zero-fill rides along the rasterizer walks (True), or is one
+# launch before the forward (False; same result, used to time the rasterizer kernels
+# without the fill): the whole gradient for float atomics, the rows absent from the
+# tile lists for records (the gather writes the others).
 FUSED_FILL = True
 
 
 def _records_fit(textures: Tensor, dtype: torch.dtype) -> bool:
     """The gather keeps one texture row in LDS: at most 80 KB."""
     return textures.shape[1] * textures.shape[2] * 4 * torch.tensor([], dtype=dtype).element_size() <= 80 * 1024
+
+
+def _use_records(textures: Tensor, dtype: torch.dtype) -> bool:
+    """Texture gradients through records (see TEXTURE_GRAD_RECORDS)."""
+    if TEXTURE_GRAD_RECORDS is None:
+        dt = 1 if dtype == torch.float64 else 0
+        return bool(lib().tgs_texture_gather_fast(dt, textures.shape[1], textures.shape[2]))
+    return bool(TEXTURE_GRAD_RECORDS) and _records_fit(textures, dtype)
 
 
 class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
@@ -413,14 +429,19 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             # texture rows present in the tile lists (packed: the entries' Gaussians)
             listed = flatten_ids if gaussian_ids is None else gaussian_ids[flatten_ids.long()].int()
             rows = _C.mark_rows(listed, textures.shape[0])
-            if TEXTURE_GRAD_RECORDS and _records_fit(textures, means2d.dtype):
+            if _use_records(textures, means2d.dtype):
                 ctx.tex_counts = _C.tex_counts_buffer(textures)
-                fwd_res = FWD_FILL_RESIDUES & 0xFF
-                if fwd_res:
-                    zero_rows = (v_tex, rows, fwd_res << 8)  # mask value 0, these residues
-                bwd_res = 0xFF & ~fwd_res
-                # the backward's share; mask value 2 (no row) when the forward took every residue
-                ctx.v_tex = (v_tex, rows, (bwd_res << 8) if bwd_res else 2)
+                if not FUSED_FILL:
+                    # the unlisted rows by one launch now; the walks fill nothing (mask value 2: no row)
+                    _C.zero_fill_rows_(v_tex, rows, 0)
+                    ctx.v_tex = (v_tex, rows, 2)
+                else:
+                    fwd_res = FWD_FILL_RESIDUES & 0xFF
+                    if fwd_res:
+                        zero_rows = (v_tex, rows, fwd_res << 8)  # mask value 0, these residues
+                    bwd_res = 0xFF & ~fwd_res
+                    # the backward's share; mask value 2 (no row) when the forward took every residue
+                    ctx.v_tex = (v_tex, rows, (bwd_res << 8) if bwd_res else 2)
             elif not FUSED_FILL:  # one dense fill; the backward adds in place
                 _C.zero_fill_(v_tex)
                 ctx.v_tex = (v_tex, None, None)
