@@ -240,12 +240,21 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * 1 - render_alphas ([TB]:456-457), which in float32 keeps T only to an
  * absolute 6e-8 (6e-4 relative at the 1e-4 stop); given to
  * tgs_rasterize_bwd, the backward starts from T itself.
- * live_entries (may be NULL; no reference counterpart): uint8[waves][n_isects]
+ * live_entries (may be NULL; no reference counterpart): uint64[waves][n_isects]
  * with waves = ceil(tile_size^2 / 64), the wavefronts of a tile; the forward
- * sets byte [w][i] to 1 where some pixel of wave w of entry i's tile
- * composited entry i, 0 where none did, for every i up to the last entry
- * wave w reached.  Given to tgs_rasterize_bwd, the backward visits only those
- * entries (the others contribute nothing to any gradient).
+ * sets [w][i] to the mask of the lanes of wave w of entry i's tile that
+ * composited entry i (bit l: pixel 64 w + l of the tile; 0 where none did), for
+ * every i up to the last entry wave w reached.  Given to tgs_rasterize_bwd, the
+ * backward visits only entries with a nonzero mask (the others contribute
+ * nothing to any gradient).
+ * pair_samples (may be NULL; no reference counterpart; needs live_entries,
+ * float32 compute, float32 or bfloat16 textures without mip2, K <= 3, 16-byte
+ * alignment): float32[waves][n_isects][64][4].  For each composited pair inside
+ * the texture range the forward stores its bilinear sample (K colour channels,
+ * zeros up to 3, then alpha) at [w][i][r], r = the pair's rank among the lanes
+ * of mask [w][i] that are inside the range.  Given to tgs_rasterize_bwd with the
+ * same live_entries (uv_grad_mode TGS_UV_NONE), the backward reads the samples
+ * instead of the texels and takes the masks' keep decisions.
  * tex_counts int32[N] (may be NULL; no reference counterpart; accumulated,
  * caller zero-fills): per texture row, the number of (pixel, entry) pairs
  * whose backward adds texture gradient (composited and inside the texture
@@ -259,7 +268,8 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * zero_rows_mask[n] == 1 this call zero-fills along the way, using memory
  * bandwidth the rasterizer leaves idle.  zero_rows_select picks the rows: its
  * low byte is the mask value to fill (0 or 1), bits 8-15 a set of residues of
- * row mod 8 (0 = every row).  With the mask of tgs_mark_rows this prepares the
+ * row mod 8 (0 = every row), bits 16-23 the fill stores (1 KB each) per visited
+ * list entry and wavefront (0 = 1 in the forward, 2 in the backward).  With the mask of tgs_mark_rows this prepares the
  * texture gradient: the forward fills the rows with mask 1 (those the backward
  * accumulates into) and the backward those with mask 0, or, with the records
  * of tgs_rasterize_bwd, the rows with mask 0 are split between forward and
@@ -305,9 +315,9 @@ int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       void *render_alphas, void *render_normals, void *render_distort,
                       void *render_median, int32_t *last_ids, int32_t *median_ids,
                       void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                      void *render_trans, uint8_t *live_entries, int32_t *tex_counts,
-                      void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
-                      int zero_rows_select, void *workspace, size_t workspace_size,
+                      void *render_trans, uint64_t *live_entries, void *pair_samples,
+                      int32_t *tex_counts, void *zero_rows_dst, const uint8_t *zero_rows_mask,
+                      size_t zero_row_bytes, int zero_rows_select, void *workspace, size_t workspace_size,
                       tgs_stream_t stream);
 
 /* Allocation-free variant of tgs_rasterize_fwd (no counterpart in the
@@ -338,7 +348,8 @@ int tgs_rasterize_fwd_noalloc(int dtype, int64_t C, int64_t N, int32_t K, const 
                               void *render_alphas, void *render_normals, void *render_distort,
                               void *render_median, int32_t *last_ids, int32_t *median_ids,
                               void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                              void *render_trans, uint8_t *live_entries, int32_t *tex_counts,
+                              void *render_trans, uint64_t *live_entries, void *pair_samples,
+                              int32_t *tex_counts,
                               void *zero_rows_dst, const uint8_t *zero_rows_mask, size_t zero_row_bytes,
                               int zero_rows_select, tgs_stream_t stream);
 
@@ -351,7 +362,7 @@ int tgs_rasterize_fwd_noalloc(int dtype, int64_t C, int64_t N, int32_t K, const 
  * accumulator with render_colors[..., K-1] (that includes the texture colour
  * and T*background of the channel); non-NULL gives the true gradient.
  * live_entries: NULL (visit every entry) or the array the forward of the same
- * inputs wrote.  render_trans: NULL (start from 1 - render_alphas, as the
+ * inputs wrote; pair_samples: NULL or the forward's, with its live_entries.  render_trans: NULL (start from 1 - render_alphas, as the
  * reference) or the forward's final transmittance.
  * Texture-gradient records (all NULL, or all given; no reference counterpart):
  * tex_counts from the forward of the same inputs, tex_rec_incl int64[N] its
@@ -385,7 +396,8 @@ int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, const void *me
                       const void *render_colors, const void *render_alphas,
                       const int32_t *last_ids, const int32_t *median_ids,
                       const void *render_accum_depth, const void *render_trans,
-                      const uint8_t *live_entries, const int32_t *tex_counts, const int64_t *tex_rec_incl,
+                      const uint64_t *live_entries, const void *pair_samples, const int32_t *tex_counts,
+                      const int64_t *tex_rec_incl,
                       int32_t *tex_rec_cursor, void *tex_records, const void *v_render_colors,
                       const void *v_render_alphas,
                       const void *v_render_normals, const void *v_render_distort,

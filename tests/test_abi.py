@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for n in names:
         assert hasattr(L, n), n
-    assert L.tgs_abi_version() == 1
+    assert L.tgs_abi_version() == 2
 
 
 def test_ctypes_prototypes_match_header_arity():
@@ -82,17 +82,17 @@ def test_host_validation_errors_without_gpu():
     # 5 channels are rejected before any device work (the reference dispatches 1..5 but
     # its python layer pads 5 -> 8, which it rejects)
     rc = L.tgs_rasterize_fwd(0, 1, 10, 5, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 16, 1, 1, null, null,
-                             0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
+                             0, 0.0, 1.0, *([null] * 16), 0, 0, null, 0, null)
     assert rc != 0 and b"channels" in L.tgs_last_error()
     # any tile size whose square is at most 1024 (the reference's tile_size^2 threads)
     rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 33, 1, 1, null, null,
-                             0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
+                             0, 0.0, 1.0, *([null] * 16), 0, 0, null, 0, null)
     assert rc != 0 and b"tile_size" in L.tgs_last_error()
     # the packed-record workspace is required and sized by tgs_rasterize_workspace_bytes
     assert L.tgs_rasterize_workspace_bytes(0, 2, 1000) >= 2 * 1000 * 20 * 4
     assert L.tgs_rasterize_workspace_bytes(1, 2, 1000) >= 2 * 1000 * 20 * 8
     rc = L.tgs_rasterize_fwd(0, 1, 10, 3, *([null] * 6), 0, 0, 3.0, 3.0, null, null, null, 0, 16, 16, 16, 1, 1, null, null,
-                             0, 0.0, 1.0, *([null] * 15), 0, 0, null, 0, null)
+                             0, 0.0, 1.0, *([null] * 16), 0, 0, null, 0, null)
     assert rc != 0 and b"workspace" in L.tgs_last_error()
     # camera and tile ids must fit in 32 bits
     rc = L.tgs_isect_tiles_count(1 << 20, 1, null, null, 16, 4096, 4096, null, null, null, 0, null)

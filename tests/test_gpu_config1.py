@@ -31,12 +31,14 @@ def _stages(dtype):
                                            (torch.float64, "exact", "records"), (torch.float32, "none", "atomics"),
                                            (torch.float32, "exact", "hybrid"), (torch.float64, "none", "hybrid"),
                                            (torch.float32, "none", "dense_fill"),
-                                           (torch.float32, "none", "records_dense_fill")])
+                                           (torch.float32, "none", "records_dense_fill"),
+                                           (torch.float32, "none", "pair_samples")])
 def test_config1_full_frame(dtype, uv, path, monkeypatch):
     """Texture-gradient paths: "records" - every texture pair through per-pair records and
     the row gather, with the unlisted-row fill split between forward and backward;
     "records_dense_fill" - records, the unlisted rows zero-filled by one launch before the
-    forward; "atomics" - float atomics only; "hybrid" - rows with at least 16 pairs
+    forward; "pair_samples" - records, the backward reading the forward's texture samples
+    instead of the texels; "atomics" - float atomics only; "hybrid" - rows with at least 16 pairs
     through records (many rows at this size), the others atomics; "dense_fill" - float
     atomics with the gradient zero-filled by one launch before the forward instead of
     along the rasterizer walks; "default" - as shipped (records with the streaming gather
@@ -45,6 +47,9 @@ def test_config1_full_frame(dtype, uv, path, monkeypatch):
     if path == "records":
         monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
         monkeypatch.setattr(_wrapper, "FWD_FILL_RESIDUES", 0x0F)
+    elif path == "pair_samples":
+        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
+        monkeypatch.setattr(_wrapper, "PAIR_SAMPLES", True)
     elif path == "records_dense_fill":
         monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
         monkeypatch.setattr(_wrapper, "FUSED_FILL", False)

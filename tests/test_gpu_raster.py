@@ -231,11 +231,11 @@ def test_raster_bwd_channels_nobg(K):
 @pytest.mark.parametrize("textured", [True, False])
 @pytest.mark.parametrize("tile_size", [16, 8])
 def test_live_entries_skip_only_unused_entries(textured, tile_size):
-    """The forward records, per wavefront and list entry, whether any of the wave's
-    pixels composited the entry; the backward then visits only those entries.  The
-    bytes are 0/1, every pixel's last composited entry is marked for its wave, and the
-    backward over live entries gives the full walk's gradients (float64, so only the
-    atomics' summation order differs)."""
+    """The forward records, per wavefront and list entry, the mask of the wave's lanes
+    that composited the entry; the backward then visits only entries with a nonzero
+    mask.  Every pixel's last composited entry has the pixel's bit set in its wave's
+    mask, and the backward over live entries gives the full walk's gradients (float64,
+    so only the atomics' summation order differs)."""
     dtype = torch.float64
     sc = scenes.config1(n=2500, width=80, height=64, tex_res=12, seed=3, sh_degree=None)
     st = gpu_stages(sc, dtype, tile_size=tile_size)
@@ -247,7 +247,7 @@ def test_live_entries_skip_only_unused_entries(textured, tile_size):
     tex = sc.textures.to(DEV, dtype) if textured else None
     flat = st["flatten_ids"]
     waves = (tile_size * tile_size + 63) // 64
-    live = torch.zeros(waves, flat.numel(), dtype=torch.uint8, device=DEV)
+    live = torch.zeros(waves, flat.numel(), dtype=torch.int64, device=DEV)
     if textured:
         fwd = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(
             st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], None, None, s.width, s.height,
@@ -257,12 +257,25 @@ def test_live_entries_skip_only_unused_entries(textured, tile_size):
                                                      None, s.width, s.height, tile_size, st["offsets"], flat, 0.0,
                                                      live_entries=live)
     lv = live.cpu()
-    assert int(lv.max()) <= 1 and 0 < int(lv.sum()) < lv.numel()
+    assert bool((lv != 0).any()) and bool((lv == 0).any())
     alpha, last = fwd[1][0, ..., 0].cpu(), fwd[5][0].cpu()
     ii, jj = torch.meshgrid(torch.arange(s.height), torch.arange(s.width), indexing="ij")
-    wave = ((ii % tile_size) * tile_size + jj % tile_size) // 64
+    pix = (ii % tile_size) * tile_size + jj % tile_size  # pixel of its tile
+    wave, bit = pix // 64, pix % 64
     hit = alpha > 0
-    assert hit.any() and bool((lv[wave[hit], last[hit].long()] == 1).all())
+    assert hit.any() and bool((((lv[wave[hit], last[hit].long()] >> bit[hit]) & 1) == 1).all())
+    # a pixel's bit is set only for entries up to its last composited one
+    w_, i_ = torch.nonzero(lv != 0, as_tuple=True)
+    offs = st["offsets"].reshape(-1).cpu().long()
+    tile = torch.searchsorted(offs, i_, right=True) - 1
+    tw = (s.width + tile_size - 1) // tile_size
+    for b in range(64):
+        sel = ((lv[w_, i_] >> b) & 1) == 1
+        p = w_[sel] * 64 + b
+        y = (tile[sel] // tw) * tile_size + p // tile_size
+        x = (tile[sel] % tw) * tile_size + p % tile_size
+        assert bool(((y < s.height) & (x < s.width)).all())
+        assert bool((last[y, x] >= i_[sel]).all())
 
     gen = torch.Generator().manual_seed(5)
     H, W = s.height, s.width
@@ -331,14 +344,14 @@ def test_texture_gradient_records_match_direct_accumulation(dtype, tex_res):
 
 
 def _fwd_pair(st, colors, opac, tex, monkeypatch, tile_size):
-    """The forward with and without the candidate pre-pass; the live-entry bytes start
-    as 0xFF so entries a wave never reached compare equal too."""
+    """The forward with and without the candidate pre-pass; the composited-lane masks
+    start as all ones so entries a wave never reached compare equal too."""
     s = st["scene"]
     waves = (tile_size * tile_size + 63) // 64
     res = {}
     for cull in (True, False):
         monkeypatch.setattr(_ops, "FWD_CULL", cull)
-        live = torch.full((waves, st["flatten_ids"].numel()), 255, dtype=torch.uint8, device=DEV)
+        live = torch.full((waves, st["flatten_ids"].numel()), -1, dtype=torch.int64, device=DEV)
         if tex is not None:
             cnt = _ops.tex_counts_buffer(tex)
             fo = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(
@@ -370,7 +383,7 @@ def assert_prepass_exact(res):
 @pytest.mark.parametrize("tile_size", [16, 8])
 def test_forward_candidate_prepass_is_exact(dtype, textured, tile_size, monkeypatch):
     """The forward's candidate pre-pass skips only list entries no pixel of a wave can
-    keep: every output, the live-entry bytes and the texture pair counts are bit-identical
+    keep: every output, the composited-lane masks and the texture pair counts are bit-identical
     to the walk over every entry (the contribution sums to float atomic ordering)."""
     sc = scenes.config1(n=2500, width=80, height=64, tex_res=12, seed=3, sh_degree=None)
     st = gpu_stages(sc, dtype, tile_size=tile_size)
@@ -460,3 +473,61 @@ def test_texel_marks_cover_the_gradient_and_change_nothing():
     listed = _ops.mark_rows(st["flatten_ids"], N).bool()
     assert not bool(marks[~listed].any())
     assert int(marks.count_nonzero()) >= int(touched.count_nonzero())
+
+
+@pytest.mark.parametrize("tex_dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("tile_size,width,height", [(16, 80, 64), (8, 75, 61)])
+def test_pair_samples_replace_the_texel_reads(tex_dtype, tile_size, width, height):
+    """With the forward's pair samples (its texture sample of every composited in-range
+    pair) the reference-rule backward reads no texels and takes the forward's
+    composited-lane masks for its keep decisions: every gradient equals the backward
+    that re-reads the texels (float32; the samples are the same values, so only the
+    order of the float atomics differs), through the records path and through float
+    atomics, and the forward's outputs do not change."""
+    dtype = torch.float32
+    sc = scenes.config1(n=2500, width=width, height=height, tex_res=12, seed=9, sh_degree=None)
+    st = gpu_stages(sc, dtype, tile_size=tile_size)
+    s = st["scene"]
+    C, N = st["radii"].shape
+    g = torch.Generator().manual_seed(10)
+    colors = torch.rand(C, N, 3, generator=g, dtype=torch.float64).to(DEV, dtype)
+    opac = sc.opacities.double()[None].repeat(C, 1).to(DEV, dtype)
+    tex = sc.textures.to(DEV, dtype).to(tex_dtype)
+    flat = st["flatten_ids"]
+    fargs = (st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], None, None, s.width, s.height,
+             tile_size, st["offsets"], flat, 0.0, 1.0)
+    live = _ops.live_entries_buffer(flat, tile_size)
+    live0 = _ops.live_entries_buffer(flat, tile_size)
+    smp = _ops.pair_samples_buffer(flat, tile_size)
+    cnt, cnt0 = _ops.tex_counts_buffer(tex), _ops.tex_counts_buffer(tex)
+    f1 = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(*fargs, live_entries=live, tex_counts=cnt,
+                                                             pair_samples=smp)
+    f0 = _ops.rasterize_to_pixels_fwd_textured_gaussians_full(*fargs, live_entries=live0, tex_counts=cnt0)
+    for i, (a, b) in enumerate(zip(f0, f1)):
+        if i == 7:  # contribution sums: float atomics in any order
+            assert rel_err(b, a) < 1e-6
+        else:
+            assert torch.equal(a, b), i
+    assert torch.equal(cnt, cnt0)
+    H, W = s.height, s.width
+    gen = torch.Generator().manual_seed(11)
+    vc = torch.randn(C, H, W, 3, generator=gen).to(DEV)
+    va = torch.randn(C, H, W, 1, generator=gen).to(DEV)
+    vn = torch.randn(C, H, W, 3, generator=gen).to(DEV)
+    vd = torch.randn(C, H, W, 1, generator=gen).to(DEV)
+    vm = torch.randn(C, H, W, 1, generator=gen).to(DEV)
+    densify = torch.zeros_like(st["means2d"])
+    bargs = [st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], densify, None, None, W, H,
+             tile_size, st["offsets"], flat, 1.0, f1[0], f1[1], f1[5], f1[6], vc, va, vn, vd, vm, False]
+    for records in (False, True):
+        opts = dict(live_entries=live, render_trans=f1[10])
+        if records:
+            opts["tex_counts"] = cnt
+        ref = _ops.rasterize_to_pixels_bwd_textured_gaussians(*bargs, **opts)
+        if records:
+            opts["tex_counts"] = cnt.clone()
+        got = _ops.rasterize_to_pixels_bwd_textured_gaussians(*bargs, pair_samples=smp, **opts)
+        for k, (a, b) in enumerate(zip(ref[1:], got[1:])):
+            assert torch.isfinite(b).all(), (records, k)
+            assert rel_err(b, a) < 1e-5, (records, k, rel_err(b, a))
+        assert float(got[5].abs().sum()) > 0

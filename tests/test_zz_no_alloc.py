@@ -68,6 +68,6 @@ def test_noalloc_refuses_more_than_the_scratch():
     for dtype, per in ((0, 29 * 4), (1, 29 * 8)):  # TGS_F32, TGS_F64: 29 values per entry
         too_many = SCRATCH // per + 1
         rc = L.tgs_rasterize_fwd_noalloc(dtype, 1, too_many, 3, *([null] * 6), 4, 4, 1.0, 1.0, null, null, null, 0,
-                                         16, 16, 16, 1, 1, null, null, 0, 0.0, 1.0, *([null] * 15), 0, 0, null)
+                                         16, 16, 16, 1, 1, null, null, 0, 0.0, 1.0, *([null] * 16), 0, 0, null)
         assert rc == 1, rc
         assert b"static scratch" in L.tgs_last_error()

@@ -16,4 +16,4 @@ void set_error(const char *fmt, ...) {
 }  // namespace tgs
 
 extern "C" const char *tgs_last_error(void) { return tgs::g_err; }
-extern "C" int tgs_abi_version(void) { return 1; }
+extern "C" int tgs_abi_version(void) { return 2; }

@@ -81,15 +81,6 @@ template <> __device__ __forceinline__ Texel<double> load_texel<double>(const do
 // bounds for cameras >= 1).
 constexpr int kRec = 20;
 constexpr int kRecA = 2, kRecRow = 11, kRecOp = 12, kRecCol = 13, kRecNrm = 17;
-#ifndef TGS_BWD_ZERO_STEPS
-#define TGS_BWD_ZERO_STEPS 2
-#endif
-// fused zero-fill stores per visited entry in the backward (it visits only live entries)
-constexpr int kBwdZeroSteps = TGS_BWD_ZERO_STEPS;
-#ifndef TGS_FWD_ZERO_STEPS
-#define TGS_FWD_ZERO_STEPS 1
-#endif
-constexpr int kFwdZeroSteps = TGS_FWD_ZERO_STEPS;
 constexpr int kAbc = 9;  // backward scratch: centred sums of dL/dcr per (camera, Gaussian)
 
 template <typename S> struct RasterArgs {
@@ -113,9 +104,14 @@ template <typename S> struct RasterArgs {
     S *out_accd;  // optional: sum_i w_i depth_i per pixel (the forward's distortion accumulator)
     S *out_T;     // optional: the final transmittance per pixel (the backward's starting T)
     int32_t *last_ids, *median_ids;
-    // optional: per wave of a tile, one byte per list entry, nonzero where some pixel of
-    // the wave composited the entry ([waves per tile][n_isects]; forward writes, backward reads)
-    uint8_t *live;
+    // optional: per wave of a tile and list entry, the 64-bit mask of the wave's lanes
+    // that composited the entry ([waves per tile][n_isects]; forward writes, backward reads)
+    uint64_t *live;
+    // optional (float32, native or bf16 textures, K <= 3, with live): the texture sample
+    // of every composited pair inside the texture range, RGB (K channels) and alpha,
+    // [waves per tile][n_isects][64][4], packed per (wave, entry) in lane order over the
+    // lanes that have one; the backward reads them instead of the texels
+    S *samples;
     // measurement only (tgs_set_texel_marks): one byte per texel of the textures,
     // set to 1 by the forward for every texel it reads (native float textures)
     uint8_t *tex_seen;
@@ -143,6 +139,7 @@ template <typename S> struct RasterArgs {
     const uint8_t *zmask;
     int64_t zrow16;
     int32_t zsel, zres;
+    int32_t zsteps;  // fill stores (1 KB each) per visited list entry and wave
 };
 
 template <typename S> __device__ __forceinline__ S rec_int(int32_t v);
@@ -433,6 +430,23 @@ __device__ __forceinline__ S bilerp_blend(const BWeights<S> &w, const Texels<S> 
     return tfma(w.w11, t.t11.c[3], tfma(w.w01, t.t01.c[3], tfma(w.w10, t.t10.c[3], w.w00 * t.t00.c[3])));
 }
 
+// One cached texture sample (RGB, alpha) of a pair: 16 bytes in float32.
+template <typename S> __device__ __forceinline__ void store_sample(S *q, const S v[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = v[k];
+}
+template <> __device__ __forceinline__ void store_sample<float>(float *q, const float v[4]) {
+    *(float4 *)q = make_float4(v[0], v[1], v[2], v[3]);
+}
+template <typename S> __device__ __forceinline__ void load_sample(const S *q, S v[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = q[k];
+}
+template <> __device__ __forceinline__ void load_sample<float>(const float *q, float v[4]) {
+    const float4 t = *(const float4 *)q;
+    v[0] = t.x, v[1] = t.y, v[2] = t.z, v[3] = t.w;
+}
+
 // Texture-gradient update of one pixel for one texel row: the two horizontally
 // adjacent texels (u0, v) and (u1, v), RGBA each, 8 values; off = byte offset of
 // the first, du = byte distance to the second (0 when u0 == u1).
@@ -627,8 +641,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 #pragma unroll
     for (int k = 0; k < K; ++k) pc[k] = 0;
     bool done = !inside;
-    // live-entry bytes of this wave, one per list entry, written per chunk of 64 entries
-    uint8_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
+    // composited-lane masks of this wave, one per list entry, written per chunk of 64 entries
+    uint64_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
+    S *smp = (a.samples && live) ? a.samples + (int64_t)(r / kWave) * a.n_isects * (kWave * 4) : nullptr;
     // the box of this wavefront's pixel centres (full tile width: a wave may span partial rows)
     const int wv = r / kWave;
     const int rr0 = (wv * kWave) / a.ts, rr1 = min(a.ts - 1, (wv * kWave + kWave - 1) / a.ts);
@@ -644,14 +659,14 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
         bool cand = e < end;
         if (kCullable && a.cull && cand) cand = fwd_candidate<S, TEX>(a, ids[e], bx0, bx1, by0, by1);
         uint64_t todo = __ballot(cand);
-        uint64_t lw = 0;  // bit i: entry lo + i composited by some pixel of this wave
+        uint64_t my_mask = 0;  // lanes of this wave that composited entry lo + lane
         while (todo) {
             if (__ballot(!done) == 0) break;
             const int bit = __builtin_ctzll(todo);
             todo &= todo - 1;
             const int32_t idx = __builtin_amdgcn_readfirstlane(lo + bit);
 #pragma unroll
-            for (int zs = 0; zs < kFwdZeroSteps; ++zs) zr.step(lane);
+            for (int zs = 0; zs < a.zsteps; ++zs) zr.step(lane);
             const int32_t g = ids[idx];
             const GaussGeom<S> G = load_geom<S>(a, g);
             const GaussShade<S, K> H = load_shade<S, K>(a, g);
@@ -726,10 +741,25 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
                     if (tu && lane == 0) atomicAdd(a.tex_cnt + G.row, (int32_t)__popcll(tu));
                 }
             }
-            lw |= (uint64_t)(__ballot(used) != 0) << bit;
+            const uint64_t um = __ballot(used);
+            if (lane == bit) my_mask = um;
+            if constexpr ((TEX == kTexNative || TEX == kTexBF16) && K <= 3) {
+                if (smp && um) {
+                    // this pair's sample, at its rank among the wave's in-range composited lanes
+                    const uint64_t ut = __ballot(used && inr);
+                    if (used && inr) {
+                        const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ut >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)ut, 0u));
+                        S v4[4] = {0, 0, 0, a_s};
+#pragma unroll
+                        for (int k = 0; k < K; ++k) v4[k] = tex[k];
+                        store_sample(smp + ((int64_t)idx * kWave + pos) * 4, v4);
+                    }
+                }
+            }
         }
-        // the backward reads these bytes only up to the wave's last composited entry
-        if (live && e < end) live[e] = (uint8_t)((lw >> lane) & 1);
+        // the backward reads these masks only up to the wave's last composited entry
+        if (live && e < end) live[e] = my_mask;
     }
     zr.drain(lane);
     if (inside) {
@@ -824,7 +854,10 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
     }
     const bool uv_term = TEX && TEX != kTexMip2 && a.uv_mode != TGS_UV_NONE;  // mip2: the reference's rule only
     const int32_t first = min(end - 1, wave_last);
-    const uint8_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
+    const uint64_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
+    // the forward's samples: only for the reference uv rule (the others need the texels)
+    const S *smp = (a.samples && live && !uv_term) ? a.samples + (int64_t)(r / kWave) * a.n_isects * (kWave * 4)
+                                                   : nullptr;
     // per-Gaussian gradient fields in reduce-scatter order: dL/dA (9), colour (K),
     // normal (3), mean2d (2), opacity, |mean2d| (2); this lane adds field `slot`
     constexpr int kGradFields = 17 + K;
@@ -857,13 +890,16 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         todo ^= 1ull << bit;
         // wave-uniform (todo is a ballot); stated so the list and record reads stay scalar
         const int32_t idx = __builtin_amdgcn_readfirstlane(lo + bit);
+        // with the forward's samples, its composited-lane mask decides which lanes keep the pair
+        const uint64_t kept = smp ? as_const(live)[idx] : 0;
 #pragma unroll
-        for (int zs = 0; zs < kBwdZeroSteps; ++zs) zr.step(lane);
+        for (int zs = 0; zs < a.zsteps; ++zs) zr.step(lane);
         const int32_t g = ids[idx];
         const GaussGeom<S> G = load_geom<S>(a, g);
         const GaussShade<S, K> H = load_shade<S, K>(a, g);
         PairGeom<S> p = pair_geom<S>(px, py, G, a.vis_bias, a.g_weight);
         bool valid = inside && idx <= bin_final && p.okz;
+        if (smp) valid = valid && ((kept >> lane) & 1);
         S a_s = 1, tex[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) tex[k] = 0;
@@ -879,7 +915,26 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
             if (inr) a_s = mip2_blend<S, K>(mm, a.textures + (int64_t)G.row * a.tex_stride, tex);
         } else if constexpr (TEX) {
             inr = valid && in_tex_range(p, a);
-            if (inr) {
+            bool cached = false;
+            if constexpr (K <= 3) {
+                if (smp) {
+                    // the sample the forward stored, at the lane's rank among the in-range kept lanes
+                    cached = true;
+                    const uint64_t ut = __ballot(inr);
+                    if (inr) {
+                        bl = bilerp_setup<S>(p.sx, p.sy, a);
+                        bw = bweights(bl);
+                        const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ut >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)ut, 0u));
+                        S v4[4];
+                        load_sample(smp + ((int64_t)idx * kWave + pos) * 4, v4);
+#pragma unroll
+                        for (int k = 0; k < K; ++k) tex[k] = v4[k];
+                        a_s = v4[3];
+                    }
+                }
+            }
+            if (inr && !cached) {
                 bl = bilerp_setup<S>(p.sx, p.sy, a);
                 bw = bweights(bl);
                 tt = fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl);
@@ -1836,6 +1891,10 @@ static int dispatch(const RasterArgs<S> &a0, int K, int tex, hipStream_t st) {
     if (grid == 0) {  // no kernel to fuse the row fill into
         return a.zdst ? tgs_zero_fill_rows(a.zdst, (size_t)a.zrow16 * 16, a.N, a.zmask, a.zsel, st) : TGS_OK;
     }
+    // fill stores per visited entry: each wave drains what is left of its rows at its end.
+    // Pacing measured at 1M x 50x50 (backward, 18.4 GB of rows): 2 to 12 stores per entry
+    // give the same kernel time; the walk and the fill share the memory bandwidth.
+    if (a.zdst && a.zsteps == 0) a.zsteps = BWD ? 2 : 1;
 // bf16 texture storage is instantiated for float compute only
 #define TGS_CASE(KK)                                                                      \
     case KK:                                                                              \
@@ -1877,7 +1936,8 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
                         const int32_t *flatten_ids, int64_t n_isects, double g_weight, void *workspace,
                         void *zdst, const uint8_t *zmask, size_t zrow_bytes, int zsel, const int64_t *gids,
                         int64_t nnz) {
-    // zsel: low byte the mask value to fill, bits 8-15 the rows (by row & 7) to take; 0 there = all
+    // zsel: low byte the mask value to fill, bits 8-15 the rows (by row & 7) to take (0 there =
+    // all), bits 16-23 the fill stores per visited list entry (0 there = fill_steps' choice)
     memset(&a, 0, sizeof(a));
     a.gids = gids;
     a.E = gids ? nnz : C * N;
@@ -1886,6 +1946,7 @@ static void fill_common(RasterArgs<S> &a, int64_t C, int64_t N, const void *mean
     a.zrow16 = (int64_t)(zrow_bytes / 16);
     a.zsel = zsel & 0xff;
     a.zres = (zsel >> 8) & 0xff ? (zsel >> 8) & 0xff : 0xff;
+    a.zsteps = (zsel >> 16) & 0xff;
     a.C = C; a.N = N; a.n_isects = n_isects;
     a.means2d = (const S *)means2d; a.rt = (const S *)rt; a.colors = (const S *)colors;
     a.opac = (const S *)opac; a.normals = (const S *)normals; a.textures = (const S *)textures;
@@ -1947,6 +2008,17 @@ template <typename S> static void set_mip(RasterArgs<S> &a, int tex_mode, int64_
     if (tex_mode != kTexMip2) return;
     a.mip_log = mip_log_of(tex_h, tex_w);
     a.tex_stride = ((int64_t(1) << (2 * (a.mip_log + 1))) - 1) / 3 * 4;
+}
+
+// pair samples: float32 compute, native or bfloat16 textures, at most 3 colour channels
+// (the fourth value is the alpha sample), with the composited-lane masks
+static int check_pair_samples(const char *who, const void *samples, const void *live, int dtype, int tex_mode,
+                              int32_t K) {
+    TGS_CHECK_ARG(!samples || (live && dtype == TGS_F32 && (tex_mode == kTexNative || tex_mode == kTexBF16) && K <= 3 &&
+                               ((uintptr_t)samples % 16) == 0),
+                  "%s: pair_samples need live_entries, float32 compute, bilinear textures (float32 or bfloat16), "
+                  "at most 3 channels and 16-byte alignment", who);
+    return TGS_OK;
 }
 
 static int check_zero_rows(const void *dst, const uint8_t *mask, size_t row_bytes) {
@@ -2100,8 +2172,8 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  double g_weight, void *render_colors, void *render_alphas, void *render_normals,
                                  void *render_distort, void *render_median, int32_t *last_ids, int32_t *median_ids,
                                  void *contrib_sum, void *contrib_count, void *render_accum_depth,
-                                 void *render_trans, uint8_t *live_entries, int32_t *tex_counts,
-                                 void *zero_rows_dst,
+                                 void *render_trans, uint64_t *live_entries, void *pair_samples,
+                                 int32_t *tex_counts, void *zero_rows_dst,
                                  const uint8_t *zero_rows_mask,
                                  size_t zero_row_bytes, int zero_rows_select, void *workspace, size_t workspace_size,
                                  tgs_stream_t stream) {
@@ -2120,6 +2192,8 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
                   "rasterize_fwd: workspace of %zu bytes required", workspace_bytes(dtype, 1, E));
     rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
     if (rc) return rc;
+    rc = check_pair_samples("rasterize_fwd", pair_samples, live_entries, dtype, tex_mode, K);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
 #define TGS_FWD(S)                                                                                              \
     {                                                                                                           \
@@ -2137,6 +2211,7 @@ extern "C" int tgs_rasterize_fwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.out_T = (S *)render_trans;                                                                            \
         a.cull = cull;                                                                                          \
         a.live = live_entries;                                                                                  \
+        a.samples = (S *)pair_samples;                                                                          \
         a.tex_cnt = textures ? tex_counts : nullptr;                                                            \
         a.tex_seen = (tex_mode == kTexNative && g_texel_marks && g_texel_marks_n >= N * tex_h * tex_w)          \
                          ? g_texel_marks : nullptr;                                                             \
@@ -2196,7 +2271,8 @@ extern "C" int tgs_rasterize_fwd_noalloc(int dtype, int64_t C, int64_t N, int32_
                                          void *render_colors, void *render_alphas, void *render_normals,
                                          void *render_distort, void *render_median, int32_t *last_ids,
                                          int32_t *median_ids, void *contrib_sum, void *contrib_count,
-                                         void *render_accum_depth, void *render_trans, uint8_t *live_entries,
+                                         void *render_accum_depth, void *render_trans, uint64_t *live_entries,
+                                         void *pair_samples,
                                          int32_t *tex_counts, void *zero_rows_dst, const uint8_t *zero_rows_mask,
                                          size_t zero_row_bytes, int zero_rows_select, tgs_stream_t stream) {
     constexpr size_t cap = TGS_RASTERIZE_NOALLOC_SCRATCH_BYTES;
@@ -2219,7 +2295,7 @@ extern "C" int tgs_rasterize_fwd_noalloc(int dtype, int64_t C, int64_t N, int32_
                              height, tile_size, tile_width, tile_height, tile_offsets, flatten_ids, n_isects,
                              gs_contrib_threshold, g_weight, render_colors, render_alphas, render_normals,
                              render_distort, render_median, last_ids, median_ids, contrib_sum, contrib_count,
-                             render_accum_depth, render_trans, live_entries, tex_counts, zero_rows_dst,
+                             render_accum_depth, render_trans, live_entries, pair_samples, tex_counts, zero_rows_dst,
                              zero_rows_mask, zero_row_bytes, zero_rows_select, ws, cap, stream);
 }
 
@@ -2233,7 +2309,7 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                                  const int32_t *flatten_ids, int64_t n_isects, double g_weight,
                                  const void *render_colors, const void *render_alphas, const int32_t *last_ids,
                                  const int32_t *median_ids, const void *render_accum_depth,
-                                 const void *render_trans, const uint8_t *live_entries,
+                                 const void *render_trans, const uint64_t *live_entries, const void *pair_samples,
                                  const int32_t *tex_counts,
                                  const int64_t *tex_rec_incl, int32_t *tex_rec_cursor, void *tex_records,
                                  const void *v_render_colors, const void *v_render_alphas,
@@ -2268,6 +2344,8 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
                   "rasterize_bwd: workspace of %zu bytes required", workspace_bytes(dtype, 1, E));
     rc = check_zero_rows(zero_rows_dst, zero_rows_mask, zero_row_bytes);
     if (rc) return rc;
+    rc = check_pair_samples("rasterize_bwd", pair_samples, live_entries, dtype, tex_mode, K);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
 #define TGS_BWD(S)                                                                                              \
     {                                                                                                           \
@@ -2281,7 +2359,8 @@ extern "C" int tgs_rasterize_bwd(int dtype, int64_t C, int64_t N, int32_t K, con
         a.r_med = median_ids; a.v_c = (const S *)v_render_colors; a.v_a = (const S *)v_render_alphas;           \
         a.r_accd = (const S *)render_accum_depth;                                                               \
         a.r_T = (const S *)render_trans;                                                                        \
-        a.live = (uint8_t *)live_entries;                                                                       \
+        a.live = (uint64_t *)live_entries;                                                                      \
+        a.samples = (S *)pair_samples;                                                                          \
         const bool recs = textures && tex_records && tex_counts && tex_rec_incl && tex_rec_cursor &&           \
                           gather_fits<S>(tex_h, tex_w) &&                                                        \
                           (!zero_rows_dst || ((uintptr_t)zero_rows_mask % 4) == 0);                              \

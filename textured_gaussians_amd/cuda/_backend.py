@@ -55,12 +55,12 @@ _PROTOS = {
     "tgs_sh_bwd_cams": (c_int, [c_int, c_i64, c_i64, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
     "tgs_rasterize_fwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
                                   vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl, c_dbl,
-                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, c_int, vp,
-                                  c_size, vp]),
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, c_int,
+                                  vp, c_size, vp]),
     "tgs_rasterize_fwd_noalloc": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl,
                                           c_dbl, vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64,
                                           c_dbl, c_dbl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                                          c_size, c_int, vp]),
+                                          vp, c_size, c_int, vp]),
     "tgs_set_raster_events": (c_int, [c_int, vp, vp]),
     "tgs_set_texel_marks": (c_int, [vp, c_i64]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
@@ -79,7 +79,7 @@ _PROTOS = {
     "tgs_prefix_sum_i32": (c_int, [c_i64, vp, vp, vp, c_size, vp]),
     "tgs_rasterize_bwd": (c_int, [c_int, c_i64, c_i64, c_i32, vp, vp, vp, vp, vp, vp, c_i64, c_i64, c_dbl, c_dbl,
                                   vp, vp, vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, c_i64, c_dbl,
-                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
+                                  vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, c_int, vp, c_size, vp]),
     "tgs_gemm_bf16": (c_int, [c_int, c_int, c_int, c_i64, c_i64, c_i64, vp, c_i64, vp, c_i64, vp, vp, c_i64, vp,
                               c_i64, c_i32, vp]),

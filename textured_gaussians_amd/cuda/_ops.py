@@ -112,6 +112,10 @@ FWD_NO_CULL = 0x400  # OR-ed into the forward's dtype: visit every list entry (n
 BWD_RECORD_ROWS = 0x800  # OR-ed into the backward's dtype: only rows with a nonzero count take records
 # The forward's candidate pre-pass (same outputs; off only to compare against the full walk)
 FWD_CULL = True
+# Fused row fill: stores (1 KB each) per visited list entry and wavefront; 0 takes the
+# library's (1 in the forward, 2 in the backward).  The remainder of a wave's rows goes
+# out when its walk ends.
+FILL_STEPS = 0
 
 
 def _tex_dtype_flag(textured, textures, means2d):
@@ -431,6 +435,8 @@ def _zero_rows_args(zero_rows, N, select):
         return None, None, 0, 0
     dst, mask = zero_rows[0], zero_rows[1]
     sel = int(zero_rows[2]) if len(zero_rows) > 2 else select
+    if FILL_STEPS and not (sel >> 16) & 0xFF:
+        sel |= (int(FILL_STEPS) & 0xFF) << 16
     _check_input(dst, "zero_rows dst")
     _check_input(mask, "zero_rows mask")
     assert dst.shape[0] == N and mask.numel() >= N and mask.dtype == torch.uint8
@@ -483,7 +489,7 @@ def _packed_args(means2d, gaussian_ids, textures, n_gaussians):
 def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, trx, try_, normals, backgrounds,
                 masks, image_width, image_height, tile_size, tile_offsets, flatten_ids, gs_contrib_threshold,
                 g_weight, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None, n_gaussians=None,
-                mip_log_res=None, noalloc=False):
+                mip_log_res=None, noalloc=False, pair_samples=None):
     """noalloc: call tgs_rasterize_fwd_noalloc, whose workspace is the library's static
     per-GPU scratch (the outputs are still allocated here)."""
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
@@ -535,7 +541,8 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
               _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(), float(gs_contrib_threshold), float(g_weight),
               _p(renders), _p(alphas), _p(rnormals), _p(rdistort), _p(rmedian), _p(last_ids), _p(median_ids),
               _p(csum), _p(ccnt), _p(accd), _p(trans), _live_arg(live_entries, flatten_ids, tile_size),
-              _counts_arg(tex_counts, textures.shape[0] if textured else 0), zd, zm, zb, zs, *ws_args,
+              _samples_arg(pair_samples, live_entries), _counts_arg(tex_counts, textures.shape[0] if textured else 0),
+              zd, zm, zb, zs, *ws_args,
               _stream(means2d)),
               "rasterize_to_pixels_fwd")
     return renders, alphas, rnormals, rdistort, rmedian, last_ids, median_ids, csum, ccnt, accd, trans
@@ -551,10 +558,11 @@ def rasterize_to_pixels_fwd_textured_gaussians(means2d, ray_transforms, colors, 
 
 
 def live_entries_buffer(flatten_ids: Tensor, tile_size: int) -> Tensor:
-    """uint8 [waves per tile, n_isects] for the forward to record which list entries
-    each wavefront composited (no reference counterpart)."""
+    """int64 [waves per tile, n_isects] for the forward to record, per wavefront and list
+    entry, the mask of the wave's lanes that composited the entry (no reference
+    counterpart)."""
     waves = (tile_size * tile_size + 63) // 64
-    return torch.empty((waves, flatten_ids.numel()), dtype=torch.uint8, device=flatten_ids.device)
+    return torch.empty((waves, flatten_ids.numel()), dtype=torch.int64, device=flatten_ids.device)
 
 
 def _live_arg(live_entries, flatten_ids, tile_size):
@@ -562,9 +570,33 @@ def _live_arg(live_entries, flatten_ids, tile_size):
         return None
     _check_input(live_entries, "live_entries")
     waves = (tile_size * tile_size + 63) // 64
-    if live_entries.dtype != torch.uint8 or live_entries.numel() != waves * flatten_ids.numel():
-        raise RuntimeError(f"live_entries must be uint8 [{waves}, {flatten_ids.numel()}]")
+    if live_entries.dtype != torch.int64 or live_entries.numel() != waves * flatten_ids.numel():
+        raise RuntimeError(f"live_entries must be int64 [{waves}, {flatten_ids.numel()}]")
     return _p(live_entries)
+
+
+def pair_samples_bytes(flatten_ids: Tensor, tile_size: int) -> int:
+    """Bytes of ``pair_samples_buffer`` (16 per lane of every wavefront and list entry)."""
+    waves = (tile_size * tile_size + 63) // 64
+    return waves * flatten_ids.numel() * 64 * 16
+
+
+def pair_samples_buffer(flatten_ids: Tensor, tile_size: int) -> Tensor:
+    """float32 [waves per tile, n_isects, 64, 4]: the forward stores the texture sample of
+    each composited in-range pair (RGB, alpha) for the backward, which then reads no
+    texels (no reference counterpart).  Only the slots of composited pairs are written."""
+    waves = (tile_size * tile_size + 63) // 64
+    return torch.empty((waves, flatten_ids.numel(), 64, 4), dtype=torch.float32, device=flatten_ids.device)
+
+
+def _samples_arg(pair_samples, live_entries):
+    if pair_samples is None:
+        return None
+    _check_input(pair_samples, "pair_samples")
+    if live_entries is None or pair_samples.dtype != torch.float32 or \
+            pair_samples.numel() != live_entries.numel() * 64 * 4:
+        raise RuntimeError("pair_samples must be float32 [waves, n_isects, 64, 4] beside live_entries")
+    return _p(pair_samples)
 
 
 def tex_counts_buffer(textures: Tensor) -> Tensor:
@@ -644,7 +676,7 @@ def texture_adam_rows(records, visibility, tex_hi, tex_lo, exp_avg, exp_avg_sq, 
 
 
 def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_entries=None, tex_counts=None,
-                                                    gaussian_ids=None):
+                                                    gaussian_ids=None, pair_samples=None):
     """As ``rasterize_to_pixels_fwd_textured_gaussians`` plus a 10th output, the
     per-pixel distortion accumulator sum_i w_i colors[i, K-1] [C,H,W], and an 11th,
     the final transmittance T [C,H,W] (for the backward's ``render_trans``).
@@ -653,9 +685,11 @@ def rasterize_to_pixels_fwd_textured_gaussians_full(*args, zero_rows=None, live_
     (from ``live_entries_buffer``) receives the entries each wavefront composited,
     for the backward to visit only those.  ``tex_counts`` (from ``tex_counts_buffer``)
     receives per texture row the pairs that will add texture gradient, for the
-    backward's record-and-gather path."""
+    backward's record-and-gather path.  ``pair_samples`` (from ``pair_samples_buffer``,
+    with ``live_entries``) receives the texture samples of the composited pairs, for a
+    backward under the ``bilinear`` rule to read instead of the texels."""
     return _raster_fwd(True, *args, zero_rows=zero_rows, live_entries=live_entries, tex_counts=tex_counts,
-                       gaussian_ids=gaussian_ids)
+                       gaussian_ids=gaussian_ids, pair_samples=pair_samples)
 
 
 def rasterize_to_pixels_fwd_2dgs(means2d, ray_transforms, colors, opacities, normals, backgrounds, masks,
@@ -680,7 +714,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
                 g_weight, render_colors, render_alphas, last_ids, median_ids, v_render_colors, v_render_alphas,
                 v_render_normals, v_render_distort, v_render_median, absgrad, render_accum_depth=None,
                 v_textures_out=None, zero_rows=None, live_entries=None, tex_counts=None, gaussian_ids=None,
-                n_gaussians=None, mip_log_res=None, render_trans=None, record_rows=False):
+                n_gaussians=None, mip_log_res=None, render_trans=None, record_rows=False, pair_samples=None):
     ts = dict(means2d=means2d, ray_transforms=ray_transforms, colors=colors, opacities=opacities,
               normals=normals, densify=densify, tile_offsets=tile_offsets, flatten_ids=flatten_ids,
               render_colors=render_colors, render_alphas=render_alphas, last_ids=last_ids,
@@ -741,7 +775,8 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
               image_height, tile_size, tw_t, th_t, _p(tile_offsets), _p(flatten_ids), flatten_ids.numel(),
               float(g_weight),
               _p(render_colors), _p(render_alphas), _p(last_ids), _p(median_ids), _p(render_accum_depth),
-              _p(render_trans), _live_arg(live_entries, flatten_ids, tile_size), _p(rc_), _p(ri_), _p(rcur_),
+              _p(render_trans), _live_arg(live_entries, flatten_ids, tile_size),
+              _samples_arg(pair_samples, live_entries), _p(rc_), _p(ri_), _p(rcur_),
               _p(rrec_),
               _p(v_render_colors),
               _p(v_render_alphas), _p(v_render_normals), _p(v_render_distort), _p(v_render_median), uv_mode,
@@ -758,7 +793,7 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
 
 # Keyword options of the textured backward ops (no reference counterparts):
 # render_accum_depth, render_trans, v_textures_out, zero_rows, live_entries,
-# tex_counts, gaussian_ids (see _raster_bwd and include/tgs_hip.h).
+# pair_samples, tex_counts, gaussian_ids (see _raster_bwd and include/tgs_hip.h).
 def rasterize_to_pixels_bwd_textured_gaussians(*args, **opts):
     """Reference ``bilinear`` backward rule (no d(texture sample)/ds)."""
     return _raster_bwd(True, UV_NONE, *args, **opts)

@@ -390,6 +390,29 @@ def _records_fit(textures: Tensor, dtype: torch.dtype) -> bool:
     return textures.shape[1] * textures.shape[2] * 4 * torch.tensor([], dtype=dtype).element_size() <= 80 * 1024
 
 
+# The forward keeps the texture sample of every composited in-range pair for the
+# backward (16 bytes per pair, in slots of 1 KB per wavefront and list entry: 17 GB at
+# 1M Gaussians, 1297x840), which then reads no texels: True (when it fits), False.
+# Measured at that size (one MI355X): the forward +0.3 ms (1.1 GB of samples), the
+# backward -0.4 ms without its fused row fill and about even with it (the walk then
+# shares the memory bandwidth with 18.4 GB of fill stores), so it stays off.
+PAIR_SAMPLES = False
+# at most this fraction of the device memory free when the forward runs
+PAIR_SAMPLES_MEMORY_FRACTION = 0.25
+
+
+def _use_pair_samples(bwd_fn, means2d, colors, textures, flatten_ids, tile_size) -> bool:
+    """Pair samples apply to the reference ``bilinear`` rule (the other rules need the
+    texels' differences) in float32 with at most 3 channels, when the slots fit in a
+    quarter of the free device memory."""
+    if not PAIR_SAMPLES or bwd_fn != tgs_fns["bilinear"][1] or means2d.dtype != torch.float32:
+        return False
+    if colors.shape[-1] > 3 or textures.dim() != 4 or flatten_ids.numel() == 0:
+        return False
+    free, _ = torch.cuda.mem_get_info(means2d.device)
+    return _C.pair_samples_bytes(flatten_ids, tile_size) <= PAIR_SAMPLES_MEMORY_FRACTION * free
+
+
 def _use_records(textures: Tensor, dtype: torch.dtype) -> bool:
     """Texture gradients through records (see TEXTURE_GRAD_RECORDS)."""
     if TEXTURE_GRAD_RECORDS is None:
@@ -452,14 +475,19 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
                     # rows with many pairs through records and the gather (it overwrites them)
                     ctx.tex_counts = _C.tex_counts_buffer(textures)
                     ctx.record_rows = True
-        # the forward records which list entries each wavefront composited; the backward
-        # visits only those
+        # the forward records which lanes of each wavefront composited each list entry; the
+        # backward visits only those entries, and under the reference rule in float32 it reads
+        # the forward's texture samples instead of the texels
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
+        ctx.samples = None
+        if ctx.live is not None and _use_pair_samples(bwd_fn, means2d, colors, textures, flatten_ids, tile_size):
+            ctx.samples = _C.pair_samples_buffer(flatten_ids, tile_size)
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count, accum_depth, trans) = _make_lazy_cuda_func(fwd_fn + "_full")(
             means2d, ray_transforms, colors, opacities, textures, texture_range_x, texture_range_y, normals,
             backgrounds, masks, width, height, tile_size, isect_offsets, flatten_ids, gs_contrib_threshold,
-            g_weight, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids)
+            g_weight, zero_rows=zero_rows, live_entries=ctx.live, tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids,
+            pair_samples=ctx.samples)
         if ctx.tex_counts is not None:
             if ctx.record_rows:  # only the heavy rows take records
                 ctx.tex_counts = torch.where(ctx.tex_counts >= HEAVY_ROW_PAIRS, ctx.tex_counts, 0)
@@ -498,8 +526,10 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
             v_render_alphas.contiguous(), v_render_normals.contiguous(), v_render_distort.contiguous(),
             v_render_median.contiguous(), ctx.absgrad, render_accum_depth=accum_depth if exact else None,
             render_trans=trans, v_textures_out=v_tex_buf, zero_rows=zero_rows, live_entries=ctx.live,
-            tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids, record_rows=ctx.record_rows)
+            tex_counts=ctx.tex_counts, gaussian_ids=gaussian_ids, record_rows=ctx.record_rows,
+            pair_samples=ctx.samples)
         ctx.live = None
+        ctx.samples = None
         ctx.tex_counts = None
         if ctx.adam is not None:
             opt, rows = ctx.adam
