@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-layouts", action="store_true", help="skip the mipmapped2 and packed-layout legs")
     ap.add_argument("--fwd-no-cull", action="store_true",
                     help="forward without the candidate pre-pass (every list entry visited; same outputs)")
+    ap.add_argument("--lds-atomics", action="store_true",
+                    help="also measure the GPU's LDS atomic-add rates (float32, float64, uint32, plain)")
     ap.add_argument("--grad-paths", action="store_true",
                     help="also time the texture-gradient variants (records + gather, hybrid) on the headline scene")
     return ap.parse_args()
@@ -294,12 +296,14 @@ def grad_paths_leg(tg, sc, params, steps, warmup):
     """The headline iteration under each texture-gradient variant of the autograd layer
     (all give the same gradients): per-pair records summed per row by the gather kernel
     (the product path: the backward walk zero-fills the unlisted rows), the same with the
-    forward walk filling them, float atomics into the row-filled gradient, the same with
+    backward reading the forward's pair samples instead of the texels, the same with the
+    forward walk filling the unlisted rows, float atomics into the row-filled gradient, the same with
     one dense fill, and records for the rows with at least 64 pairs only (atomics for the
     rest).  The records variants run the rasterizer backward without texture-gradient
     atomics; the gather runs after it, inside the backward op."""
     from textured_gaussians_amd.cuda import _wrapper
     variants = [("records_gather", {}),
+                ("records_pair_samples", {"PAIR_SAMPLES": True}),
                 ("records_forward_fills_unlisted", {"TEXTURE_GRAD_RECORDS": True, "FWD_FILL_RESIDUES": 0xFF}),
                 ("atomics_fused_fill", {"TEXTURE_GRAD_RECORDS": False}),
                 ("atomics_dense_fill", {"TEXTURE_GRAD_RECORDS": False, "FUSED_FILL": False}),
@@ -582,6 +586,11 @@ def main():
                      "time_ms": round(t_fwd + t_bwd, 3)},
         "raster_bwd_only": {"achieved": round(achieved_bwd, 1), "frac": round(achieved_bwd / HBM_PEAK_GBS, 4)},
     }
+    if args.lds_atomics:
+        from textured_gaussians_amd.cuda import _ops
+        rates = _ops.lds_atomic_rates(dev)
+        line["lds_atomics"] = {"unit": "lanes per clock per CU", "addresses": "random, 40 KB LDS buffer",
+                               **{k: round(v, 3) for k, v in rates.items()}}
     if args.grad_paths:
         print("[bench] texture-gradient variants", file=sys.stderr, flush=True)
         line["grad_paths"] = grad_paths_leg(tg, sc, params, max(3, args.steps // 2), 2)

@@ -289,6 +289,13 @@ int tgs_set_raster_events(int which, void *begin, void *end);
  * zero-fills); the number of nonzero bytes is the count of distinct texels read.
  * NULL, 0 clears it.  One setting per process. */
 int tgs_set_texel_marks(uint8_t *marks, int64_t n_texels);
+/* Measurement hook (no reference counterpart): LDS atomic-add throughput of the
+ * current GPU, in lanes per clock per CU, for n (1..4) access kinds over random
+ * addresses of a 40 KB LDS buffer: [0] float32 atomic add, [1] float64 atomic
+ * add, [2] uint32 atomic add, [3] float32 read, add and write (plain accesses).
+ * The texture-gradient gather sums in float64 because of [0] against [1] on
+ * gfx950.  Synchronizes the stream; allocates a few KB of device memory. */
+int tgs_lds_atomic_rates(double *lanes_per_clock_per_cu, int32_t n, tgs_stream_t stream);
 /* Bytes of one texture-gradient record (see tgs_rasterize_bwd). */
 size_t tgs_tex_record_bytes(int dtype);
 /* 1 if tgs_rasterize_bwd sums texture-gradient records of this compute dtype and

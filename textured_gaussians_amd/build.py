@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libtgs_hip.so")
 ARCH = os.environ.get("TGS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.cpp", "projection.hip", "isect.hip", "sh.hip", "raster.hip", "adam.hip", "fill.hip", "neural.hip",
-           "mipmap.hip", "ssim.hip"]
+           "mipmap.hip", "ssim.hip", "lds_rates.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

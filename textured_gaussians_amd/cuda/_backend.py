@@ -63,6 +63,7 @@ _PROTOS = {
                                           vp, c_size, c_int, vp]),
     "tgs_set_raster_events": (c_int, [c_int, vp, vp]),
     "tgs_set_texel_marks": (c_int, [vp, c_i64]),
+    "tgs_lds_atomic_rates": (c_int, [vp, c_i32, vp]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
     "tgs_tex_record_bytes": (c_size, [c_int]),
     "tgs_texture_gather_fast": (c_int, [c_int, c_i64, c_i64]),

@@ -966,6 +966,19 @@ def set_texel_marks(marks: Optional[Tensor]) -> None:
     check(lib().tgs_set_texel_marks(_p(marks), marks.numel()), "set_texel_marks")
 
 
+def lds_atomic_rates(device=None) -> dict:
+    """LDS atomic-add throughput of the GPU, lanes per clock per CU, by access kind
+    (measurement hook, no reference counterpart; see tgs_lds_atomic_rates)."""
+    import ctypes
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    out = (ctypes.c_double * 4)()
+    with torch.cuda.device(dev):
+        check(lib().tgs_lds_atomic_rates(ctypes.cast(out, ctypes.c_void_p), 4,
+                                         torch.cuda.current_stream(dev).cuda_stream), "lds_atomic_rates")
+    return {"atomic_add_f32": out[0], "atomic_add_f64": out[1], "atomic_add_u32": out[2],
+            "read_add_write_f32": out[3]}
+
+
 def zero_fill_(t: Tensor) -> Tensor:
     """Zero-fills a contiguous device tensor in place with one native launch on
     the current stream (no reference counterpart; used for the texture
