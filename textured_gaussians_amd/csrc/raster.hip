@@ -1227,28 +1227,60 @@ struct __align__(16) GatherEntry {
     int64_t start;   // its first record
 };
 
-__global__ __launch_bounds__(256) void gather_list_kernel(int64_t N, const uint8_t *__restrict__ mask,
-                                                          const int32_t *__restrict__ cnt,
-                                                          const int32_t *__restrict__ cur,
-                                                          const int64_t *__restrict__ incl,
-                                                          GatherEntry *__restrict__ list, int64_t cap,
-                                                          uint32_t *__restrict__ count) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool sel = false;
-    int32_t n = 0;
-    if (r < N) {
-        n = min(cnt[r], cur[r]);  // records actually written
-        sel = mask ? mask[r] == 1 : n > 0;
+// Rows are taken kGatherListRows per workgroup (4 per thread): the workgroup sums its
+// selected rows in LDS and claims its place in the list with one atomic (per-wave
+// atomics on the one counter serialize at the memory side: 0.18 ms at N = 1M).
+constexpr int kGatherListThreads = 1024, kGatherListPer = 4;
+constexpr int64_t kGatherListRows = (int64_t)kGatherListThreads * kGatherListPer;
+
+__global__ __launch_bounds__(kGatherListThreads) void gather_list_kernel(int64_t N, const uint8_t *__restrict__ mask,
+                                                                         const int32_t *__restrict__ cnt,
+                                                                         const int32_t *__restrict__ cur,
+                                                                         const int64_t *__restrict__ incl,
+                                                                         GatherEntry *__restrict__ list, int64_t cap,
+                                                                         uint32_t *__restrict__ count) {
+    constexpr int kWaves = kGatherListThreads / kWave;
+    __shared__ uint32_t wave_base[kWaves];
+    __shared__ uint32_t block_base;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    bool sel[kGatherListPer];
+    int32_t n[kGatherListPer];
+    uint32_t before[kGatherListPer];  // selected rows of this wave ahead of this lane's row k
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < kGatherListPer; ++k) {
+        const int64_t r = (int64_t)blockIdx.x * kGatherListRows + (int64_t)k * kGatherListThreads + threadIdx.x;
+        sel[k] = false;
+        n[k] = 0;
+        if (r < N) {
+            n[k] = min(cnt[r], cur[r]);  // records actually written
+            sel[k] = mask ? mask[r] == 1 : n[k] > 0;
+        }
+        const uint64_t bits = __ballot(sel[k]);
+        before[k] = total + __builtin_amdgcn_mbcnt_hi((uint32_t)(bits >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bits, 0u));
+        total += (uint32_t)__popcll(bits);
     }
-    const uint64_t bits = __ballot(sel);
-    if (bits == 0) return;
-    uint32_t base = 0;
-    if ((threadIdx.x & (kWave - 1)) == 0) base = atomicAdd(count, (uint32_t)__popcll(bits));
-    base = __builtin_amdgcn_readfirstlane(base);
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bits >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bits, 0u));
-    // the list has room for every row a mask of tgs_mark_rows can select; rows past `cap`
-    // (another mask) are not listed, and the gather reads at most `cap` entries
-    if (sel && (int64_t)base + rank < cap) list[base + rank] = GatherEntry{(int32_t)r, n, r ? incl[r - 1] : 0};
+    if (lane == 0) wave_base[wave] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t t = wave_base[w];
+            wave_base[w] = sum;
+            sum += t;
+        }
+        block_base = sum ? atomicAdd(count, sum) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = block_base + wave_base[wave];
+#pragma unroll
+    for (int k = 0; k < kGatherListPer; ++k) {
+        const int64_t r = (int64_t)blockIdx.x * kGatherListRows + (int64_t)k * kGatherListThreads + threadIdx.x;
+        // the list has room for every row a mask of tgs_mark_rows can select; rows past
+        // `cap` (another mask) are not listed, and the gather reads at most `cap` entries
+        const int64_t pos = (int64_t)base + before[k];
+        if (sel[k] && pos < cap) list[pos] = GatherEntry{(int32_t)r, n[k], r ? incl[r - 1] : 0};
+    }
 }
 
 __device__ __forceinline__ GatherEntry load_entry(cptr<GatherEntry> L, int32_t k) {
@@ -2055,8 +2087,8 @@ static int launch_gather(int64_t N, int64_t tex_h, int64_t tex_w, const RasterAr
     const int64_t cap = ((int64_t)rec_bytes<S>(1, a.E) - 16) / (int64_t)sizeof(GatherEntry);
     zero_kernel<uint32_t><<<1, 64, 0, st>>>(1, count);
     if (N > 0)
-        gather_list_kernel<<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(N, mask, a.rec_cnt, a.rec_cur, a.rec_incl, list,
-                                                                       cap, count);
+        gather_list_kernel<<<(unsigned)ceil_div(N, kGatherListRows), kGatherListThreads, 0, st>>>(
+            N, mask, a.rec_cnt, a.rec_cur, a.rec_incl, list, cap, count);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const size_t lds2 = g2_lds_bytes(tex_h * tex_w);
