@@ -531,3 +531,45 @@ def test_pair_samples_replace_the_texel_reads(tex_dtype, tile_size, width, heigh
             assert torch.isfinite(b).all(), (records, k)
             assert rel_err(b, a) < 1e-5, (records, k, rel_err(b, a))
         assert float(got[5].abs().sum()) > 0
+
+
+def test_gather_rows_with_many_records():
+    """Large magnified splats put thousands of records on a texture row, many on one texel
+    quad: the row gather then adds chunks past the ones its adders prefetch (4 x 2 x 64
+    records) and its float64 LDS atomics meet on equal addresses.  The product path (records
+    and the gather) matches the oracle over the whole frame in float32, and equals the
+    float-atomic path's gradients."""
+    from textured_gaussians_amd.cuda import _wrapper
+    from gpu_util import all_tiles, raster_tile_parity
+    dtype = torch.float32
+    sc = scenes.config1(n=400, width=128, height=96, tex_res=50, logscale_mu=-2.2, logscale_sigma=0.6, seed=21)
+    st = gpu_stages(sc, dtype)
+    s = st["scene"]
+    colors = torch.clamp_min(0.28209479177387814 * s.colors[None, :, 0, :] + 0.5, 0.0).contiguous()
+    opac = s.opacities[None].contiguous()
+    tex = s.textures
+    cnt = _ops.tex_counts_buffer(tex)
+    _ops.rasterize_to_pixels_fwd_textured_gaussians_full(
+        st["means2d"], st["rt"], colors, opac, tex, 3.0, 3.0, st["normals"], None, None, s.width, s.height, 16,
+        st["offsets"], st["flatten_ids"], 0.0, 1.0, tex_counts=cnt)
+    assert int(cnt.max()) > 4 * 4 * 2 * 64, int(cnt.max())
+    assert bool(_wrapper._use_records(tex, dtype))
+    stats = raster_tile_parity(st, colors, opac, tex, s.backgrounds, dtype, all_tiles(st))
+    print("many records per row", int(cnt.max()), stats)
+    # the same gradients through float atomics
+    grads = {}
+    saved = _wrapper.TEXTURE_GRAD_RECORDS
+    try:
+        for mode in (True, False):
+            _wrapper.TEXTURE_GRAD_RECORDS = mode
+            t = tex.clone().requires_grad_(True)
+            m2 = st["means2d"].clone().requires_grad_(True)
+            out = _wrapper.rasterize_to_pixels_textured_gaussians(
+                m2, st["rt"], colors, opac, t, 3.0, 3.0, st["normals"], torch.zeros_like(st["means2d"]), s.width,
+                s.height, 16, st["offsets"], st["flatten_ids"], backgrounds=s.backgrounds)
+            (out[0].square().sum() + out[1].sum()).backward()
+            grads[mode] = (t.grad.clone(), m2.grad.clone())
+    finally:
+        _wrapper.TEXTURE_GRAD_RECORDS = saved
+    for a, b in zip(grads[False], grads[True]):
+        assert rel_err(b, a) < 1e-5, rel_err(b, a)
