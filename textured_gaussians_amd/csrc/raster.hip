@@ -40,6 +40,7 @@
 // contraction disabled and explicit FMAs, so both take identical keep/skip
 // decisions.
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 
@@ -551,47 +552,65 @@ template <> __device__ __forceinline__ RecHead<double> load_rec_head<double>(con
 // box [x0, x1] x [y0, y1]: false only if no pixel of the box can keep the pair in the
 // kernel's own arithmetic, so skipping the entry changes no output.  With cr affine in
 // the pixel (cr = crc + u A0 + v A1 around the box centre, |u| <= hx, |v| <= hy), each
-// |cr_k| lies within |crc_k| -/+ (hx |A0_k| + hy |A1_k|), widened by the rounding of
-// the kernel's two float32 FMAs; |s| is then at least |cr.xy|_min / |cr.z|_max.  The
-// pair's sigma = 1/2 min(|s|^2, 2 |d|^2) is bounded below by that and by the distance
-// from the mean to the box.  A texture can hold any alpha, so a textured entry is
-// skipped only when the whole box lies outside the texture range (a_s = 1 there) and
-// op (vis_bias + g e^-sigma) stays below 1/255; the plain kernel needs the second part
-// only.  Margins cover float32 rounding of sigma (1e-3 + 1e-5 relative) and of alpha
-// (1e-3 relative).  Non-positive or non-finite opacities and NaN bounds keep the entry.
+// |cr_k| lies within |crc_k| -/+ (hx |A0_k| + hy |A1_k|), widened by err_k, a bound on
+// the rounding of the kernel's two FMAs and on the rounding of this test itself; |s| is
+// then at least |cr.xy|_min / |cr.z|_max.  The pair's sigma = 1/2 min(|s|^2, 2 |d|^2) is
+// bounded below by that and by the distance from the mean to the box.  A texture can hold
+// any alpha, so a textured entry is skipped only when the whole box lies outside the
+// texture range (a_s = 1 there) and op (vis_bias + g e^-sigma) stays below 1/255; the plain
+// kernel needs the second part only.  Non-positive or non-finite opacities and NaN bounds
+// keep the entry.
+//
+// Float32 entries are tested in float32 (a float64 test held the forward at 119 VGPRs,
+// four waves per SIMD; this one at 75, six).  Rounding, with u = 2^-24 and
+// M = (|dxc| + hx) |A0_k| + (|dyc| + hy) |A1_k| + |C0_k|:
+//  * the kernel's cr_k: dx = mx - px rounded, then two FMAs: <= 3 u M;
+//  * this test's crc_k (mx - cx rounded, two FMAs) <= 3 u M, the spread's FMA and product
+//    <= 2 u M, the two subtractions of lo / additions of hi <= 4 u M: <= 9 u M;
+//  * err_k = 1.2e-6 M >= (3 + 9) u M = 7.2e-7 M, with room for err_k's own rounding;
+//  * the quotient, squares and sums of sigma's bound: a few u relative, inside the 1e-5
+//    relative and 1e-3 absolute margins (which also cover the kernel's rcp and squares);
+//    an |dxc| - hx that cancels leaves an absolute error <= 2 u (|dxc| + hx) ddx in ddx^2,
+//    <= 2e-6 for ddx < 1 (tile boxes have hx, hy <= 16) and <= 34 u relative above;
+//  * __expf and the products against 1/255: relative errors ~1e-6, inside the 1e-3 alpha
+//    margin.
+// Float64 entries keep the float64 test with err_k = 1e-15 M.
 template <typename S, int TEX>
-__device__ __forceinline__ bool fwd_candidate(const RasterArgs<S> &a, int32_t g, double x0, double x1, double y0,
-                                              double y1) {
+__device__ __forceinline__ bool fwd_candidate(const RasterArgs<S> &a, int32_t g, float x0, float x1, float y0,
+                                              float y1) {
+    using B = typename std::conditional<sizeof(S) == 4, float, double>::type;
     const RecHead<S> R = load_rec_head<S>(a.rec + (int64_t)g * kRec);
-    const double mx = R.v[0], my = R.v[1], op = R.v[kRecOp];
-    const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1), hx = 0.5 * (x1 - x0), hy = 0.5 * (y1 - y0);
-    const double dxc = mx - cx, dyc = my - cy;
-    const double dxm = fabs(dxc) + hx, dym = fabs(dyc) + hy;
-    const double eps = sizeof(S) == 4 ? 4e-7 : 1e-15;
-    double lo[3], hi[3];
+    const B mx = (B)R.v[0], my = (B)R.v[1], op = (B)R.v[kRecOp];
+    const B cx = B(0.5) * (B(x0) + B(x1)), cy = B(0.5) * (B(y0) + B(y1));
+    const B hx = B(0.5) * (B(x1) - B(x0)), hy = B(0.5) * (B(y1) - B(y0));
+    const B dxc = mx - cx, dyc = my - cy;
+    const B dxm = fabs(dxc) + hx, dym = fabs(dyc) + hy;
+    const B eps = sizeof(S) == 4 ? B(1.2e-6) : B(1e-15);
+    B lo[3], hi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double a0 = R.v[kRecA + k], a1 = R.v[kRecA + 3 + k], c0 = R.v[kRecA + 6 + k];
-        const double crc = c0 - dxc * a0 - dyc * a1;
-        const double spread = hx * fabs(a0) + hy * fabs(a1);
-        const double err = eps * (dxm * fabs(a0) + dym * fabs(a1) + fabs(c0));
+        const B a0 = (B)R.v[kRecA + k], a1 = (B)R.v[kRecA + 3 + k], c0 = (B)R.v[kRecA + 6 + k];
+        const B crc = c0 - dxc * a0 - dyc * a1;
+        const B spread = hx * fabs(a0) + hy * fabs(a1);
+        const B err = eps * (dxm * fabs(a0) + dym * fabs(a1) + fabs(c0));
         lo[k] = fabs(crc) - spread - err;
         hi[k] = fabs(crc) + spread + err;
     }
-    const double ddx = fmax(0.0, fabs(dxc) - hx), ddy = fmax(0.0, fabs(dyc) - hy);
-    double sig = ddx * ddx + ddy * ddy;  // 1/2 of 2 |d|^2
+    const B ddx = fmax(B(0), fabs(dxc) - hx), ddy = fmax(B(0), fabs(dyc) - hy);
+    B sig = ddx * ddx + ddy * ddy;  // 1/2 of 2 |d|^2
     bool out_of_range = false;
     {
         // hi[2] == 0 means cr.z == 0 over the box (no pixel keeps the pair): the bounds go
         // to +inf, or NaN where lo is 0 too, which keeps the entry
-        const double lx = fmax(0.0, lo[0]) / hi[2] * (1.0 - 1e-5), ly = fmax(0.0, lo[1]) / hi[2] * (1.0 - 1e-5);
-        sig = fmin(sig, 0.5 * (lx * lx + ly * ly));
-        out_of_range = lx > (double)a.trx * (1.0 + 1e-5) || ly > (double)a.try_ * (1.0 + 1e-5);
+        const B shrink = B(1) - B(1e-5);
+        const B lx = fmax(B(0), lo[0]) / hi[2] * shrink, ly = fmax(B(0), lo[1]) / hi[2] * shrink;
+        sig = fmin(sig, B(0.5) * (lx * lx + ly * ly));
+        out_of_range = lx > (B)a.trx * (B(1) + B(1e-5)) || ly > (B)a.try_ * (B(1) + B(1e-5));
     }
-    sig = sig * (1.0 - 1e-5) - 1e-3;
-    if (!(op > 0.0) || !(op < 1e30) || !(sig == sig)) return true;
-    const double vis_ub = (double)a.vis_bias + fmax(0.0, (double)a.g_weight * exp(-fmax(sig, 0.0)));
-    const bool faint = op * vis_ub < (1.0 / 255.0) * (1.0 - 1e-3);
+    sig = sig * (B(1) - B(1e-5)) - B(1e-3);
+    if (!(op > B(0)) || !(op < B(1e30)) || !(sig == sig)) return true;
+    const B vis_ub = (B)a.vis_bias + fmax(B(0), (B)a.g_weight * tgs_exp<B>(-fmax(sig, B(0))));
+    const bool faint = op * vis_ub < (B(1) / B(255)) * (B(1) - B(1e-3));
     if constexpr (TEX == kTexNone) return !faint;
     return !(faint && out_of_range);
 }
@@ -647,8 +666,8 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
     // the box of this wavefront's pixel centres (full tile width: a wave may span partial rows)
     const int wv = r / kWave;
     const int rr0 = (wv * kWave) / a.ts, rr1 = min(a.ts - 1, (wv * kWave + kWave - 1) / a.ts);
-    const double bx0 = tx * a.ts + 0.5, bx1 = tx * a.ts + a.ts - 0.5;
-    const double by0 = ty * a.ts + rr0 + 0.5, by1 = ty * a.ts + rr1 + 0.5;
+    const float bx0 = tx * a.ts + 0.5f, bx1 = tx * a.ts + a.ts - 0.5f;
+    const float by0 = ty * a.ts + rr0 + 0.5f, by1 = ty * a.ts + rr1 + 0.5f;
     constexpr bool kCullable = TEX != kTexMip2;  // the mip2 sample's validity depends on the footprint
 
     // chunks of 64 entries: each lane tests one entry of the chunk, then the wave walks the
@@ -778,8 +797,13 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 }
 
 // ----------------------------------------------------------------- backward
-template <typename S, int K, int TEX>
-__global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
+// LEAN: the instantiation for the common float32 case, with the texture gradient through
+// per-pair records only, the reference's uv rule (no d(sample)/ds term) and no forward
+// samples.  Dropping the texel values kept for the uv term, the float-atomic staging and
+// the sample reads takes it from 125 to 96 VGPRs at K = 3: five waves per SIMD instead of
+// four (K = 4 stays at four: at five it would spill).
+template <typename S, int K, int TEX, bool LEAN>
+__global__ __launch_bounds__(256, sizeof(S) == 4 ? (LEAN && K <= 3 ? 5 : 4) : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
     extern __shared__ __align__(16) char smem[];
     // per-wave staging of texture-gradient updates (texel rows x 64 pixels: 2 rows for
     // bilinear, 4 for mip2, two per level)
@@ -852,11 +876,12 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
         ds_u = S(2) * a.su;
         ds_v = S(2) * a.sv;
     }
-    const bool uv_term = TEX && TEX != kTexMip2 && a.uv_mode != TGS_UV_NONE;  // mip2: the reference's rule only
+    // mip2: the reference's rule only
+    const bool uv_term = !LEAN && TEX && TEX != kTexMip2 && a.uv_mode != TGS_UV_NONE;
     const int32_t first = min(end - 1, wave_last);
     const uint64_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
     // the forward's samples: only for the reference uv rule (the others need the texels)
-    const S *smp = (a.samples && live && !uv_term) ? a.samples + (int64_t)(r / kWave) * a.n_isects * (kWave * 4)
+    const S *smp = (!LEAN && a.samples && live && !uv_term) ? a.samples + (int64_t)(r / kWave) * a.n_isects * (kWave * 4)
                                                    : nullptr;
     // per-Gaussian gradient fields in reduce-scatter order: dL/dA (9), colour (K),
     // normal (3), mean2d (2), opacity, |mean2d| (2); this lane adds field `slot`
@@ -954,7 +979,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? 4 : 1) void raster_bwd_kernel
             tl = __ballot(valid && inr);
             tpos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(tl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tl, 0u));
             // records for this entry's row, or float atomics (uniform: one row per entry)
-            use_rec = a.recs && (!a.rec_rows || as_const(a.rec_cnt)[G.row] > 0);
+            use_rec = LEAN || (a.recs && (!a.rec_rows || as_const(a.rec_cnt)[G.row] > 0));
             if (use_rec && tl) {
                 // claim this entry's slots in the row's record segment now; used at the end
                 if (lane == 0) rbase = atomicAdd(a.rec_cur + G.row, (int32_t)__popcll(tl));
@@ -1904,11 +1929,20 @@ template <typename S, int K, int TEX> static void launch_bwd(const RasterArgs<S>
     if (stage > 64 * 1024) {  // mip2 in float64: 72 KB of staging per 256-thread block
         static bool raised = false;
         if (!raised)
-            raised = hipFuncSetAttribute((const void *)raster_bwd_kernel<S, K, TEX>,
+            raised = hipFuncSetAttribute((const void *)raster_bwd_kernel<S, K, TEX, false>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage) == hipSuccess;
     }
+    // the lean instantiation: float32, texture gradient through records for every row,
+    // reference uv rule, no forward samples (it needs no LDS staging)
+    constexpr bool kLeanOk = sizeof(S) == 4 && (TEX == kTexNative || TEX == kTexBF16);
+    const bool lean = kLeanOk && a.recs && !a.rec_rows && a.uv_mode == TGS_UV_NONE && !a.samples;
     record_raster_event(1, 0, st);
-    raster_bwd_kernel<S, K, TEX><<<grid, bs, stage, st>>>(a);
+    if constexpr (kLeanOk) {
+        if (lean) raster_bwd_kernel<S, K, TEX, true><<<grid, bs, 0, st>>>(a);
+        else raster_bwd_kernel<S, K, TEX, false><<<grid, bs, stage, st>>>(a);
+    } else {
+        raster_bwd_kernel<S, K, TEX, false><<<grid, bs, stage, st>>>(a);
+    }
     record_raster_event(1, 1, st);
 }
 
