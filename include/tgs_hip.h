@@ -262,6 +262,11 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * workspace: device scratch of at least tgs_rasterize_workspace_bytes(dtype,
  * C, N) bytes (no reference counterpart: the reference reads Gaussian
  * attributes in place); it holds packed per-Gaussian records for the call.
+ * When it holds tgs_rasterize_tile_order_bytes(C, tile_width, tile_height)
+ * more bytes, the call also writes there the order in which it hands the tiles
+ * to workgroups (longest list first, which balances the XCDs and shortens the
+ * tail); without that room the tiles go in runs of 16 that take turns over the
+ * XCDs.  The outputs are the same either way.
  * Calls that share one workspace must be ordered on one stream.
  * zero_rows_dst (may be NULL; no reference counterpart): a buffer of N rows of
  * zero_row_bytes bytes (16-byte aligned, a multiple of 16) whose rows with
@@ -275,6 +280,10 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
  * of tgs_rasterize_bwd, the rows with mask 0 are split between forward and
  * backward (disjoint residues) and the gather writes the rows with mask 1.  */
 size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
+/* Bytes of the optional tile-order region after the rasterizer workspace (no
+ * reference counterpart): ceil(C * tile_width * tile_height * 4 / 256) * 256, or 0
+ * when the tiles exceed 32768 (no order is sorted then). */
+size_t tgs_rasterize_tile_order_bytes(int64_t C, int32_t tile_width, int32_t tile_height);
 /* Measurement hook (no reference counterpart): while set, tgs_rasterize_fwd
  * (which = 0) or tgs_rasterize_bwd (which = 1) records `begin` and `end`
  * (hipEvent_t) on its stream right before and right after the rasterizer

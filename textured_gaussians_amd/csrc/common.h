@@ -228,14 +228,17 @@ template <typename T> __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
-// Block-index remap so that consecutive work items land on the same XCD.
-// The dispatcher hands workgroup b to XCD (b mod 8); this returns the work
-// item for b such that each XCD owns one contiguous range of items.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+// Block-index remap in runs: run j of `run` consecutive work items goes to XCD j mod 8,
+// so every XCD takes runs from the whole range (its share of the work follows the
+// average when the work per item varies smoothly over the range, e.g. over an image)
+// while neighbouring items of a run share its L2.  Items past the last whole group of
+// 8 runs keep the dispatcher's order.
+__device__ __forceinline__ uint32_t xcd_runs(uint32_t b, uint32_t nb, uint32_t run) {
     constexpr uint32_t X = 8;
-    const uint32_t q = nb / X, r = nb % X;
+    const uint32_t full = nb / (X * run) * (X * run);
+    if (b >= full) return b;
     const uint32_t x = b % X, k = b / X;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+    return ((k / run) * X + x) * run + k % run;
 }
 
 }  // namespace tgs

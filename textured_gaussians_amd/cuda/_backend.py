@@ -65,6 +65,7 @@ _PROTOS = {
     "tgs_set_texel_marks": (c_int, [vp, c_i64]),
     "tgs_lds_atomic_rates": (c_int, [vp, c_i32, vp]),
     "tgs_rasterize_workspace_bytes": (c_size, [c_int, c_i64, c_i64]),
+    "tgs_rasterize_tile_order_bytes": (c_size, [c_i64, c_i32, c_i32]),
     "tgs_tex_record_bytes": (c_size, [c_int]),
     "tgs_texture_gather_fast": (c_int, [c_int, c_i64, c_i64]),
     "tgs_mipmap_texels": (c_i64, [c_i32]),

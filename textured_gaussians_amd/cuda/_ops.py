@@ -87,6 +87,27 @@ def _stream(t: Tensor):
     return torch.cuda.current_stream(t.device).cuda_stream
 
 
+def wait_event(ev) -> None:
+    """Wait on the host for a recorded event by polling it.  hipEventSynchronize on
+    these (non-blocking) events was measured to return ~400 us after the event
+    completed, and a synchronous copy ~70 us after, with the device idle meanwhile
+    whenever the host has the next launches to issue; polling returns within a few
+    microseconds."""
+    while not ev.query():
+        pass
+
+
+def read_scalar(t: Tensor) -> int:
+    """int(t) of a one-element device tensor: an asynchronous copy to pinned host
+    memory on the current stream, then ``wait_event`` on it."""
+    host = torch.empty(1, dtype=t.dtype, pin_memory=True)
+    host.copy_(t.reshape(1), non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(t.device))
+    wait_event(ev)
+    return int(host[0])
+
+
 def _check_input(t: Tensor, name: str) -> None:
     if not t.is_cuda:
         raise RuntimeError(f"{name} must be a CUDA tensor")
@@ -207,7 +228,7 @@ def fully_fused_projection_packed_fwd_2dgs(means, quats, scales, viewmats, Ks, i
         temp = torch.empty(max(1, L.tgs_isect_count_temp_bytes(C * N)), dtype=torch.uint8, device=dev)
         check(L.tgs_projection_2dgs_packed_count(dt, C, N, *args, _p(flags), _p(incl), _p(temp), temp.numel(), st),
               "fully_fused_projection_packed_fwd_2dgs")
-        nnz = int(incl[-1].item()) if C * N > 0 else 0  # device -> host, as the reference's two-pass kernel
+        nnz = read_scalar(incl[-1]) if C * N > 0 else 0  # device -> host, as the reference's two-pass kernel
         indptr = torch.empty(C + 1, dtype=torch.int64, device=dev)
         camera_ids = torch.empty(nnz, dtype=torch.int64, device=dev)
         gaussian_ids = torch.empty(nnz, dtype=torch.int64, device=dev)
@@ -305,7 +326,7 @@ def isect_tiles(means2d, radii, depths, camera_ids, gaussian_ids, C, tile_size, 
         temp = torch.empty(max(tb, 1), dtype=torch.uint8, device=dev)
         check(L.tgs_isect_tiles_count(C_, N, _p(means2d), _p(radii), tile_size, tile_width, tile_height, _p(tpg),
                                       _p(cum), _p(temp), tb, st), "isect_tiles")
-        n_isects = int(cum[-1].item())  # device -> host (isect_tiles.cu:200-201)
+        n_isects = read_scalar(cum[-1])  # device -> host (isect_tiles.cu:200-201)
         isect_ids = torch.empty(n_isects, dtype=torch.int64, device=dev)
         flatten_ids = torch.empty(n_isects, dtype=torch.int32, device=dev)
         if n_isects == 0:
@@ -421,8 +442,12 @@ def compute_sh_bwd_cams(degrees_to_use, dirs, coeffs, masks, v_colors, compute_v
 
 
 # --------------------------------------------------------------- rasterizer
-def _raster_workspace(dt, C, N, dev):
-    nbytes = lib().tgs_rasterize_workspace_bytes(dt, C, N)
+def _raster_workspace(dt, C, N, dev, tile_offsets):
+    """The rasterizer workspace with room for the tile order after it (the kernels then
+    take the tiles longest list first)."""
+    L = lib()
+    nbytes = L.tgs_rasterize_workspace_bytes(dt, C, N)
+    nbytes += L.tgs_rasterize_tile_order_bytes(tile_offsets.size(0), tile_offsets.size(2), tile_offsets.size(1))
     return torch.empty((nbytes,), dtype=torch.uint8, device=dev)
 
 
@@ -529,7 +554,8 @@ def _raster_fwd(textured, means2d, ray_transforms, colors, opacities, textures, 
         if noalloc:
             ws = None
         else:
-            ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
+            ws = (_raster_workspace(dt, 1, nnz, dev, tile_offsets) if gids is not None
+                  else _raster_workspace(dt, C, N, dev, tile_offsets))
         zd, zm, zb, zs = _zero_rows_args(zero_rows, N, 1)
         entry = lib().tgs_rasterize_fwd_noalloc if noalloc else lib().tgs_rasterize_fwd
         ws_args = () if noalloc else (_p(ws), ws.numel())
@@ -636,7 +662,7 @@ class TexRecordPlan:
         self._temp = temp  # kept until the copy has run
 
     def total(self) -> int:
-        self.done.synchronize()
+        wait_event(self.done)
         return int(self.total_host[0])
 
 
@@ -761,7 +787,8 @@ def _raster_bwd(textured, uv_mode, means2d, ray_transforms, colors, opacities, t
         v_opacities = torch.zeros_like(opacities)
         v_means2d_abs = torch.zeros_like(means2d) if absgrad else None
         v_densify = torch.zeros_like(densify)
-        ws = _raster_workspace(dt, 1, nnz, dev) if gids is not None else _raster_workspace(dt, C, N, dev)
+        ws = (_raster_workspace(dt, 1, nnz, dev, tile_offsets) if gids is not None
+              else _raster_workspace(dt, C, N, dev, tile_offsets))
         zd, zm, zb, zs = _zero_rows_args(zero_rows, N, 0)
         if render_trans is not None:
             _check_input(render_trans, "render_trans")

@@ -26,6 +26,7 @@ from .cuda._wrapper import (
     spherical_harmonics,
     spherical_harmonics_views,
 )
+from .cuda._ops import wait_event
 from .utils import depth_to_normal
 
 
@@ -72,7 +73,7 @@ def _check_inverse(pending) -> None:
     if pending is None:
         return
     info_host, done = pending
-    done.synchronize()
+    wait_event(done)
     if bool((info_host != 0).any()):
         raise torch.linalg.LinAlgError(
             "linalg.inv: a view matrix is singular, the inversion could not be completed")
