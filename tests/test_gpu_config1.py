@@ -35,20 +35,23 @@ def _stages(dtype):
                                            (torch.float32, "none", "pair_samples")])
 def test_config1_full_frame(dtype, uv, path, monkeypatch):
     """Texture-gradient paths: "records" - every texture pair through per-pair records and
-    the row gather, with the unlisted-row fill split between forward and backward;
-    "records_dense_fill" - records, the unlisted rows zero-filled by one launch before the
-    forward; "pair_samples" - records, the backward reading the forward's texture samples
-    instead of the texels; "atomics" - float atomics only; "hybrid" - rows with at least 16 pairs
-    through records (many rows at this size), the others atomics; "dense_fill" - float
-    atomics with the gradient zero-filled by one launch before the forward instead of
-    along the rasterizer walks; "default" - as shipped (records with the streaming gather
-    in float32, float atomics in float64)."""
+    the row gather, the backward reading the texels, with the unlisted-row fill split
+    between forward and backward; "records_dense_fill" - records, the unlisted rows
+    zero-filled by one launch before the forward; "pair_samples" - float atomics with the
+    backward reading the forward's texture samples instead of the texels; "atomics" - float
+    atomics only; "hybrid" - rows with at least 16 pairs through records (many rows at this
+    size), the others atomics; "dense_fill" - float atomics with the gradient zero-filled
+    by one launch before the forward instead of along the rasterizer walks; "default" - as
+    shipped (float32: records with the streaming gather, the backward reading the forward's
+    pair samples under the reference rule; float64: float atomics)."""
     from textured_gaussians_amd.cuda import _wrapper
     if path == "records":
         monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
+        monkeypatch.setattr(_wrapper, "PAIR_SAMPLES", False)
         monkeypatch.setattr(_wrapper, "FWD_FILL_RESIDUES", 0x0F)
     elif path == "pair_samples":
-        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)
+        monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", False)
+        monkeypatch.setattr(_wrapper, "HEAVY_ROW_PAIRS", 0)
         monkeypatch.setattr(_wrapper, "PAIR_SAMPLES", True)
     elif path == "records_dense_fill":
         monkeypatch.setattr(_wrapper, "TEXTURE_GRAD_RECORDS", True)

@@ -855,13 +855,15 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const RasterArgs<S> a) 
 }
 
 // ----------------------------------------------------------------- backward
-// LEAN: the instantiation for the common float32 case, with the texture gradient through
-// per-pair records only, the reference's uv rule (no d(sample)/ds term) and no forward
-// samples.  Dropping the texel values kept for the uv term, the float-atomic staging and
-// the sample reads takes it from 125 to 96 VGPRs at K = 3: five waves per SIMD instead of
-// four (K = 4 stays at four: at five it would spill).
-template <typename S, int K, int TEX, bool LEAN>
-__global__ __launch_bounds__(256, sizeof(S) == 4 ? (LEAN && K <= 3 ? 5 : 4) : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
+// LEAN: the instantiations for the common float32 case, with the texture gradient through
+// per-pair records only and the reference's uv rule (no d(sample)/ds term); LEAN = 1 reads
+// the texels, LEAN = 2 the forward's pair samples (K <= 3).  Dropping the texel values
+// kept for the uv term and the float-atomic staging takes LEAN = 1 from 125 to 96 VGPRs at
+// K = 3: five waves per SIMD instead of four (K = 4 stays at four: at five it would
+// spill).  LEAN = 2 needs 100 and runs at four: at five it spilled 8 VGPRs and measured
+// 0.1 ms slower per backward at 1M Gaussians, 1297x840 (6.86 against 6.77 ms).
+template <typename S, int K, int TEX, int LEAN>
+__global__ __launch_bounds__(256, sizeof(S) == 4 ? (LEAN == 1 && K <= 3 ? 5 : 4) : 1) void raster_bwd_kernel(const RasterArgs<S> a) {
     extern __shared__ __align__(16) char smem[];
     // per-wave staging of texture-gradient updates (texel rows x 64 pixels: 2 rows for
     // bilinear, 4 for mip2, two per level)
@@ -939,8 +941,9 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? (LEAN && K <= 3 ? 5 : 4) : 1)
     const int32_t first = min(end - 1, wave_last);
     const uint64_t *live = a.live ? a.live + (int64_t)(r / kWave) * a.n_isects : nullptr;
     // the forward's samples: only for the reference uv rule (the others need the texels)
-    const S *smp = (!LEAN && a.samples && live && !uv_term) ? a.samples + (int64_t)(r / kWave) * a.n_isects * (kWave * 4)
-                                                   : nullptr;
+    const S *smp = (LEAN == 2 || (!LEAN && a.samples && live && !uv_term))
+                       ? a.samples + (int64_t)(r / kWave) * a.n_isects * (kWave * 4)
+                       : nullptr;
     // per-Gaussian gradient fields in reduce-scatter order: dL/dA (9), colour (K),
     // normal (3), mean2d (2), opacity, |mean2d| (2); this lane adds field `slot`
     constexpr int kGradFields = 17 + K;
@@ -1000,7 +1003,7 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? (LEAN && K <= 3 ? 5 : 4) : 1)
             inr = valid && in_tex_range(p, a);
             bool cached = false;
             if constexpr (K <= 3) {
-                if (smp) {
+                if (LEAN == 2 || smp) {
                     // the sample the forward stored, at the lane's rank among the in-range kept lanes
                     cached = true;
                     const uint64_t ut = __ballot(inr);
@@ -1010,18 +1013,22 @@ __global__ __launch_bounds__(256, sizeof(S) == 4 ? (LEAN && K <= 3 ? 5 : 4) : 1)
                         const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ut >> 32),
                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)ut, 0u));
                         S v4[4];
-                        load_sample(smp + ((int64_t)idx * kWave + pos) * 4, v4);
+                        // entry base uniform (scalar), lane offset 32-bit: a scalar-base load
+                        const S *se = smp + (int64_t)idx * (kWave * 4);
+                        load_sample(se + (uint32_t)pos * 4u, v4);
 #pragma unroll
                         for (int k = 0; k < K; ++k) tex[k] = v4[k];
                         a_s = v4[3];
                     }
                 }
             }
-            if (inr && !cached) {
-                bl = bilerp_setup<S>(p.sx, p.sy, a);
-                bw = bweights(bl);
-                tt = fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl);
-                a_s = bilerp_blend<S, K>(bw, tt, tex);
+            if constexpr (LEAN != 2) {
+                if (inr && !cached) {
+                    bl = bilerp_setup<S>(p.sx, p.sy, a);
+                    bw = bweights(bl);
+                    tt = fetch_texels<S, TEX>(a.textures, (int64_t)G.row * a.tex_stride, bl);
+                    a_s = bilerp_blend<S, K>(bw, tt, tex);
+                }
             }
         }
         const S alpha = pair_alpha(H.op, p.vis, a_s);
@@ -1987,19 +1994,21 @@ template <typename S, int K, int TEX> static void launch_bwd(const RasterArgs<S>
     if (stage > 64 * 1024) {  // mip2 in float64: 72 KB of staging per 256-thread block
         static bool raised = false;
         if (!raised)
-            raised = hipFuncSetAttribute((const void *)raster_bwd_kernel<S, K, TEX, false>,
+            raised = hipFuncSetAttribute((const void *)raster_bwd_kernel<S, K, TEX, 0>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage) == hipSuccess;
     }
-    // the lean instantiation: float32, texture gradient through records for every row,
-    // reference uv rule, no forward samples (it needs no LDS staging)
+    // the lean instantiations: float32, texture gradient through records for every row,
+    // reference uv rule (they need no LDS staging); 1 reads the texels, 2 the forward's
+    // pair samples (K <= 3)
     constexpr bool kLeanOk = sizeof(S) == 4 && (TEX == kTexNative || TEX == kTexBF16);
-    const bool lean = kLeanOk && a.recs && !a.rec_rows && a.uv_mode == TGS_UV_NONE && !a.samples;
+    const bool lean = kLeanOk && a.recs && !a.rec_rows && a.uv_mode == TGS_UV_NONE;
     record_raster_event(1, 0, st);
     if constexpr (kLeanOk) {
-        if (lean) raster_bwd_kernel<S, K, TEX, true><<<grid, bs, 0, st>>>(a);
-        else raster_bwd_kernel<S, K, TEX, false><<<grid, bs, stage, st>>>(a);
+        if (lean && !a.samples) raster_bwd_kernel<S, K, TEX, 1><<<grid, bs, 0, st>>>(a);
+        else if (K <= 3 && lean && a.samples && a.live) raster_bwd_kernel<S, K, TEX, (K <= 3 ? 2 : 1)><<<grid, bs, 0, st>>>(a);
+        else raster_bwd_kernel<S, K, TEX, 0><<<grid, bs, stage, st>>>(a);
     } else {
-        raster_bwd_kernel<S, K, TEX, false><<<grid, bs, stage, st>>>(a);
+        raster_bwd_kernel<S, K, TEX, 0><<<grid, bs, stage, st>>>(a);
     }
     record_raster_event(1, 1, st);
 }

@@ -392,20 +392,23 @@ def _records_fit(textures: Tensor, dtype: torch.dtype) -> bool:
 
 # The forward keeps the texture sample of every composited in-range pair for the
 # backward (16 bytes per pair, in slots of 1 KB per wavefront and list entry: 17 GB at
-# 1M Gaussians, 1297x840), which then reads no texels: True (when it fits), False.
-# Measured at that size (one MI355X): the forward +0.3 ms (1.1 GB of samples), the
-# backward -0.4 ms without its fused row fill and about even with it (the walk then
-# shares the memory bandwidth with 18.4 GB of fill stores), so it stays off.
-PAIR_SAMPLES = False
+# 1M Gaussians, 1297x840), which then reads no texels.  None (default): when the
+# backward will run with texture-gradient records (its lean sample-reading
+# instantiation) and the slots fit; True: whenever they apply and fit; False: never.
+# Measured at that size (one MI355X, one process): forward +0.1 ms (1.1 GB of samples),
+# backward with records -0.4 ms (7.18 -> 6.77 ms), 17.43 -> 17.12 ms per iteration.
+PAIR_SAMPLES = None
 # at most this fraction of the device memory free when the forward runs
 PAIR_SAMPLES_MEMORY_FRACTION = 0.25
 
 
-def _use_pair_samples(bwd_fn, means2d, colors, textures, flatten_ids, tile_size) -> bool:
+def _use_pair_samples(bwd_fn, means2d, colors, textures, flatten_ids, tile_size, records: bool) -> bool:
     """Pair samples apply to the reference ``bilinear`` rule (the other rules need the
     texels' differences) in float32 with at most 3 channels, when the slots fit in a
-    quarter of the free device memory."""
-    if not PAIR_SAMPLES or bwd_fn != tgs_fns["bilinear"][1] or means2d.dtype != torch.float32:
+    quarter of the free device memory (see PAIR_SAMPLES for when they are taken)."""
+    if PAIR_SAMPLES is False or (PAIR_SAMPLES is None and not records):
+        return False
+    if bwd_fn != tgs_fns["bilinear"][1] or means2d.dtype != torch.float32:
         return False
     if colors.shape[-1] > 3 or textures.dim() != 4 or flatten_ids.numel() == 0:
         return False
@@ -480,7 +483,9 @@ class _RasterizeToPixelsTexturedGaussians(torch.autograd.Function):
         # the forward's texture samples instead of the texels
         ctx.live = _C.live_entries_buffer(flatten_ids, tile_size) if means2d.is_cuda else None
         ctx.samples = None
-        if ctx.live is not None and _use_pair_samples(bwd_fn, means2d, colors, textures, flatten_ids, tile_size):
+        records = ctx.tex_counts is not None and not ctx.record_rows
+        if (ctx.live is not None and any(ctx.needs_input_grad)
+                and _use_pair_samples(bwd_fn, means2d, colors, textures, flatten_ids, tile_size, records)):
             ctx.samples = _C.pair_samples_buffer(flatten_ids, tile_size)
         (render_colors, render_alphas, render_normals, render_distort, render_median, last_ids, median_ids,
          gs_contrib_sum, gs_contrib_count, accum_depth, trans) = _make_lazy_cuda_func(fwd_fn + "_full")(
