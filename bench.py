@@ -516,6 +516,8 @@ def main():
     achieved_bwd = ab["bytes_bwd"] / (t_bwd * 1e-3) / 1e9
     from textured_gaussians_amd.cuda import _wrapper
     records = _wrapper._use_records(params["textures"], torch.float32)
+    # the backward reads the forward's pair samples (the lean instantiation 2) on the records path
+    samples = records and _wrapper.PAIR_SAMPLES is not False
     listed_rows = int(torch.unique(out[-1]["flatten_ids"].long() % args.n).numel())
     meta_ref = {k: out[-1][k].detach() for k in ("means2d", "ray_transforms", "normals")}
     del out
@@ -564,7 +566,8 @@ def main():
                    "n_isects": ab["I"], "visible": ab["V"], "texels_read": ab["U"], "texels_with_gradient": ab["U_g"]},
         "mpix_per_s": round(ab["P"] / (ms * 1e-3) / 1e6, 2),
         "raster_ms": {"fwd": round(t_fwd, 3), "bwd": round(t_bwd - t_gat, 3), "gather": round(t_gat, 3)},
-        "texture_gradient": ("per-pair records + row gather (float64 LDS sums)" if records
+        "texture_gradient": (("per-pair records + row gather (float64 LDS sums)"
+                              + (", backward reading the forward's pair samples" if samples else "")) if records
                              else "float atomics into the row-filled gradient"),
         "raster_hbm_frac_fwd_bwd": round(achieved_pair / HBM_PEAK_GBS, 4),
         "plain_2dgs": plain,
@@ -580,8 +583,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved_pair, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_pair / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "frac_of_stream_peak": round(achieved_pair / practical, 4), "stream_peak_GBs": practical,
-                     "kernel": ("raster_fwd_kernel<float,3,1> + raster_bwd_kernel<float,3,1>"
-                                + (" + tex_gather2_kernel" if records else "")),
+                     "kernel": ("raster_fwd_kernel<float,3,1> + "
+                                + ("raster_bwd_kernel<float,3,1,2>" if samples else
+                                   "raster_bwd_kernel<float,3,1,1>" if records else "raster_bwd_kernel<float,3,1,0>")
+                                + (" + tex_gather2_kernel<4,6>" if records else "")),
                      "algorithmic_bytes": B, "bytes_fwd": ab["bytes_fwd"], "bytes_bwd": ab["bytes_bwd"],
                      "time_ms": round(t_fwd + t_bwd, 3)},
         "raster_bwd_only": {"achieved": round(achieved_bwd, 1), "frac": round(achieved_bwd / HBM_PEAK_GBS, 4)},

@@ -282,7 +282,7 @@ int tgs_sh_bwd_cams(int dtype, int64_t C, int64_t n, int32_t K, int32_t degree, 
 size_t tgs_rasterize_workspace_bytes(int dtype, int64_t C, int64_t N);
 /* Bytes of the optional tile-order region after the rasterizer workspace (no
  * reference counterpart): ceil(C * tile_width * tile_height * 4 / 256) * 256, or 0
- * when the tiles exceed 32768 (no order is sorted then). */
+ * when the tiles exceed 262144 (no order is sorted then). */
 size_t tgs_rasterize_tile_order_bytes(int64_t C, int32_t tile_width, int32_t tile_height);
 /* Measurement hook (no reference counterpart): while set, tgs_rasterize_fwd
  * (which = 0) or tgs_rasterize_bwd (which = 1) records `begin` and `end`
