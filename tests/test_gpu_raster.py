@@ -217,6 +217,29 @@ def test_raster_bwd_textured_f32(uv):
     _bwd_case(torch.float32, uv)
 
 
+@pytest.mark.parametrize("uv,textured,K,use_bg", [("none", True, 3, True), ("double", True, 3, True),
+                                                  ("exact", True, 3, True), ("none", True, 2, False),
+                                                  ("none", False, 3, True), ("exact", False, 4, True)])
+def test_raster_bwd_f32_per_element(uv, textured, K, use_bg):
+    """The float32 scenes of the tests above, element by element over every tile of the
+    frame instead of in the L2 norm: every gradient element within 1e-3 relative (with a
+    floor of 1e-4 of its row's largest element, plus three times the float32 oracle's own
+    error on it), the rows whose pairs sit on float32 decision thresholds (1/255, the
+    texture-range edge, the 1e-4 stop, the median) at the looser bound stated in gpu_util,
+    the background gradient too when there is a background.  K = 1 stays with the float64
+    kernels (test_raster_bwd_channels_nobg): its colour is the depth channel itself, a
+    one-element row whose value can cancel to a few parts in 1e4 of its distortion terms,
+    which the backward rebuilds by subtraction from the forward's totals in float32 (seen:
+    3.140e-5 against 3.162e-5, the float32 oracle 3.164e-5), so the row floor is no floor."""
+    from gpu_util import all_tiles, raster_tile_parity
+    sc, st, colors, opac, bg = _inputs(torch.float32, K=K)
+    dv = lambda t: t.to(DEV, torch.float32).contiguous()  # noqa: E731
+    tex = dv(sc.textures) if textured else None
+    stats = raster_tile_parity(st, dv(colors), dv(opac), tex, dv(bg) if use_bg else None, torch.float32,
+                               all_tiles(st), uv=uv)
+    print("per element", uv, textured, K, use_bg, stats)
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("rule", ["none", "exact"])
 def test_raster_bwd_plain(dtype, rule):
