@@ -275,7 +275,9 @@ __global__ __launch_bounds__(256) void projection_bwd_kernel(
         quat_rot_vjp(q, vRq, vq);
         const T vs0 = vRS0.x * Rq.m[0][0] + vRS0.y * Rq.m[1][0] + vRS0.z * Rq.m[2][0];
         const T vs1 = vRS1.x * Rq.m[0][1] + vRS1.y * Rq.m[1][1] + vRS1.z * Rq.m[2][1];
-        if (sparse) {
+        // one camera (or sparse outputs): each Gaussian has at most one entry, so its
+        // gradients are stored, not added (the caller zero-fills the outputs)
+        if (sparse || C == 1) {
             v_means[og * 3] = vmu.x;
             v_means[og * 3 + 1] = vmu.y;
             v_means[og * 3 + 2] = vmu.z;

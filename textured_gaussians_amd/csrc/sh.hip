@@ -134,9 +134,10 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(int64_t n, int K, int degre
     if (want_dir) {
         // through u = d / |d|: dL/dd = (g - (g.u) u) / |d|
         const T gd = gu * x + gv * y + gw * z;
-        atomic_add(v_dirs + i * 3, (gu - gd * x) * inv);
-        atomic_add(v_dirs + i * 3 + 1, (gv - gd * y) * inv);
-        atomic_add(v_dirs + i * 3 + 2, (gw - gd * z) * inv);
+        // one lane per direction: stored (the caller zero-fills v_dirs)
+        v_dirs[i * 3] = (gu - gd * x) * inv;
+        v_dirs[i * 3 + 1] = (gv - gd * y) * inv;
+        v_dirs[i * 3 + 2] = (gw - gd * z) * inv;
     }
 }
 
@@ -242,9 +243,9 @@ __global__ __launch_bounds__(256) void sh_bwd_staged_kernel(int64_t n, int K, co
             }
             if (want_dir) {
                 const float gd = gu * x + gv * y + gw * z;
-                atomic_add(v_dirs + i * 3, (gu - gd * x) * inv);
-                atomic_add(v_dirs + i * 3 + 1, (gv - gd * y) * inv);
-                atomic_add(v_dirs + i * 3 + 2, (gw - gd * z) * inv);
+                v_dirs[i * 3] = (gu - gd * x) * inv;  // one lane per direction
+                v_dirs[i * 3 + 1] = (gv - gd * y) * inv;
+                v_dirs[i * 3 + 2] = (gw - gd * z) * inv;
             }
         }
     }
