@@ -294,16 +294,16 @@ def cpu_baseline(sc, meta_gpu, n_tiles_sample, runs=3):
 
 def grad_paths_leg(tg, sc, params, steps, warmup):
     """The headline iteration under each texture-gradient variant of the autograd layer
-    (all give the same gradients): per-pair records summed per row by the gather kernel
-    (the product path: the backward walk zero-fills the unlisted rows), the same with the
-    backward reading the forward's pair samples instead of the texels, the same with the
-    forward walk filling the unlisted rows, float atomics into the row-filled gradient, the same with
-    one dense fill, and records for the rows with at least 64 pairs only (atomics for the
-    rest).  The records variants run the rasterizer backward without texture-gradient
-    atomics; the gather runs after it, inside the backward op."""
+    (all give the same gradients): per-pair records summed per row by the gather kernel,
+    the backward reading the forward's pair samples (the product path: the backward walk
+    zero-fills the unlisted rows), the same with the backward reading the texels, the same
+    with the forward walk filling the unlisted rows, float atomics into the row-filled
+    gradient, the same with one dense fill, and records for the rows with at least 64 pairs
+    only (atomics for the rest).  The records variants run the rasterizer backward without
+    texture-gradient atomics; the gather runs after it, inside the backward op."""
     from textured_gaussians_amd.cuda import _wrapper
     variants = [("records_gather", {}),
-                ("records_pair_samples", {"PAIR_SAMPLES": True}),
+                ("records_texels", {"PAIR_SAMPLES": False}),
                 ("records_forward_fills_unlisted", {"TEXTURE_GRAD_RECORDS": True, "FWD_FILL_RESIDUES": 0xFF}),
                 ("atomics_fused_fill", {"TEXTURE_GRAD_RECORDS": False}),
                 ("atomics_dense_fill", {"TEXTURE_GRAD_RECORDS": False, "FUSED_FILL": False}),
