@@ -1593,8 +1593,11 @@ __global__ __launch_bounds__(512, 4) void tex_gather_kernel(int32_t tex_w, int64
 //    XOR-ed by g2_swz(t), so the atomics of one channel over random texels reach every
 //    bank (unswizzled, channel c of every texel falls in the same few banks).  A writer
 //    lane reads a texel as two 16-byte pieces and undoes the permutation in registers.
-constexpr int kG2Adders = 4;
-constexpr int kG2Writers = 8;  // 6 measured 1% slower (5.63 against 5.57 ms at 1M x 50x50), 3 adders slower still
+// 6 + 6 waves, measured in one process against the other splits of 12 (1M Gaussians:
+// 50x50 rows 5.38 ms against 5.56 for 4 + 8 and 6.7 for 8 + 4; 32x32 rows 4.57 against
+// 4.79; 8 cameras x 500k, 32x32: 7.5 against 8.1) and against 4 + 6 (5.63 ms)
+constexpr int kG2Adders = 6;
+constexpr int kG2Writers = 6;
 constexpr int kG2Pre = 2;     // record chunks per adder and row loaded two steps ahead
 constexpr int kG2Block = 16;  // list entries per register block
 
