@@ -586,7 +586,7 @@ def main():
                      "kernel": ("raster_fwd_kernel<float,3,1> + "
                                 + ("raster_bwd_kernel<float,3,1,2>" if samples else
                                    "raster_bwd_kernel<float,3,1,1>" if records else "raster_bwd_kernel<float,3,1,0>")
-                                + (" + tex_gather2_kernel<6,6>" if records else "")),
+                                + (" + tex_gather2_kernel<6,10>" if records else "")),
                      "algorithmic_bytes": B, "bytes_fwd": ab["bytes_fwd"], "bytes_bwd": ab["bytes_bwd"],
                      "time_ms": round(t_fwd + t_bwd, 3)},
         "raster_bwd_only": {"achieved": round(achieved_bwd, 1), "frac": round(achieved_bwd / HBM_PEAK_GBS, 4)},

@@ -1593,11 +1593,13 @@ __global__ __launch_bounds__(512, 4) void tex_gather_kernel(int32_t tex_w, int64
 //    XOR-ed by g2_swz(t), so the atomics of one channel over random texels reach every
 //    bank (unswizzled, channel c of every texel falls in the same few banks).  A writer
 //    lane reads a texel as two 16-byte pieces and undoes the permutation in registers.
-// 6 + 6 waves, measured in one process against the other splits of 12 (1M Gaussians:
-// 50x50 rows 5.38 ms against 5.56 for 4 + 8 and 6.7 for 8 + 4; 32x32 rows 4.57 against
-// 4.79; 8 cameras x 500k, 32x32: 7.5 against 8.1) and against 4 + 6 (5.63 ms)
+// 6 + 10 waves (16: the largest workgroup; its lanes fit in 120 registers), measured in one
+// process against other splits: 1M Gaussians, 50x50 rows 5.08-5.10 ms against 5.35-5.50
+// for 6 + 6, 5.08-5.12 for 5 + 11, 5.18-5.20 for 4 + 12, 5.21-5.28 for 7 + 9 and 5 + 9,
+// 5.75-5.88 for 6 + 8 and 8 + 8; 8 cameras x 500k, 32x32 rows 6.9-7.2 ms, as for 6 + 6.
+// Of 12 waves, 6 + 6 had measured best (4 + 8 5.56 ms, 8 + 4 6.7 ms).
 constexpr int kG2Adders = 6;
-constexpr int kG2Writers = 6;
+constexpr int kG2Writers = 10;
 constexpr int kG2Pre = 2;     // record chunks per adder and row loaded two steps ahead
 constexpr int kG2Block = 16;  // list entries per register block
 
