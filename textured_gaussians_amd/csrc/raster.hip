@@ -1761,10 +1761,12 @@ __global__ __launch_bounds__((AD + WR) * kWave) void tex_gather2_kernel(int32_t 
                 const int32_t src = (buf ^ 1) * texels * 2;  // f64x2 index of the buffer
                 uint32_t *mb = masks + (buf ^ 1) * mw;
                 u32x4_t *dst = (u32x4_t *)(g_tex + (int64_t)row * texels * 4);
-                // kB = 4 texels per lane and batch, measured in one process against 2, 3, 5, 7
-                // and 8: 32x32 rows (8 cameras x 500k) 6.9-7.2 ms against 7.4-7.7 for 8, 50x50
-                // rows (1M) 5.38-5.53 ms against 5.43-5.51 for 8; fewer registers (122 against 162)
-                constexpr int kStride = WR * kWave, kB = 4;
+                // kB = 2 texels per lane and batch, measured in one process: with 6 + 10 waves,
+                // 50x50 rows (1M) 5.04-5.09 ms against 5.12-5.13 for 4 and 5.19-5.23 with one
+                // prefetched record chunk instead of two; 32x32 rows (8 cameras x 500k) 6.6-7.1
+                // against 7.05-7.13.  With 6 + 6 waves, 4 had beaten 8 (32x32 rows 6.9-7.2 ms against
+                // 7.4-7.7, and 122 registers against 162, which let the workgroup grow to 16 waves)
+                constexpr int kStride = WR * kWave, kB = 2;
                 const int wl = (wave - AD) * kWave + lane;  // lane among the writers
                 for (int32_t base = wl; base < texels; base += kB * kStride) {
                     // a batch of kB texels per lane: mask words, then sums (a marked texel's
